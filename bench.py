@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Throughput of the batched 2D FDTD hot path on one MI355X.
+
+Workload (one bench "step" = one full simulation of one batch): 2D TMz,
+600 x 600 cells, dx = 0.5 mm, Courant 0.5, 15-cell CPML (the stepped box is the
+571 x 571 cavity the reference's inert CPML encloses), 16 TX = 16 RX on a
+200-cell ring, 800 time steps, single-pole Debye head phantom (seed 0), then
+the running DFT of all 256 traces at 301 frequencies over 0.5-3 GHz into the
+(301, 16, 16) S-matrix.  Inputs are resident on the device before timing.
+
+Prints one JSON line (see DESIGN.md for the byte model behind `roofline`).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "ceep-v1_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+NX = NY = 600
+DX = 0.5e-3
+N_CPML = 15
+T = 800
+B = 16
+COURANT = 0.5
+FREQS = np.linspace(0.5e9, 3e9, 301)
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell-step: ez, hx, hy, J read + written once (32 B per
+# TX) and the shared ca, cb, k1, k2 maps read once per launch (16 B per cell)
+BYTES_PER_CELL_PER_TX = 32
+BYTES_PER_CELL_SHARED = 16
+KERNEL = "step2d_kernel<1, 4>"   # Debye pole, 4 TX planes per thread
+
+
+def _workload():
+    import ceep_amd
+    from ceep_amd.constants import cfl_timestep_2d
+    from ceep_amd.waveforms import modulated_gaussian
+
+    dt = cfl_timestep_2d(DX, DX, COURANT)
+    pos = ceep_amd.phantoms.ring_positions((300, 300), 200, B)
+    w = modulated_gaussian(T, dt, 2e9, 1.5e9)
+    ph = ceep_amd.phantoms.DebyeHeadPhantom2D(NX, NY, 0)
+    s = ceep_amd.BatchedFDTD2D(nx=NX, ny=NY, dx=DX, total_steps=T, cpml_thickness=N_CPML,
+                               source_positions=pos, probe_positions=pos, frequency=2e9,
+                               courant=COURANT, waveform=w, cpml_R=1e-6)
+    s.set_phantom(ph)
+    s._build()
+    return s, ph, pos, w, dt
+
+
+def _device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+            return "torch.cuda.synchronize"
+    except Exception:
+        pass
+    return "library-synchronous"
+
+
+def _cpu_baseline(ph, pos, w, dt, steps):
+    """The oracle's literal FP64 NumPy restatement with the reference's per-row
+    CPML loops (its CPU cost profile), single core, first `steps` steps."""
+    from oracle import ade, fdtd2d
+
+    m = ph.get_debye_maps()
+    poles = [ade.debye_pole(m["delta_eps"], m["tau"], dt)]
+    fdtd2d.run_literal(NX, NY, DX, dt, N_CPML, pos, pos, w[:2], m["eps_inf"], m["sigma"],
+                       poles=poles, ref_cost=True)       # warm-up
+    t0 = time.perf_counter()
+    fdtd2d.run_literal(NX, NY, DX, dt, N_CPML, pos, pos, w[:steps], m["eps_inf"], m["sigma"],
+                       poles=poles, ref_cost=True)
+    el = time.perf_counter() - t0
+    return {
+        "value": NX * NY * B * steps / el / 1e9,
+        "unit": "GCell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"first {steps} of {T} steps of this workload (full 600x600 grid, B=16, "
+                   f"Debye) through the oracle's FP64 NumPy restatement with the reference's "
+                   f"per-row CPML loops, one host core, {el:.1f} s; host "
+                   f"{platform.processor() or platform.machine()}, os.cpu_count()={os.cpu_count()}"),
+    }
+
+
+def _traffic():
+    path = os.path.join(ROOT, "profiles", "step2d_debye_600x600x16_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
+              "the batched FDTD path shards TX across GPUs with no exchange and is not "
+              "launched on more than one GPU here.", file=sys.stderr)
+        sys.exit(2)
+
+    s, ph, pos, w, dt = _workload()
+    h = s._handle
+    box = h.box_info()
+    cells = int(box[0] * box[1])
+
+    def step():
+        h.reset()
+        h.run(copy=False)
+        return h.dft(FREQS)
+
+    for _ in range(args.warmup):
+        step()
+    sync = _device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        S = step()
+    _device_sync()
+    el = time.perf_counter() - t0
+    ms_per_step = el / args.steps * 1e3
+    value = cells * B * T / (el / args.steps) / 1e9
+
+    # kernel-only timing with HIP events on the library's stream
+    ms_run = h.time_runs(3)
+    ms_launch = ms_run / T
+    bytes_launch = cells * (B * BYTES_PER_CELL_PER_TX + BYTES_PER_CELL_SHARED)
+    achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = _cpu_baseline(ph, pos, w, dt, args.cpu_steps)
+
+    line = {
+        "metric": "GCell-steps/s (600²×16TX, 3D 200³); %HBM peak; fwd+adjoint / fwd time ratio",
+        "value": round(value, 3),
+        "unit": "GCell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Debye head phantom)",
+        "config": {
+            "workload": ("2D TMz 600x600 dx=0.5mm Courant 0.5, 15-cell CPML (stepped box "
+                         f"{box[0]}x{box[1]}), 16 TX/RX ring r=200, 800 steps, 1-pole Debye head "
+                         "seed 0, 301-frequency DFT 0.5-3 GHz to the 16x16 S-matrix"),
+            "cells_counted": "stepped box",
+            "value_full_grid_convention": round(NX * NY * B * T / (el / args.steps) / 1e9, 3),
+            "batch": B, "time_steps": T, "sync": sync,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": KERNEL,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": _traffic(),
+            "algorithmic_bytes_per_launch": bytes_launch,
+            "launch_ms": round(ms_launch, 5),
+        },
+        "fwd_adjoint_ratio": None,
+        "cpu_baseline": cpu,
+        "s_matrix_shape": list(S.shape),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,223 @@
+"""ctypes binding of libceep_fdtd.so (the C ABI of include/ceep_fdtd.h).
+
+The library is built in-tree by ``ceep-v1_amd/build.py``.  There is no CPU
+fallback: if the shared library is missing or no HIP device is usable, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libceep_fdtd.so")
+
+CEEP_OK = 0
+CEEP_EINVAL = 1
+CEEP_EHIP = 2
+CEEP_ENOMEM = 3
+CEEP_ESTATE = 4
+CEEP_EUNSUPPORTED = 5
+
+BOX_REFERENCE = 0
+BOX_FULL = 1
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (restype, argtypes); every symbol include/ceep_fdtd.h declares
+SIGNATURES = {
+    "ceep_fdtd_last_error": (ctypes.c_char_p, []),
+    "ceep_fdtd_abi_version": (_I, []),
+    "ceep_fdtd_create": (_I, [_I, _PI64, _I, _I, _I, _I, _I, _D, _D, ctypes.POINTER(_P)]),
+    "ceep_fdtd_destroy": (_I, [_P]),
+    "ceep_fdtd_set_materials": (_I, [_P, _PD, _PD, _PD, _PD]),
+    "ceep_fdtd_set_debye": (_I, [_P, _PD, _PD]),
+    "ceep_fdtd_set_lorentz": (_I, [_P, _PD, _PD, _PD]),
+    "ceep_fdtd_clear_poles": (_I, [_P]),
+    "ceep_fdtd_set_sources": (_I, [_P, _PI64, _PD, _I]),
+    "ceep_fdtd_set_probes": (_I, [_P, _PI64]),
+    "ceep_fdtd_reset": (_I, [_P]),
+    "ceep_fdtd_run": (_I, [_P, _PD]),
+    "ceep_fdtd_get_traces": (_I, [_P, _PD]),
+    "ceep_fdtd_dft": (_I, [_P, _PD, _I, _PD]),
+    "ceep_fdtd_get_field": (_I, [_P, _I, _PD]),
+    "ceep_fdtd_energy": (_I, [_P, _PD]),
+    "ceep_fdtd_time_runs": (_I, [_P, _I, _PD]),
+    "ceep_fdtd_set_tuning": (_I, [_P, _I, _I]),
+    "ceep_fdtd_box_info": (_I, [_P, _PI64]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    """A HIP/runtime failure reported by libceep_fdtd."""
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load the library once; raise ImportError if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(
+            f"libceep_fdtd.so not found at {_LIB_PATH}; build it with "
+            "`python ceep-v1_amd/build.py` (there is no CPU fallback)")
+    lib = ctypes.CDLL(_LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc == CEEP_OK:
+        return
+    msg = (load().ceep_fdtd_last_error() or b"").decode(errors="replace")
+    if rc == CEEP_EINVAL:
+        raise ValueError(msg)
+    if rc == CEEP_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == CEEP_ESTATE:
+        raise RuntimeError(msg)
+    if rc == CEEP_ENOMEM:
+        raise MemoryError(msg)
+    raise NativeError(msg)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_PD)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_PI64)
+
+
+class Handle:
+    """Owns one ``ceep_fdtd*``; methods map 1:1 onto the C ABI."""
+
+    def __init__(self, ndim, shape, cpml_n, box_mode, batch, n_probes, total_steps, dx, dt):
+        self._lib = load()
+        self._h = _P()
+        shp = np.ascontiguousarray(shape, dtype=np.int64)
+        check(self._lib.ceep_fdtd_create(ndim, iptr(shp), int(cpml_n), int(box_mode),
+                                         int(batch), int(n_probes), int(total_steps),
+                                         float(dx), float(dt), ctypes.byref(self._h)))
+        self.ndim = ndim
+        self.shape = tuple(int(s) for s in shape)
+        self.batch = batch
+        self.n_probes = n_probes
+        self.total_steps = total_steps
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.ceep_fdtd_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _f64(a, shape):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shape))
+
+    def set_materials(self, eps_r, sigma_e, mu_r=None, sigma_m=None):
+        e = self._f64(eps_r, self.shape)
+        s = self._f64(sigma_e, self.shape)
+        m = self._f64(mu_r, self.shape) if mu_r is not None else None
+        sm = self._f64(sigma_m, self.shape) if sigma_m is not None else None
+        check(self._lib.ceep_fdtd_set_materials(self._h, dptr(e), dptr(s),
+                                                dptr(m) if m is not None else None,
+                                                dptr(sm) if sm is not None else None))
+
+    def set_debye(self, delta_eps, tau):
+        de, ta = self._f64(delta_eps, self.shape), self._f64(tau, self.shape)
+        check(self._lib.ceep_fdtd_set_debye(self._h, dptr(de), dptr(ta)))
+
+    def set_lorentz(self, delta_eps, omega_0, delta):
+        de = self._f64(delta_eps, self.shape)
+        w0 = self._f64(omega_0, self.shape)
+        dl = self._f64(delta, self.shape)
+        check(self._lib.ceep_fdtd_set_lorentz(self._h, dptr(de), dptr(w0), dptr(dl)))
+
+    def clear_poles(self):
+        check(self._lib.ceep_fdtd_clear_poles(self._h))
+
+    def set_sources(self, positions, waveform):
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.int64).reshape(self.batch, self.ndim))
+        w = np.ascontiguousarray(np.asarray(waveform, dtype=np.float64))
+        if w.ndim == 1:
+            if w.shape[0] != self.total_steps:
+                raise ValueError(f"waveform needs {self.total_steps} samples, got {w.shape[0]}")
+            per_batch = 0
+        elif w.shape == (self.batch, self.total_steps):
+            per_batch = 1
+        else:
+            raise ValueError(f"waveform must be ({self.total_steps},) or "
+                             f"({self.batch}, {self.total_steps}), got {w.shape}")
+        check(self._lib.ceep_fdtd_set_sources(self._h, iptr(pos), dptr(w), per_batch))
+
+    def set_probes(self, positions):
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.int64).reshape(self.n_probes, self.ndim))
+        check(self._lib.ceep_fdtd_set_probes(self._h, iptr(pos) if pos.size else None))
+
+    def reset(self):
+        check(self._lib.ceep_fdtd_reset(self._h))
+
+    def run(self, copy=True):
+        if not copy:
+            check(self._lib.ceep_fdtd_run(self._h, None))
+            return None
+        out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)
+        check(self._lib.ceep_fdtd_run(self._h, dptr(out)))
+        return out
+
+    def traces(self):
+        out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)
+        check(self._lib.ceep_fdtd_get_traces(self._h, dptr(out)))
+        return out
+
+    def dft(self, freqs):
+        f = np.ascontiguousarray(np.atleast_1d(np.asarray(freqs, dtype=np.float64)))
+        out = np.zeros((f.size, self.n_probes, self.batch, 2), dtype=np.float64)
+        check(self._lib.ceep_fdtd_dft(self._h, dptr(f), int(f.size), dptr(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def field(self, comp):
+        out = np.zeros((self.batch,) + self.shape, dtype=np.float64)
+        check(self._lib.ceep_fdtd_get_field(self._h, int(comp), dptr(out)))
+        return out
+
+    def energy(self):
+        v = ctypes.c_double()
+        check(self._lib.ceep_fdtd_energy(self._h, ctypes.byref(v)))
+        return v.value
+
+    def time_runs(self, reps):
+        v = ctypes.c_double()
+        check(self._lib.ceep_fdtd_time_runs(self._h, int(reps), ctypes.byref(v)))
+        return v.value
+
+    def set_tuning(self, key, value):
+        check(self._lib.ceep_fdtd_set_tuning(self._h, int(key), int(value)))
+
+    def box_info(self):
+        out = np.zeros(5, dtype=np.int64)
+        check(self._lib.ceep_fdtd_box_info(self._h, iptr(out)))
+        return out

@@ -1,0 +1,14 @@
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ceep {
+
+// traces [B][P][T]; tw [T][F] = (cos(w_f s dt), -sin(w_f s dt)); S [F][P][B].
+hipError_t launch_dft(const float* traces, const float2* tw, int B, int P, int T, int F,
+                      float dt, float2* S, hipStream_t s);
+
+// partial[nblocks] = per-workgroup sums of x[i]^2 over n floats.
+hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,
+                        hipStream_t s);
+
+}  // namespace ceep

@@ -1,0 +1,74 @@
+// Trace -> S-matrix DFT and field-energy reduction (gfx950).
+//
+// DFT: S[f][rx][tx] = dt * sum_s trace[tx][rx][s] * exp(-i w_f s dt), the sum of
+// DFTMonitor.update over the run (src/ceep/solvers/dft.py:75-90 with the
+// pre-increment step index of fdtd_2d.py:287-289), laid out as
+// MultistaticSParameters.s_matrix (s_params.py:70-99).  One thread owns one
+// (f, rx, tx) output and keeps its two Kahan-compensated FP32 accumulators in
+// registers; consecutive lanes take consecutive frequencies, so the twiddle
+// table [T][F] is read coalesced and the trace sample is a wave-uniform load.
+#include <hip/hip_runtime.h>
+#include "reduce_kernels.h"
+
+namespace ceep {
+
+__global__ __launch_bounds__(256) void dft_kernel(const float* __restrict__ traces,
+                                                  const float2* __restrict__ tw, int B,
+                                                  int P, int T, int F, float dt,
+                                                  float2* __restrict__ S) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pair = blockIdx.y;            // tx * P + rx
+  if (f >= F) return;
+  const int tx = pair / P, rx = pair % P;
+  const float* __restrict__ tr = traces + (long long)pair * T;
+  float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
+  for (int s = 0; s < T; ++s) {
+    const float v = tr[s];
+    const float2 w = tw[(long long)s * F + f];
+    // Kahan: the product is formed first, then compensated accumulation
+    const float yr = __fmul_rn(w.x, v) - cr;
+    const float tr_ = sr + yr;
+    cr = (tr_ - sr) - yr;
+    sr = tr_;
+    const float yi = __fmul_rn(w.y, v) - ci;
+    const float ti = si + yi;
+    ci = (ti - si) - yi;
+    si = ti;
+  }
+  S[((long long)f * P + rx) * B + tx] = make_float2(sr * dt, si * dt);
+}
+
+hipError_t launch_dft(const float* traces, const float2* tw, int B, int P, int T, int F,
+                      float dt, float2* S, hipStream_t s) {
+  dim3 blk(64, 1, 1);
+  dim3 grd((F + 63) / 64, B * P, 1);
+  hipLaunchKernelGGL(dft_kernel, grd, blk, 0, s, traces, tw, B, P, T, F, dt, S);
+  return hipGetLastError();
+}
+
+// Sum of squares of `n` floats; one partial per workgroup (host sums them in FP64).
+__global__ __launch_bounds__(256) void sumsq_kernel(const float* __restrict__ x, long long n,
+                                                    float* __restrict__ partial) {
+  __shared__ float red[256];
+  float acc = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float v = x[i];
+    acc += v * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(sumsq_kernel, dim3(nblocks), dim3(256), 0, s, x, n, partial);
+  return hipGetLastError();
+}
+
+}  // namespace ceep

@@ -1,0 +1,723 @@
+// Host side of libceep_fdtd.so: the C ABI declared in include/ceep_fdtd.h.
+//
+// A handle holds the batched solver's device state for one geometry: field
+// buffers over the stepped box, FP32 coefficient maps, ADE currents, source
+// and probe tables and the (batch, probes, steps) trace buffer.  The time loop
+// is a sequence of kernel launches on the handle's stream, captured once as a
+// hipGraph and replayed for every run.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ceep_fdtd.h"
+#include "fdtd_common.h"
+#include "fdtd2d_kernels.h"
+#include "fdtd3d_kernels.h"
+#include "reduce_kernels.h"
+
+namespace {
+
+constexpr double kEps0 = 8.854187817e-12;            // core/constants.py:24
+constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1e-7;  // core/constants.py:27
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define CEEP_HIP(call)                                                               \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(CEEP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return CEEP_OK;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(CEEP_ENOMEM, std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                                 " B): " + hipGetErrorString(e));
+  }
+  return CEEP_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct ceep_fdtd {
+  int ndim = 2, box_mode = 0, n = 0, B = 0, P = 0, T = 0;
+  int64_t N[3] = {1, 1, 1};
+  double dx = 0, dt = 0;
+  int org = 0;                 // grid index of box cell 0 (same on every axis)
+  int M[3] = {1, 1, 1};        // box size per axis
+  int pitch = 0;               // floats per innermost row
+  long long row_block = 0;     // 3D: floats per x-slab (M[1] * pitch)
+  long long plane = 0;         // floats per batch plane/volume
+  long long full_cells = 0;    // cells of the full grid
+
+  // host-side inputs
+  std::vector<double> eps_r, sigma_e, mu_r, sigma_m;
+  bool magnetic = false;
+  std::vector<double> deb_de, deb_tau, lor_de, lor_w0, lor_dl;
+  bool has_debye = false, has_lorentz = false;
+  std::vector<int64_t> src, prb;
+  bool have_sources = false, have_probes = false;
+
+  // device state
+  float* f[2][6] = {{nullptr}};
+  float* jd[3] = {nullptr, nullptr, nullptr};
+  float* jl[2][3] = {{nullptr}};
+  float *ca = nullptr, *cb = nullptr, *da = nullptr, *dbi = nullptr;
+  float *dk1 = nullptr, *dk2 = nullptr, *lk1 = nullptr, *lk2 = nullptr, *lk3 = nullptr;
+  int* src2 = nullptr;
+  long long* src3 = nullptr;
+  float* wave = nullptr;
+  long long wave_stride = 0;
+  short* probe_head = nullptr;
+  short* probe_next = nullptr;
+  float* traces = nullptr;
+  float* partial = nullptr;
+  float2* tw = nullptr;        // DFT twiddles [T][F] for tw_freqs
+  float2* S = nullptr;         // DFT output [F][P][B]
+  std::vector<double> tw_freqs;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one per starting buffer parity
+  int parity = 0;              // ping-pong buffer holding the current state
+
+  bool mats_dirty = true;
+  bool has_run = false;
+  int bb = 4;
+  int use_graph = 1;
+  float ch = 0.f, idx = 0.f;
+
+  long long box_floats() const { return (long long)B * plane; }
+  int poles() const { return (has_debye ? 1 : 0) | (has_lorentz ? 2 : 0); }
+  long long full_index(int64_t x, int64_t y, int64_t z) const {
+    return ndim == 2 ? x * N[1] + y : (x * N[1] + y) * N[2] + z;
+  }
+  long long box_index(int a, int b, int c) const {
+    return ndim == 2 ? (long long)a * pitch + b
+                     : (long long)a * row_block + (long long)b * pitch + c;
+  }
+  void drop_graph() {
+    for (auto& g : gexec) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
+};
+
+namespace {
+
+int upload_f32(float* dst, const std::vector<float>& v, hipStream_t s) {
+  CEEP_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  return CEEP_OK;
+}
+
+// FP64 coefficient maps (fdtd_2d_batched.py:152-165, fdtd_3d_batched.py:282-296,
+// dispersive.py:253-299, core/grid.py:178-184) over the box, stored as FP32.
+int commit_materials(ceep_fdtd* h) {
+  if (!h->mats_dirty) return CEEP_OK;
+  const long long nbox = h->plane;
+  std::vector<float> ca(nbox, 0.f), cb(nbox, 0.f);
+  std::vector<float> da, dbi, dk1, dk2, lk1, lk2, lk3;
+  if (h->ndim == 3 && h->magnetic) { da.assign(nbox, 1.f); dbi.assign(nbox, 0.f); }
+  if (h->has_debye) { dk1.assign(nbox, 0.f); dk2.assign(nbox, 0.f); }
+  if (h->has_lorentz) { lk1.assign(nbox, 0.f); lk2.assign(nbox, 0.f); lk3.assign(nbox, 0.f); }
+  const double dt = h->dt, inv = 1.0 / h->dx;
+  const int mz = h->ndim == 3 ? h->M[2] : 1;
+  for (int a = 0; a < h->M[0]; ++a)
+    for (int b = 0; b < h->M[1]; ++b)
+      for (int c = 0; c < mz; ++c) {
+        const long long g = h->full_index(a + h->org, b + h->org, h->ndim == 3 ? c + h->org : 0);
+        const long long o = h->ndim == 2 ? h->box_index(a, b, 0) : h->box_index(a, b, c);
+        double eps_add = 0.0;
+        if (h->has_debye) {
+          const double tau = h->deb_tau[g];
+          if (tau > 0) {
+            const double denom = 2.0 * tau + dt;
+            const double k1 = (2.0 * tau - dt) / denom;
+            const double k2 = (2.0 * kEps0 * h->deb_de[g] * dt) / denom;
+            eps_add += k2 / 2.0;
+            dk1[o] = (float)k1;
+            dk2[o] = (float)(k2 / dt);
+          }
+        }
+        if (h->has_lorentz) {
+          const double w0 = h->lor_w0[g];
+          if (w0 > 0) {
+            const double d = h->lor_dl[g], w02 = w0 * w0, dt2 = dt * dt;
+            const double denom = 4.0 + 2.0 * d * dt + w02 * dt2;
+            const double k1 = (8.0 - 2.0 * w02 * dt2) / denom;
+            const double k3 = -(4.0 - 2.0 * d * dt + w02 * dt2) / denom;
+            const double k2 = 2.0 * kEps0 * h->lor_de[g] * w02 * dt2 / denom;
+            eps_add += k2 / 2.0;
+            lk1[o] = (float)k1;
+            lk3[o] = (float)k3;
+            lk2[o] = (float)(k2 / dt);
+          }
+        }
+        double eps = h->eps_r[g] * kEps0;
+        if (h->has_debye || h->has_lorentz) eps = eps + eps_add;
+        const double s = h->sigma_e[g] * dt / (2.0 * eps);
+        ca[o] = (float)((1.0 - s) / (1.0 + s));
+        cb[o] = (float)((dt / eps) / (1.0 + s));
+        if (!da.empty()) {
+          const double mu = h->mu_r[g] * kMu0;
+          const double sm = h->sigma_m[g] * dt / (2.0 * mu);
+          da[o] = (float)((1.0 - sm) / (1.0 + sm));
+          dbi[o] = (float)((dt / mu) / (1.0 + sm) * inv);
+        }
+      }
+  hipStream_t s = h->stream;
+  int rc;
+  if ((rc = upload_f32(h->ca, ca, s)) || (rc = upload_f32(h->cb, cb, s))) return rc;
+  dfree(h->da); dfree(h->dbi); dfree(h->dk1); dfree(h->dk2);
+  dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
+  if (!da.empty()) {
+    if ((rc = dalloc(&h->da, nbox)) || (rc = dalloc(&h->dbi, nbox))) return rc;
+    if ((rc = upload_f32(h->da, da, s)) || (rc = upload_f32(h->dbi, dbi, s))) return rc;
+  }
+  if (h->has_debye) {
+    if ((rc = dalloc(&h->dk1, nbox)) || (rc = dalloc(&h->dk2, nbox))) return rc;
+    if ((rc = upload_f32(h->dk1, dk1, s)) || (rc = upload_f32(h->dk2, dk2, s))) return rc;
+  }
+  if (h->has_lorentz) {
+    if ((rc = dalloc(&h->lk1, nbox)) || (rc = dalloc(&h->lk2, nbox)) ||
+        (rc = dalloc(&h->lk3, nbox)))
+      return rc;
+    if ((rc = upload_f32(h->lk1, lk1, s)) || (rc = upload_f32(h->lk2, lk2, s)) ||
+        (rc = upload_f32(h->lk3, lk3, s)))
+      return rc;
+  }
+  // ADE current buffers follow the pole set
+  const int ncomp = h->ndim == 2 ? 1 : 3;
+  for (int k = 0; k < 3; ++k) {
+    dfree(h->jd[k]);
+    dfree(h->jl[0][k]);
+    dfree(h->jl[1][k]);
+  }
+  const size_t jbytes = (size_t)h->box_floats() * sizeof(float);
+  for (int k = 0; k < ncomp; ++k) {
+    if (h->has_debye) {
+      if ((rc = dalloc(&h->jd[k], h->box_floats()))) return rc;
+      CEEP_HIP(hipMemsetAsync(h->jd[k], 0, jbytes, s));
+    }
+    if (h->has_lorentz) {
+      for (int i = 0; i < 2; ++i) {
+        if ((rc = dalloc(&h->jl[i][k], h->box_floats()))) return rc;
+        CEEP_HIP(hipMemsetAsync(h->jl[i][k], 0, jbytes, s));
+      }
+    }
+  }
+  CEEP_HIP(hipStreamSynchronize(s));
+  h->drop_graph();
+  h->mats_dirty = false;
+  return CEEP_OK;
+}
+
+int zero_state(ceep_fdtd* h) {
+  hipStream_t s = h->stream;
+  const size_t bytes = (size_t)h->box_floats() * sizeof(float);
+  const int nf = h->ndim == 2 ? 3 : 6;
+  for (int k = 0; k < nf; ++k) {
+    CEEP_HIP(hipMemsetAsync(h->f[0][k], 0, bytes, s));
+    if (h->ndim == 2) CEEP_HIP(hipMemsetAsync(h->f[1][k], 0, bytes, s));
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (h->jd[k]) CEEP_HIP(hipMemsetAsync(h->jd[k], 0, bytes, s));
+    if (h->jl[0][k]) CEEP_HIP(hipMemsetAsync(h->jl[0][k], 0, bytes, s));
+    if (h->jl[1][k]) CEEP_HIP(hipMemsetAsync(h->jl[1][k], 0, bytes, s));
+  }
+  h->parity = 0;
+  return CEEP_OK;
+}
+
+// Enqueue total_steps steps starting from ping-pong buffer `start`.
+int launch_steps(ceep_fdtd* h, int start) {
+  hipStream_t s = h->stream;
+  const int poles = h->poles();
+  if (h->ndim == 2) {
+    ceep::Step2DArgs a{};
+    a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
+    a.ca = h->ca; a.cb = h->cb;
+    a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
+    a.l_k1 = h->lk1; a.l_k2dt = h->lk2; a.l_k3 = h->lk3;
+    a.jd = h->jd[0];
+    a.ch = h->ch; a.idx = h->idx;
+    a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+    a.T = h->T; a.P = h->P;
+    a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
+    for (int st = 0; st < h->T; ++st) {
+      const int i = (start + st) & 1, o = i ^ 1;
+      a.ez_in = h->f[i][0]; a.hx_in = h->f[i][1]; a.hy_in = h->f[i][2];
+      a.ez_out = h->f[o][0]; a.hx_out = h->f[o][1]; a.hy_out = h->f[o][2];
+      a.jl_cur = h->jl[i][0]; a.jl_prev = h->jl[o][0];
+      a.step = st;
+      CEEP_HIP(ceep::launch_step2d(a, poles, h->bb, s));
+    }
+  } else {
+    ceep::Step3DArgs a{};
+    a.Mx = h->M[0]; a.My = h->M[1]; a.Mz = h->M[2]; a.pitch = h->pitch; a.B = h->B;
+    a.sx = h->row_block; a.bstride = h->plane;
+    a.ex = h->f[0][0]; a.ey = h->f[0][1]; a.ez = h->f[0][2];
+    a.hx = h->f[0][3]; a.hy = h->f[0][4]; a.hz = h->f[0][5];
+    for (int k = 0; k < 3; ++k) a.jd[k] = h->jd[k];
+    a.ca = h->ca; a.cb = h->cb; a.da = h->da; a.dbi = h->dbi;
+    a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
+    a.l_k1 = h->lk1; a.l_k2dt = h->lk2; a.l_k3 = h->lk3;
+    a.ch = h->ch; a.idx = h->idx;
+    a.src_cell = h->src3; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+    a.T = h->T; a.P = h->P;
+    a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
+    const int bb = h->bb > 4 ? 4 : h->bb;
+    for (int st = 0; st < h->T; ++st) {
+      const int i = (start + st) & 1, o = i ^ 1;
+      for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
+      a.step = st;
+      CEEP_HIP(ceep::launch_h3d(a, bb, s));
+      CEEP_HIP(ceep::launch_e3d(a, poles, bb, s));
+    }
+  }
+  return CEEP_OK;
+}
+
+int ready_check(ceep_fdtd* h) {
+  if (!h) return fail(CEEP_EINVAL, "null handle");
+  if (!h->have_sources) return fail(CEEP_ESTATE, "sources not set (ceep_fdtd_set_sources)");
+  if (!h->have_probes) return fail(CEEP_ESTATE, "probes not set (ceep_fdtd_set_probes)");
+  return commit_materials(h);
+}
+
+// Advance total_steps steps from the current state (the reference's run()
+// continues from the fields it finished with; _build starts them at zero).
+// Traces are cleared first.  Caller synchronizes.
+int run_device(ceep_fdtd* h) {
+  CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), h->stream));
+  const int start = h->parity;
+  int rc = CEEP_OK;
+  if (!h->use_graph) {
+    rc = launch_steps(h, start);
+  } else {
+    if (!h->gexec[start]) {
+      hipGraph_t g = nullptr;
+      CEEP_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      rc = launch_steps(h, start);
+      hipError_t e = hipStreamEndCapture(h->stream, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return fail(CEEP_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&h->gexec[start], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        h->gexec[start] = nullptr;
+        return fail(CEEP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+    }
+    CEEP_HIP(hipGraphLaunch(h->gexec[start], h->stream));
+  }
+  if (rc) return rc;
+  h->parity = start ^ (h->T & 1);
+  return CEEP_OK;
+}
+
+int in_box(const ceep_fdtd* h, const int64_t* p, bool interior) {
+  for (int d = 0; d < h->ndim; ++d) {
+    const int64_t a = p[d] - h->org;
+    if (interior ? (a < 1 || a > h->M[d] - 2) : (a < 0 || a > h->M[d] - 1)) return 0;
+  }
+  return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ceep_fdtd_last_error(void) { return g_err.c_str(); }
+
+int ceep_fdtd_abi_version(void) { return 1; }
+
+int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, int batch,
+                     int n_probes, int total_steps, double dx, double dt, ceep_fdtd** out) {
+  if (!out) return fail(CEEP_EINVAL, "out is null");
+  *out = nullptr;
+  if (ndim != 2 && ndim != 3) return fail(CEEP_EINVAL, "ndim must be 2 or 3");
+  if (!shape) return fail(CEEP_EINVAL, "shape is null");
+  if (box_mode != CEEP_BOX_REFERENCE && box_mode != CEEP_BOX_FULL)
+    return fail(CEEP_EINVAL, "unknown box mode");
+  if (box_mode == CEEP_BOX_FULL && ndim != 2)
+    return fail(CEEP_EINVAL, "the full-grid PEC box (run_cpu) exists for 2D only");
+  if (batch < 1 || batch > ceep::kMaxBatch) return fail(CEEP_EINVAL, "batch must be in [1, 1024]");
+  if (n_probes < 0 || n_probes > 32767) return fail(CEEP_EINVAL, "n_probes must be in [0, 32767]");
+  if (total_steps < 1) return fail(CEEP_EINVAL, "total_steps must be >= 1");
+  if (!(dx > 0) || !(dt > 0)) return fail(CEEP_EINVAL, "dx and dt must be positive");
+  if (box_mode == CEEP_BOX_REFERENCE && cpml_n < 2)
+    return fail(CEEP_EINVAL, "cpml_thickness must be >= 2 (the reference's CPML profile "
+                             "divides by cpml_thickness - 1)");
+  ceep_fdtd* h = new ceep_fdtd();
+  h->ndim = ndim; h->box_mode = box_mode; h->n = cpml_n; h->B = batch; h->P = n_probes;
+  h->T = total_steps; h->dx = dx; h->dt = dt;
+  h->org = box_mode == CEEP_BOX_REFERENCE ? cpml_n : 0;
+  h->full_cells = 1;
+  for (int d = 0; d < ndim; ++d) {
+    h->N[d] = shape[d];
+    h->M[d] = box_mode == CEEP_BOX_REFERENCE ? (int)(shape[d] - 2 * cpml_n + 1) : (int)shape[d];
+    if (h->M[d] < 3) {
+      delete h;
+      return fail(CEEP_EINVAL, "grid too small: each axis needs N - 2*cpml_thickness + 1 >= 3");
+    }
+    h->full_cells *= shape[d];
+  }
+  const int inner = h->M[ndim - 1];
+  h->pitch = (inner + 63) / 64 * 64;
+  if (ndim == 2) {
+    h->plane = (long long)h->M[0] * h->pitch;
+    if ((long long)h->plane > (1LL << 31) - 1) {
+      delete h;
+      return fail(CEEP_EUNSUPPORTED, "2D box too large for 32-bit cell indices");
+    }
+  } else {
+    h->row_block = (long long)h->M[1] * h->pitch;
+    h->plane = h->M[0] * h->row_block;
+  }
+  h->ch = (float)((dt / kMu0) * (1.0 / dx));
+  h->idx = (float)(1.0 / dx);
+  h->eps_r.assign(h->full_cells, 1.0);
+  h->sigma_e.assign(h->full_cells, 0.0);
+  int rc = CEEP_OK;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(CEEP_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  const int nf = ndim == 2 ? 3 : 6;
+  for (int k = 0; k < nf && !rc; ++k) {
+    rc = dalloc(&h->f[0][k], h->box_floats());
+    if (!rc && ndim == 2) rc = dalloc(&h->f[1][k], h->box_floats());
+  }
+  if (!rc) rc = dalloc(&h->ca, h->plane);
+  if (!rc) rc = dalloc(&h->cb, h->plane);
+  if (!rc) rc = dalloc(&h->traces, (size_t)batch * (n_probes > 0 ? n_probes : 1) * total_steps);
+  if (!rc) rc = dalloc(&h->probe_head, h->plane);
+  if (!rc) rc = dalloc(&h->probe_next, n_probes > 0 ? n_probes : 1);
+  if (!rc) rc = dalloc(&h->partial, 1024);
+  if (!rc && ndim == 2) rc = dalloc(&h->src2, batch);
+  if (!rc && ndim == 3) rc = dalloc(&h->src3, batch);
+  if (!rc) rc = zero_state(h);
+  if (!rc && hipStreamSynchronize(h->stream) != hipSuccess)
+    rc = fail(CEEP_EHIP, "stream synchronize after zeroing failed");
+  if (rc) {
+    ceep_fdtd_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_reset(ceep_fdtd* h) {
+  if (!h) return fail(CEEP_EINVAL, "null handle");
+  int rc = commit_materials(h);   // (re)allocates the ADE currents for the pole set
+  if (rc) return rc;
+  if ((rc = zero_state(h))) return rc;
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  return CEEP_OK;
+}
+
+int ceep_fdtd_destroy(ceep_fdtd* h) {
+  if (!h) return CEEP_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->drop_graph();
+  for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 6; ++k) dfree(h->f[i][k]);
+  for (int k = 0; k < 3; ++k) { dfree(h->jd[k]); dfree(h->jl[0][k]); dfree(h->jl[1][k]); }
+  dfree(h->ca); dfree(h->cb); dfree(h->da); dfree(h->dbi);
+  dfree(h->dk1); dfree(h->dk2); dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
+  dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next);
+  dfree(h->traces); dfree(h->partial); dfree(h->tw); dfree(h->S);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_set_materials(ceep_fdtd* h, const double* eps_r, const double* sigma_e,
+                            const double* mu_r, const double* sigma_m) {
+  if (!h || !eps_r || !sigma_e) return fail(CEEP_EINVAL, "null argument");
+  for (long long i = 0; i < h->full_cells; ++i)
+    if (!(eps_r[i] > 0) || !std::isfinite(sigma_e[i]))
+      return fail(CEEP_EINVAL, "eps_r must be > 0 and sigma_e finite everywhere");
+  h->eps_r.assign(eps_r, eps_r + h->full_cells);
+  h->sigma_e.assign(sigma_e, sigma_e + h->full_cells);
+  h->magnetic = false;
+  if (h->ndim == 3 && (mu_r || sigma_m)) {
+    h->mu_r.assign(h->full_cells, 1.0);
+    h->sigma_m.assign(h->full_cells, 0.0);
+    if (mu_r) h->mu_r.assign(mu_r, mu_r + h->full_cells);
+    if (sigma_m) h->sigma_m.assign(sigma_m, sigma_m + h->full_cells);
+    for (long long i = 0; i < h->full_cells; ++i) {
+      if (!(h->mu_r[i] > 0)) return fail(CEEP_EINVAL, "mu_r must be > 0");
+      if (h->mu_r[i] != 1.0 || h->sigma_m[i] != 0.0) h->magnetic = true;
+    }
+  } else if (h->ndim == 2 && (mu_r || sigma_m)) {
+    return fail(CEEP_EINVAL, "2D TMz takes no magnetic maps (the reference fixes da=1, db=dt/mu0)");
+  }
+  h->mats_dirty = true;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_set_debye(ceep_fdtd* h, const double* delta_eps, const double* tau) {
+  if (!h || !delta_eps || !tau) return fail(CEEP_EINVAL, "null argument");
+  h->deb_de.assign(delta_eps, delta_eps + h->full_cells);
+  h->deb_tau.assign(tau, tau + h->full_cells);
+  h->has_debye = true;
+  h->mats_dirty = true;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_set_lorentz(ceep_fdtd* h, const double* delta_eps, const double* omega_0,
+                          const double* delta) {
+  if (!h || !delta_eps || !omega_0 || !delta) return fail(CEEP_EINVAL, "null argument");
+  h->lor_de.assign(delta_eps, delta_eps + h->full_cells);
+  h->lor_w0.assign(omega_0, omega_0 + h->full_cells);
+  h->lor_dl.assign(delta, delta + h->full_cells);
+  h->has_lorentz = true;
+  h->mats_dirty = true;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_clear_poles(ceep_fdtd* h) {
+  if (!h) return fail(CEEP_EINVAL, "null handle");
+  h->has_debye = h->has_lorentz = false;
+  h->deb_de.clear(); h->deb_tau.clear(); h->lor_de.clear(); h->lor_w0.clear(); h->lor_dl.clear();
+  h->mats_dirty = true;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_set_sources(ceep_fdtd* h, const int64_t* pos, const double* waveform,
+                          int per_batch) {
+  if (!h || !pos || !waveform) return fail(CEEP_EINVAL, "null argument");
+  for (int b = 0; b < h->B; ++b) {
+    const int64_t* p = pos + (size_t)b * h->ndim;
+    for (int d = 0; d < h->ndim; ++d)
+      if (p[d] < 0 || p[d] >= h->N[d])
+        return fail(CEEP_EINVAL, "source " + std::to_string(b) + " lies outside the grid");
+    if (!in_box(h, p, h->box_mode == CEEP_BOX_REFERENCE))
+      return fail(CEEP_EUNSUPPORTED,
+                  "source " + std::to_string(b) +
+                      " lies outside the E-update interior [cpml_thickness+1, N-cpml_thickness-1]; "
+                      "these kernels step only the cavity the reference's inert CPML encloses");
+  }
+  h->src.assign(pos, pos + (size_t)h->B * h->ndim);
+  const size_t nw = per_batch ? (size_t)h->B * h->T : (size_t)h->T;
+  std::vector<float> w(nw);
+  for (size_t i = 0; i < nw; ++i) w[i] = (float)waveform[i];
+  h->wave_stride = per_batch ? h->T : 0;
+  dfree(h->wave);
+  int rc = dalloc(&h->wave, nw);
+  if (rc) return rc;
+  if ((rc = upload_f32(h->wave, w, h->stream))) return rc;
+  if (h->ndim == 2) {
+    std::vector<int> sc(h->B);
+    for (int b = 0; b < h->B; ++b)
+      sc[b] = (int)h->box_index((int)(h->src[2 * b] - h->org), (int)(h->src[2 * b + 1] - h->org), 0);
+    CEEP_HIP(hipMemcpyAsync(h->src2, sc.data(), sc.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  } else {
+    std::vector<long long> sc(h->B);
+    for (int b = 0; b < h->B; ++b)
+      sc[b] = h->box_index((int)(h->src[3 * b] - h->org), (int)(h->src[3 * b + 1] - h->org),
+                           (int)(h->src[3 * b + 2] - h->org));
+    CEEP_HIP(hipMemcpyAsync(h->src3, sc.data(), sc.size() * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+  }
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  h->drop_graph();   // the wave buffer may have moved
+  h->have_sources = true;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_set_probes(ceep_fdtd* h, const int64_t* pos) {
+  if (!h || (!pos && h->P > 0)) return fail(CEEP_EINVAL, "null argument");
+  std::vector<short> head(h->plane, (short)-1), next(h->P > 0 ? h->P : 1, (short)-1);
+  for (int q = h->P - 1; q >= 0; --q) {
+    const int64_t* p = pos + (size_t)q * h->ndim;
+    for (int d = 0; d < h->ndim; ++d)
+      if (p[d] < 0 || p[d] >= h->N[d])
+        return fail(CEEP_EINVAL, "probe " + std::to_string(q) + " lies outside the grid");
+    if (!in_box(h, p, false)) continue;   // stays exactly 0 in the reference
+    const long long o = h->ndim == 2
+                            ? h->box_index((int)(p[0] - h->org), (int)(p[1] - h->org), 0)
+                            : h->box_index((int)(p[0] - h->org), (int)(p[1] - h->org), (int)(p[2] - h->org));
+    next[q] = head[o];
+    head[o] = (short)q;
+  }
+  h->prb.assign(pos, pos + (size_t)h->P * h->ndim);
+  CEEP_HIP(hipMemcpyAsync(h->probe_head, head.data(), head.size() * sizeof(short), hipMemcpyHostToDevice, h->stream));
+  CEEP_HIP(hipMemcpyAsync(h->probe_next, next.data(), next.size() * sizeof(short), hipMemcpyHostToDevice, h->stream));
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  h->have_probes = true;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_run(ceep_fdtd* h, double* traces_out) {
+  int rc = ready_check(h);
+  if (rc) return rc;
+  if ((rc = run_device(h))) return rc;
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  h->has_run = true;
+  if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
+  return CEEP_OK;
+}
+
+int ceep_fdtd_get_traces(ceep_fdtd* h, double* out) {
+  if (!h || !out) return fail(CEEP_EINVAL, "null argument");
+  if (!h->has_run) return fail(CEEP_ESTATE, "no run yet");
+  const size_t n = (size_t)h->B * h->P * h->T;
+  std::vector<float> tmp(n);
+  if (n) CEEP_HIP(hipMemcpy(tmp.data(), h->traces, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) out[i] = tmp[i];
+  return CEEP_OK;
+}
+
+int ceep_fdtd_dft(ceep_fdtd* h, const double* freqs, int n_freq, double* s_out) {
+  if (!h || !freqs || !s_out || n_freq < 1) return fail(CEEP_EINVAL, "bad argument");
+  if (!h->has_run) return fail(CEEP_ESTATE, "no run yet");
+  if (h->P == 0) return CEEP_OK;
+  const int F = n_freq, T = h->T;
+  int rc;
+  // twiddles (cos, -sin)(w_f s dt) in FP64 -> FP32, cached per frequency list
+  if (h->tw_freqs.size() != (size_t)F || !std::equal(freqs, freqs + F, h->tw_freqs.begin())) {
+    std::vector<float2> tw((size_t)T * F);
+    for (int s = 0; s < T; ++s) {
+      const double t = s * h->dt;
+      for (int f = 0; f < F; ++f) {
+        const double w = 2.0 * M_PI * freqs[f];
+        tw[(size_t)s * F + f] = make_float2((float)std::cos(w * t), (float)(-std::sin(w * t)));
+      }
+    }
+    dfree(h->tw);
+    dfree(h->S);
+    h->tw_freqs.clear();
+    if ((rc = dalloc(&h->tw, tw.size()))) return rc;
+    if ((rc = dalloc(&h->S, (size_t)F * h->P * h->B))) return rc;
+    CEEP_HIP(hipMemcpyAsync(h->tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    CEEP_HIP(hipStreamSynchronize(h->stream));
+    h->tw_freqs.assign(freqs, freqs + F);
+  }
+  const size_t ns = (size_t)F * h->P * h->B;
+  std::vector<float2> S(ns);
+  CEEP_HIP(ceep::launch_dft(h->traces, h->tw, h->B, h->P, T, F, (float)h->dt, h->S, h->stream));
+  CEEP_HIP(hipMemcpyAsync(S.data(), h->S, ns * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < ns; ++i) { s_out[2 * i] = S[i].x; s_out[2 * i + 1] = S[i].y; }
+  return CEEP_OK;
+}
+
+int ceep_fdtd_get_field(ceep_fdtd* h, int comp, double* out) {
+  if (!h || !out) return fail(CEEP_EINVAL, "null argument");
+  const int nf = h->ndim == 2 ? 3 : 6;
+  if (comp < 0 || comp >= nf) return fail(CEEP_EINVAL, "bad field component");
+  const size_t total = (size_t)h->B * h->full_cells;
+  std::memset(out, 0, total * sizeof(double));
+  const int buf = h->ndim == 2 ? h->parity : 0;
+  std::vector<float> box(h->box_floats());
+  CEEP_HIP(hipMemcpy(box.data(), h->f[buf][comp], box.size() * sizeof(float), hipMemcpyDeviceToHost));
+  const int mz = h->ndim == 3 ? h->M[2] : 1;
+  for (int b = 0; b < h->B; ++b)
+    for (int a = 0; a < h->M[0]; ++a)
+      for (int y = 0; y < h->M[1]; ++y)
+        for (int z = 0; z < mz; ++z) {
+          const long long g = h->full_index(a + h->org, y + h->org, h->ndim == 3 ? z + h->org : 0);
+          out[(size_t)b * h->full_cells + g] = box[(size_t)b * h->plane + h->box_index(a, y, z)];
+        }
+  return CEEP_OK;
+}
+
+int ceep_fdtd_energy(ceep_fdtd* h, double* out) {
+  if (!h || !out) return fail(CEEP_EINVAL, "null argument");
+  *out = 0.0;
+  const int buf = h->ndim == 2 ? h->parity : 0;
+  const int ne = h->ndim == 2 ? 1 : 3;
+  const int nf = h->ndim == 2 ? 3 : 6;
+  double se = 0.0, sh = 0.0;
+  std::vector<float> part(1024);
+  for (int k = 0; k < nf; ++k) {
+    CEEP_HIP(ceep::launch_sumsq(h->f[buf][k], h->box_floats(), h->partial, 1024, h->stream));
+    CEEP_HIP(hipMemcpyAsync(part.data(), h->partial, 1024 * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    CEEP_HIP(hipStreamSynchronize(h->stream));
+    double acc = 0.0;
+    for (float v : part) acc += v;
+    (k < ne ? se : sh) += acc;
+  }
+  const double dV = h->ndim == 2 ? h->dx * h->dx : h->dx * h->dx * h->dx;
+  *out = (0.5 * kEps0 * se + 0.5 * kMu0 * sh) * dV;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_time_runs(ceep_fdtd* h, int reps, double* ms_per_run) {
+  if (!h || !ms_per_run || reps < 1) return fail(CEEP_EINVAL, "bad argument");
+  int rc = ready_check(h);
+  if (rc) return rc;
+  // events bracket only the time-step launches of each run, not the zeroing
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  for (auto& e : ev) CEEP_HIP(hipEventCreate(&e));
+  for (int r = 0; r < reps && !rc; ++r) {
+    rc = zero_state(h);
+    if (rc) break;
+    hipError_t e = hipEventRecord(ev[2 * r], h->stream);
+    if (e != hipSuccess) { rc = fail(CEEP_EHIP, "hipEventRecord failed"); break; }
+    rc = run_device(h);
+    if (!rc && hipEventRecord(ev[2 * r + 1], h->stream) != hipSuccess)
+      rc = fail(CEEP_EHIP, "hipEventRecord failed");
+  }
+  double total = 0.0;
+  if (!rc && hipStreamSynchronize(h->stream) != hipSuccess)
+    rc = fail(CEEP_EHIP, "stream synchronize failed");
+  for (int r = 0; r < reps && !rc; ++r) {
+    float ms = 0.f;
+    hipError_t e = hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]);
+    if (e != hipSuccess) rc = fail(CEEP_EHIP, std::string("hipEventElapsedTime: ") + hipGetErrorString(e));
+    total += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  h->has_run = true;
+  *ms_per_run = total / reps;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
+  if (!h) return fail(CEEP_EINVAL, "null handle");
+  if (key == 0) {
+    if (value != 1 && value != 2 && value != 4 && value != 8)
+      return fail(CEEP_EINVAL, "batch planes per thread must be 1, 2, 4 or 8");
+    h->bb = value;
+  } else if (key == 1) {
+    h->use_graph = value ? 1 : 0;
+  } else {
+    return fail(CEEP_EINVAL, "unknown tuning key");
+  }
+  h->drop_graph();
+  return CEEP_OK;
+}
+
+int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out) {
+  if (!h || !out) return fail(CEEP_EINVAL, "null argument");
+  out[0] = h->M[0]; out[1] = h->M[1]; out[2] = h->ndim == 3 ? h->M[2] : 1;
+  out[3] = h->pitch; out[4] = h->org;
+  return CEEP_OK;
+}
+
+}  // extern "C"

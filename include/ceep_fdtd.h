@@ -1,0 +1,137 @@
+/*
+ * ceep_fdtd.h -- C ABI of the MI355X batched FDTD engine (libceep_fdtd.so).
+ *
+ * Plain pointers and sizes only.  Host arrays are C-order float64 unless
+ * stated; positions are int64 grid indices.  Every call returns a status code
+ * (CEEP_OK = 0); on failure ceep_fdtd_last_error() returns a message that
+ * stays valid until the calling thread's next failing call.
+ *
+ * The reference has no FFI: its boundary is the Python class API of
+ * BatchedFDTD2D / BatchedFDTD3D (src/ceep/solvers/fdtd_2d_batched.py,
+ * src/ceep/solvers/fdtd_3d_batched.py).  Each entry point names the reference
+ * method whose work it takes over.  A handle owns one HIP stream and its
+ * device buffers; calls on one handle must not overlap, distinct handles are
+ * independent.
+ */
+#ifndef CEEP_FDTD_H
+#define CEEP_FDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ceep_fdtd ceep_fdtd;
+
+enum {
+  CEEP_OK = 0,
+  CEEP_EINVAL = 1,       /* bad argument (ValueError / IndexError in Python) */
+  CEEP_EHIP = 2,         /* HIP runtime error */
+  CEEP_ENOMEM = 3,       /* device allocation failed */
+  CEEP_ESTATE = 4,       /* call out of order (e.g. DFT before run) */
+  CEEP_EUNSUPPORTED = 5  /* valid input this engine does not handle */
+};
+
+/* Box modes for ceep_fdtd_create. */
+enum {
+  CEEP_BOX_REFERENCE = 0, /* BatchedFDTD.run(): E interior [n+1, N-n-1], box [n, N-n] */
+  CEEP_BOX_FULL = 1       /* BatchedFDTD2D.run_cpu(): PEC box over the full grid */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* ceep_fdtd_last_error(void);
+
+/* ABI revision of this header (bumped on incompatible changes). */
+int ceep_fdtd_abi_version(void);
+
+/* Replaces BatchedFDTD2D.__init__ + _build (fdtd_2d_batched.py:66-98, 138-193)
+ * and BatchedFDTD3D.__init__ + _build (fdtd_3d_batched.py:104-141, 266-326):
+ * allocates device state for a batch of `batch` TX and `n_probes` RX over
+ * `total_steps` steps.  ndim = 2 (shape = {nx, ny}) or 3 (shape = {nx, ny, nz});
+ * dt is supplied by the caller (reference: dx/(c sqrt 2)*0.99 in 2D,
+ * cfl_timestep_3d(dx,dx,dx,0.5)*0.99 in 3D). */
+int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, int batch,
+                     int n_probes, int total_steps, double dx, double dt, ceep_fdtd** out);
+
+int ceep_fdtd_destroy(ceep_fdtd* h);
+
+/* Replaces the coefficient part of _build (fdtd_2d_batched.py:152-165;
+ * fdtd_3d_batched.py:282-296): full-grid maps eps_r, sigma_e (and in 3D
+ * optional mu_r, sigma_m; NULL = 1 and 0). ca/cb/da/db are formed in FP64
+ * on the host, including eps_eff = eps_r*eps0 + sum(k2/2) of any ADE poles
+ * (core/grid.py:178-184), and stored as FP32 on the device. */
+int ceep_fdtd_set_materials(ceep_fdtd* h, const double* eps_r, const double* sigma_e,
+                            const double* mu_r, const double* sigma_m);
+
+/* One Debye pole per cell (DispersiveManager Debye branch,
+ * materials/dispersive.py:253-261): full-grid delta_eps and tau maps, tau <= 0
+ * meaning no pole.  With poles set, eps_r of set_materials is eps_inf. */
+int ceep_fdtd_set_debye(ceep_fdtd* h, const double* delta_eps, const double* tau);
+
+/* One Lorentz pole per cell (dispersive.py:278-299): full-grid delta_eps,
+ * omega_0 (rad/s; <= 0 meaning no pole) and delta (rad/s) maps. */
+int ceep_fdtd_set_lorentz(ceep_fdtd* h, const double* delta_eps, const double* omega_0,
+                          const double* delta);
+
+/* Remove all ADE poles. */
+int ceep_fdtd_clear_poles(ceep_fdtd* h);
+
+/* Source positions (batch x ndim) and waveform: per_batch = 0 -> one (T,)
+ * waveform for every TX (fdtd_2d_batched.py:509-515), 1 -> (batch, T). */
+int ceep_fdtd_set_sources(ceep_fdtd* h, const int64_t* pos, const double* waveform,
+                          int per_batch);
+
+/* Probe positions (n_probes x ndim) recorded for every TX (:517-526). */
+int ceep_fdtd_set_probes(ceep_fdtd* h, const int64_t* pos);
+
+/* Zero every field and ADE current (the state _build allocates,
+ * fdtd_2d_batched.py:147-149, fdtd_3d_batched.py:274-279).  A new handle
+ * starts zeroed. */
+int ceep_fdtd_reset(ceep_fdtd* h);
+
+/* Replaces the time loop of run() (fdtd_2d_batched.py:454-526,
+ * fdtd_3d_batched.py:918-958): advances total_steps steps on the device from
+ * the current state (like the reference, a second run continues from the
+ * fields the first one left) and, if traces_out != NULL, copies the (batch,
+ * n_probes, total_steps) traces to it as float64 (the probe_data layout of
+ * fdtd_2d_batched.py:188-191). */
+int ceep_fdtd_run(ceep_fdtd* h, double* traces_out);
+
+/* Copy the traces of the last run (batch, n_probes, total_steps) float64. */
+int ceep_fdtd_get_traces(ceep_fdtd* h, double* traces_out);
+
+/* Running DFT of the last run's traces (DFTMonitor.update, solvers/dft.py:55-90,
+ * summed over every step) into the MultistaticSParameters layout
+ * (s_params.py:99): s_out is (n_freq, n_probes, batch) complex as interleaved
+ * float64 (re, im). */
+int ceep_fdtd_dft(ceep_fdtd* h, const double* freqs, int n_freq, double* s_out);
+
+/* Copy the current field component comp (2D: 0 = ez, 1 = hx, 2 = hy; 3D:
+ * 0..5 = ex, ey, ez, hx, hy, hz) to a full-grid (batch, nx, ny[, nz])
+ * float64 array (cells outside the stepped box are exactly 0 in the
+ * reference and are written as 0). */
+int ceep_fdtd_get_field(ceep_fdtd* h, int comp, double* out);
+
+/* BatchedFDTD3D.compute_energy (fdtd_3d_batched.py:987-1011): sum over
+ * the batch and grid of (eps0/2 |E|^2 + mu0/2 |H|^2) dV; 2D: dV = dx*dy. */
+int ceep_fdtd_energy(ceep_fdtd* h, double* out);
+
+/* Timing: `reps` device-resident runs, each from a zeroed state (no host
+ * copies), bracketed by HIP events on the handle's stream; *ms_per_run =
+ * mean elapsed ms per run.  Leaves the state of the last run. */
+int ceep_fdtd_time_runs(ceep_fdtd* h, int reps, double* ms_per_run);
+
+/* Tuning knobs: key 0 = batch planes per thread (1,2,4,8), key 1 = use a
+ * captured hipGraph for the time loop (0/1). */
+int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
+
+/* Geometry of the stepped box: out[0..2] = box size per axis, out[3] =
+ * row pitch in floats, out[4] = box origin (grid index of box cell 0). */
+int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CEEP_FDTD_H */

@@ -1,0 +1,182 @@
+"""Write the golden vectors under tests/golden/ from the reference itself.
+
+Run once, in the build container only (the reference is not present on GPU
+machines):  ``python tests/golden/make_golden.py``.  It needs the read-only
+reference tree at /root/reference and NumPy; nothing of the reference is
+copied -- only inputs and the arrays it returns are stored.
+
+Vectors written (all float64, ``numpy.savez_compressed``, no pickles):
+
+* ``ceep_results_traces.json`` -- the three 100-sample traces the reference
+  ships in ``ceep_results.json`` (its CuPy run of ``BatchedFDTD2D.run()``).
+* ``bf2d_*.npz`` -- ``BatchedFDTD2D.run()`` under the NumPy backend
+  (src/ceep/solvers/fdtd_2d_batched.py:408-538), including sources inside the
+  CPML so the CPML arithmetic itself is pinned.
+* ``bf3d_*.npz`` -- ``BatchedFDTD3D.run()`` (fdtd_3d_batched.py:886-985).
+* ``ade2d_*.npz`` -- ``FDTD2D`` with one Debye / one Lorentz pole
+  (fdtd_2d.py:142-178, dispersive.py:240-426) on a grid the pulse never
+  crosses, pinning the ADE operation order.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+sys.dont_write_bytecode = True
+sys.path.insert(0, os.path.join(REF, "src"))
+
+
+def _traces_2d(solver):
+    try:
+        solver.run()
+    except AttributeError:
+        # fdtd_2d_batched.py:529 calls ``np.asnumpy`` under NumPy; the probe
+        # buffer is complete at that point.
+        pass
+    return np.array(solver.probe_data, dtype=np.float64)
+
+
+def golden_ceep_results():
+    with open(os.path.join(REF, "ceep_results.json")) as f:
+        d = json.load(f)
+    out = {"parameters": d["parameters"],
+           "traces": {k: v["signal_real"] for k, v in d["scenarios"].items()}}
+    with open(os.path.join(HERE, "ceep_results_traces.json"), "w") as f:
+        json.dump(out, f)
+
+
+def golden_bf2d():
+    from ceep.core.backend import set_backend
+    from ceep.solvers.fdtd_2d_batched import BatchedFDTD2D
+    set_backend("numpy")
+
+    # (a) the ceep_results.json configuration, empty scenario
+    s = BatchedFDTD2D(nx=64, ny=64, dx=0.5e-3, total_steps=100, cpml_thickness=10,
+                      source_positions=[(32, 32)], probe_positions=[(32, 32)],
+                      frequency=2e9)
+    tr = _traces_2d(s)
+    np.savez_compressed(os.path.join(HERE, "bf2d_empty64.npz"), traces=tr, dt=s.dt,
+             waveform=np.asarray(s.waveform))
+
+    # (b) sources inside the CPML and on interior cells, probes everywhere
+    # (x-lo, interior, x-hi, y-lo and corner sources; probes on the rows and
+    # columns their fields can reach, since the CPML couples only along its
+    # own axis)
+    nx, ny, n, T = 40, 36, 5, 60
+    src = [(3, 10), (20, 18), (37, 30), (20, 2), (2, 33)]
+    prb = [(2, 2), (20, 18), (10, 30), (38, 33), (6, 6), (34, 29), (1, 10), (4, 10),
+           (36, 30), (38, 30), (20, 3), (20, 1), (2, 34), (3, 33)]
+    s = BatchedFDTD2D(nx=nx, ny=ny, dx=1e-3, total_steps=T, cpml_thickness=n,
+                      source_positions=src, probe_positions=prb, frequency=3e9,
+                      delay_factor=2.0)
+    s.set_material_circle(22, 16, 6, eps_r=4.0, sigma_e=0.3)
+    tr = _traces_2d(s)
+    np.savez_compressed(os.path.join(HERE, "bf2d_pml_sources.npz"), traces=tr, dt=s.dt,
+             waveform=np.asarray(s.waveform), eps_r=s._eps_r, sigma_e=s._sigma_e,
+             sources=np.array(src), probes=np.array(prb), nx=nx, ny=ny, n=n, dx=1e-3)
+
+    # (c) interior sources, several probes, a lossy disk and a long run so the
+    # cavity walls matter
+    nx, ny, n, T = 48, 44, 6, 150
+    src = [(20, 20), (30, 25)]
+    prb = [(20, 20), (30, 25), (12, 37), (7, 7), (40, 30)]
+    s = BatchedFDTD2D(nx=nx, ny=ny, dx=0.5e-3, total_steps=T, cpml_thickness=n,
+                      source_positions=src, probe_positions=prb, frequency=4e9)
+    s.set_material_circle(26, 22, 7, eps_r=9.0, sigma_e=0.5)
+    tr = _traces_2d(s)
+    np.savez_compressed(os.path.join(HERE, "bf2d_interior.npz"), traces=tr, dt=s.dt,
+             waveform=np.asarray(s.waveform), eps_r=s._eps_r, sigma_e=s._sigma_e,
+             sources=np.array(src), probes=np.array(prb), nx=nx, ny=ny, n=n, dx=0.5e-3)
+
+
+def golden_bf3d():
+    from ceep.core.backend import set_backend
+    from ceep.solvers.fdtd_3d_batched import BatchedFDTD3D
+    set_backend("numpy")
+
+    # (a) interior sources, dielectric sphere
+    nx = ny = nz = 24
+    n, T = 3, 40
+    src = [(12, 12, 12), (14, 10, 11)]
+    prb = [(12, 12, 12), (14, 10, 11), (8, 15, 13)]
+    s = BatchedFDTD3D(nx=nx, ny=ny, nz=nz, dx=1e-3, total_steps=T, cpml_thickness=n,
+                      source_positions=src, probe_positions=prb, frequency=5e9,
+                      delay_factor=2.0)
+    s.set_material_sphere(13, 12, 12, 4, eps_r=6.0, sigma_e=0.4)
+    s.run()
+    np.savez_compressed(os.path.join(HERE, "bf3d_sphere.npz"), traces=np.array(s.probe_data),
+             dt=s.dt, waveform=np.asarray(s.waveform), eps_r=s._eps_r, sigma_e=s._sigma_e,
+             mu_r=s._mu_r, sigma_m=s._sigma_m, sources=np.array(src),
+             probes=np.array(prb), shape=np.array([nx, ny, nz]), n=n, dx=1e-3)
+
+    # (b) one source inside the CPML, magnetic material, anisotropic box
+    nx, ny, nz = 20, 18, 16
+    n, T = 3, 30
+    src = [(1, 9, 8), (10, 8, 7)]
+    prb = [(1, 9, 8), (10, 8, 7), (2, 2, 2), (17, 15, 13)]
+    s = BatchedFDTD3D(nx=nx, ny=ny, nz=nz, dx=1e-3, total_steps=T, cpml_thickness=n,
+                      source_positions=src, probe_positions=prb, frequency=5e9,
+                      delay_factor=2.0)
+    s.set_material_region(6, 14, 5, 12, 4, 12, eps_r=3.0, sigma_e=0.2, mu_r=2.0,
+                          sigma_m=50.0)
+    s.run()
+    np.savez_compressed(os.path.join(HERE, "bf3d_pml_magnetic.npz"), traces=np.array(s.probe_data),
+             dt=s.dt, waveform=np.asarray(s.waveform), eps_r=s._eps_r, sigma_e=s._sigma_e,
+             mu_r=s._mu_r, sigma_m=s._sigma_m, sources=np.array(src),
+             probes=np.array(prb), shape=np.array([nx, ny, nz]), n=n, dx=1e-3,
+             energy=s.compute_energy())
+
+
+def golden_ade2d():
+    from ceep.core.backend import set_backend
+    from ceep.core.config import GridConfig, SimulationConfig
+    from ceep.materials.dispersive import DebyePole, LorentzPole
+    from ceep.solvers.fdtd_2d import FDTD2D
+    from ceep.sources.waveforms import ModulatedGaussianSource
+    set_backend("numpy")
+
+    nx = ny = 48
+    T = 22
+    dx = 1e-3
+    for name, kw in (
+        ("debye", dict(eps_inf=4.0, debye_poles=[DebyePole(delta_eps=35.0, tau=8e-12)])),
+        ("lorentz", dict(eps_inf=3.0, debye_poles=[LorentzPole(delta_eps=5.0,
+                                                               omega_0=2 * np.pi * 5e9,
+                                                               delta=1e9)])),
+        ("debye_lorentz", dict(eps_inf=4.0, debye_poles=[
+            DebyePole(delta_eps=35.0, tau=8e-12),
+            LorentzPole(delta_eps=5.0, omega_0=2 * np.pi * 5e9, delta=1e9)])),
+    ):
+        cfg = SimulationConfig(grid=GridConfig(nx=nx, ny=ny, dx=dx, dy=dx), courant=0.5,
+                               total_steps=T)
+        src = ModulatedGaussianSource(x=24, y=24, frequency=20e9, bandwidth=150e9,
+                                      amplitude=1.0, delay_factor=2.0)
+        sim = FDTD2D(config=cfg, sources=[src], boundaries=[],
+                     probe_points=[(24, 24), (27, 22), (20, 30)])
+        sim.grid.set_material_circle(25, 23, 9, eps_r=kw["eps_inf"], sigma_e=0.7, **kw)
+        sim.run(T)
+        tr = np.array([sim.probe_data[p] for p in sim.probe_points], dtype=np.float64)
+        np.savez_compressed(os.path.join(HERE, f"ade2d_{name}.npz"), traces=tr, dt=cfg.dt,
+                 waveform=src.waveform(T, cfg.dt), eps_inf=np.asarray(sim.grid.eps_inf),
+                 sigma_e=np.asarray(sim.grid.sigma_e),
+                 delta_eps=np.asarray(sim.grid.dispersive.delta_eps[:2]),
+                 tau=np.asarray(sim.grid.dispersive.tau[:2]),
+                 omega_0=np.asarray(sim.grid.dispersive.omega_0[:2]),
+                 damping=np.asarray(sim.grid.dispersive.damping[:2]),
+                 pole_types=np.array(sim.grid.dispersive.pole_types[:2]),
+                 probes=np.array(sim.probe_points), source=np.array([24, 24]),
+                 nx=nx, ny=ny, dx=dx)
+
+
+if __name__ == "__main__":
+    golden_ceep_results()
+    golden_bf2d()
+    golden_bf3d()
+    golden_ade2d()
+    print("golden vectors written to", HERE)

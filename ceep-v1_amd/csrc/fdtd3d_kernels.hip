@@ -49,16 +49,18 @@ __global__ __launch_bounds__(256) void h3d_kernel(Step3DArgs p) {
   }
 }
 
+// E update of one component at one cell: curl is the undivided difference
+// (times 1/dx here), cb the FP32 (dt/eps)/(1+s) map value.
 template <int POLES>
 __device__ __forceinline__ float ade_update(const Step3DArgs& p, int comp, long long o,
                                             float e0, float ca, float cb, float curl,
                                             float dk1, float dk2, float lk1, float lk2,
                                             float lk3) {
-  if (POLES == 0) return ca * e0 + cb * curl;
+  if (POLES == 0) return ca * e0 + cb * (curl * p.idx);
   float jd = 0.f, jl = 0.f, jlp = 0.f, sgj = 0.f;
   if (POLES & 1) { jd = p.jd[comp][o]; sgj += 0.5f * (1.f + dk1) * jd; }
   if (POLES & 2) { jl = p.jl_cur[comp][o]; jlp = p.jl_prev[comp][o]; sgj += jl; }
-  const float en = ca * e0 + cb * curl - cb * sgj;
+  const float en = ca * e0 + cb * (curl * p.idx) - cb * sgj;
   const float de = en - e0;
   if (POLES & 1) p.jd[comp][o] = dk1 * jd + dk2 * de;
   if (POLES & 2) p.jl_prev[comp][o] = lk1 * jl + lk3 * jlp + lk2 * de;
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(256) void e3d_kernel(Step3DArgs p) {
   float ca = 0.f, cb = 0.f, dk1 = 0.f, dk2 = 0.f, lk1 = 0.f, lk2 = 0.f, lk3 = 0.f;
   if (interior) {
     ca = p.ca[cell];
-    cb = p.cb[cell] * p.idx;
+    cb = p.cb[cell];
     if (POLES & 1) { dk1 = p.d_k1[cell]; dk2 = p.d_k2dt[cell]; }
     if (POLES & 2) { lk1 = p.l_k1[cell]; lk2 = p.l_k2dt[cell]; lk3 = p.l_k3[cell]; }
   }

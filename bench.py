@@ -26,6 +26,13 @@ for _p in (ROOT, os.path.join(ROOT, "ceep-v1_amd")):
 
 import numpy as np  # noqa: E402
 
+try:
+    # torch first: libceep_fdtd.so then binds to torch's HIP runtime (same
+    # SONAME) instead of loading a second one, and torch.cuda stays usable
+    import torch  # noqa: E402,F401
+except Exception:  # pragma: no cover - torch is optional for the bench
+    torch = None
+
 NX = NY = 600
 DX = 0.5e-3
 N_CPML = 15
@@ -38,7 +45,6 @@ HBM_PEAK_GBS = 8000.0
 # TX) and the shared ca, cb, k1, k2 maps read once per launch (16 B per cell)
 BYTES_PER_CELL_PER_TX = 32
 BYTES_PER_CELL_SHARED = 16
-KERNEL = "step2d_kernel<1, 4>"   # Debye pole, 4 TX planes per thread
 
 
 def _workload():
@@ -59,13 +65,11 @@ def _workload():
 
 
 def _device_sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-            return "torch.cuda.synchronize"
-    except Exception:
-        pass
+    # every library call already returns after its stream has drained; the
+    # device-wide synchronize is the contract's barrier on top of that
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.synchronize()
+        return "torch.cuda.synchronize"
     return "library-synchronous"
 
 
@@ -109,7 +113,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--single-run", action="store_true",
+                    help="one simulation + DFT with the default tuning, no output (for rocprofv3 --pmc)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain device copy of 256 MiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        return
+    if args.single_run:
+        s, *_ = _workload()
+        s._handle.reset()
+        s._handle.run(copy=False)
+        s._handle.dft(FREQS)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
@@ -118,9 +140,22 @@ def main():
         sys.exit(2)
 
     s, ph, pos, w, dt = _workload()
+    tuning = None
+    if not args.no_autotune:
+        from ceep_amd.autotune import autotune_2d
+        tuning = autotune_2d(s, reps=2)
     h = s._handle
     box = h.box_info()
     cells = int(box[0] * box[1])
+    K = (tuning or {}).get("temporal_depth", 0)
+    if K:
+        kernel = f"step2d_tb_kernel<{tuning['rows_per_wave']}, {K}, 1>"
+        launches = -(-T // K)
+        steps_per_launch = K
+    else:
+        kernel = f"step2d_kernel<1, {(tuning or {}).get('batch_per_thread', 4)}>"
+        launches = T
+        steps_per_launch = 1
 
     def step():
         h.reset()
@@ -140,8 +175,11 @@ def main():
 
     # kernel-only timing with HIP events on the library's stream
     ms_run = h.time_runs(3)
-    ms_launch = ms_run / T
-    bytes_launch = cells * (B * BYTES_PER_CELL_PER_TX + BYTES_PER_CELL_SHARED)
+    ms_launch = ms_run / launches
+    # algorithmic bytes: every time step reads and writes the state once, so
+    # a K-step launch is credited K steps' worth (this exceeds the peak when
+    # the blocking saves traffic; `traffic` is the measured HBM side)
+    bytes_launch = cells * (B * BYTES_PER_CELL_PER_TX + BYTES_PER_CELL_SHARED) * steps_per_launch
     achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
 
     cpu = None
@@ -167,11 +205,12 @@ def main():
                          "seed 0, 301-frequency DFT 0.5-3 GHz to the 16x16 S-matrix"),
             "cells_counted": "stepped box",
             "value_full_grid_convention": round(NX * NY * B * T / (el / args.steps) / 1e9, 3),
-            "batch": B, "time_steps": T, "sync": sync,
+            "batch": B, "time_steps": T, "sync": sync, "tuning": tuning,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": KERNEL,
+            "kernel": kernel,
+            "steps_per_launch": steps_per_launch,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

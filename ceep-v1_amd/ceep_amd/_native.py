@@ -3,6 +3,12 @@
 The library is built in-tree by ``ceep-v1_amd/build.py``.  There is no CPU
 fallback: if the shared library is missing or no HIP device is usable, the
 calls raise.
+
+Using PyTorch in the same process: import torch before the first solver is
+built.  The library links the HIP runtime by its SONAME (libamdhip64.so.7),
+which torch's bundled runtime also carries, so loading torch first makes both
+share torch's runtime; loading this library first makes torch (which names
+the runtime by file name) pull in a second runtime it cannot initialise.
 """
 from __future__ import annotations
 
@@ -104,6 +110,16 @@ def dptr(a: np.ndarray):
 def iptr(a: np.ndarray):
     assert a.dtype == np.int64 and a.flags.c_contiguous
     return a.ctypes.data_as(_PI64)
+
+
+TUNE_BATCH_PER_THREAD = 0
+TUNE_GRAPH = 1
+TUNE_TEMPORAL_DEPTH = 2
+TUNE_ROWS_PER_WAVE = 3
+TUNE_BATCH_PER_WAVE = 4
+TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
+               "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
+               "batch_per_wave": TUNE_BATCH_PER_WAVE}
 
 
 class Handle:
@@ -215,7 +231,13 @@ class Handle:
         return v.value
 
     def set_tuning(self, key, value):
+        if isinstance(key, str):
+            key = TUNING_KEYS[key]
         check(self._lib.ceep_fdtd_set_tuning(self._h, int(key), int(value)))
+
+    def apply_tuning(self, tuning):
+        for k, v in (tuning or {}).items():
+            self.set_tuning(k, v)
 
     def box_info(self):
         out = np.zeros(5, dtype=np.int64)

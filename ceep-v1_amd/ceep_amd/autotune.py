@@ -1,0 +1,96 @@
+"""Launch-parameter choice for the batched kernels, measured on the device.
+
+The 2D path has two kernel families: the one-step kernel (one launch per time
+step) and the temporally blocked kernel (``temporal_depth`` K steps per launch
+on R x 64 register windows, ``batch_per_wave`` TX planes per wavefront).
+``autotune_2d`` times every candidate with HIP events on a solver's own
+handle, keeps the fastest, and records it in a per-process table (and, if
+``CEEP_TUNE_CACHE`` names a file, in that JSON file) keyed by the problem's
+shape, so later solvers of the same shape start with it.  ``tuning_for``
+returns the recorded choice or the built-in default.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from typing import Dict, Optional
+
+_TABLE: Dict[str, dict] = {}
+_LOADED = False
+
+# measured best on MI355X for the 600^2 x 16 Debye workload; used when nothing
+# was tuned for a shape
+DEFAULT_2D = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4,
+              "batch_per_thread": 4, "graph": 1}
+
+
+def _key(ndim, shape, n, batch, debye, lorentz):
+    return f"{ndim}|{'x'.join(map(str, shape))}|n{n}|B{batch}|D{int(debye)}|L{int(lorentz)}"
+
+
+def _cache_path() -> Optional[str]:
+    return os.environ.get("CEEP_TUNE_CACHE")
+
+
+def _load():
+    global _LOADED
+    if _LOADED:
+        return
+    _LOADED = True
+    path = _cache_path()
+    if path and os.path.exists(path):
+        with open(path) as f:
+            _TABLE.update(json.load(f))
+
+
+def tuning_for(ndim, shape, n, batch, debye=False, lorentz=False) -> dict:
+    _load()
+    t = _TABLE.get(_key(ndim, shape, n, batch, debye, lorentz))
+    if t is not None:
+        return dict(t)
+    return dict(DEFAULT_2D) if ndim == 2 else {"batch_per_thread": 4, "graph": 1}
+
+
+def candidates_2d(lorentz=False):
+    out = [{"temporal_depth": 0, "batch_per_thread": bb} for bb in (2, 4, 8)]
+    if not lorentz:
+        for k, r, bpw in itertools.product((1, 2, 3, 4), (16, 24, 32), (1, 2, 4)):
+            if r - 2 * k >= 4:
+                out.append({"temporal_depth": k, "rows_per_wave": r, "batch_per_wave": bpw})
+    return out
+
+
+def autotune_2d(solver, reps: int = 2, candidates=None, verbose: bool = False) -> dict:
+    """Time each candidate on ``solver``'s device state; return and record the best.
+
+    The solver must be built (``_build``); its fields are reset afterwards.
+    """
+    if not solver._built:
+        solver._build()
+    h = solver._handle
+    best, best_ms, log = None, float("inf"), []
+    for cand in candidates or candidates_2d(solver._lorentz is not None):
+        full = dict(DEFAULT_2D)
+        full.update(cand)
+        h.apply_tuning(full)
+        h.time_runs(1)                     # capture + warm
+        ms = h.time_runs(reps)
+        log.append((ms, full))
+        if verbose:
+            print(f"  {ms:9.3f} ms  {full}", flush=True)
+        if ms < best_ms:
+            best, best_ms = full, ms
+    h.apply_tuning(best)
+    h.reset()
+    key = _key(2, (solver.nx, solver.ny), solver.cpml_n, solver.batch,
+               solver._debye is not None, solver._lorentz is not None)
+    _load()
+    _TABLE[key] = best
+    path = _cache_path()
+    if path:
+        with open(path, "w") as f:
+            json.dump(_TABLE, f, indent=1)
+    solver.tuning = best
+    solver.tuning_log = log
+    return best

@@ -72,6 +72,9 @@ class BatchedFDTD2D:
         self._handle = None
         self.xp = np
         self.probe_data = None
+        # kernel launch parameters (see autotune.py); None picks the tuned or
+        # built-in defaults for this shape
+        self.tuning = None
 
     # -- materials ------------------------------------------------------------------
     def set_material_circle(self, center_x: int, center_y: int, radius: int,
@@ -161,6 +164,10 @@ class BatchedFDTD2D:
         if self._lorentz is not None:
             h.set_lorentz(*self._lorentz)
         h.set_materials(self._eps_r, self._sigma_e)
+        from .autotune import tuning_for
+        h.apply_tuning(self.tuning if self.tuning is not None else
+                       tuning_for(2, (self.nx, self.ny), self.cpml_n, self.batch,
+                                  self._debye is not None, self._lorentz is not None))
         return h
 
     def _build(self) -> None:

@@ -39,3 +39,41 @@ struct Step2DArgs {
 hipError_t launch_step2d(const Step2DArgs& a, int poles, int bb, hipStream_t s);
 
 }  // namespace ceep
+
+namespace ceep {
+
+// Temporal blocking: one wavefront holds an R x 64 window (rows = x, lanes =
+// y) of the state in registers and advances it ksteps <= K steps; the output
+// window (rows [K, R-K), lanes [K, 64-K)) is written back.  All state,
+// including the Debye current, is ping-ponged.
+struct TB2DArgs {
+  int Mx, My, pitch, B;
+  long long bstride;
+  const float* ez_in;
+  const float* hx_in;
+  const float* hy_in;
+  const float* jd_in;
+  float* ez_out;
+  float* hx_out;
+  float* hy_out;
+  float* jd_out;
+  const float* ca;                // zero outside the E interior
+  const float* cb;
+  const float* d_k1;
+  const float* d_k2dt;
+  float ch, idx;
+  const int* src_cell;            // [B] a * pitch + c
+  const float* wave;
+  long long wave_stride_b;
+  int step0, ksteps, T, P;
+  const int* probe_cell;          // [P] a * pitch + c, or -1 outside the box
+  float* traces;
+  int tiles_x, tiles_y;           // output tiles over rows / columns
+  int bpw;                        // batch planes per wavefront
+  int ntasks;                     // tiles_x * tiles_y * ceil(B / bpw)
+};
+
+// poles: 0 or 1 (Debye); R in {16, 24, 32}; K in {1, 2, 3, 4}.
+hipError_t launch_step2d_tb(const TB2DArgs& a, int poles, int R, int K, hipStream_t s);
+
+}  // namespace ceep

@@ -1,0 +1,187 @@
+// Temporally blocked batched 2D TMz step kernel for gfx950.
+//
+// Same update as step2d_kernel (fdtd_2d_batched.py:472-498, 509-526 with the
+// ADE of fdtd_2d.py:163-178), but one launch advances the state by up to K
+// steps.  A wavefront owns an R x 64 window of one batch plane: lane l holds
+// column c = c0 - K + l, register slot r holds row a = a0 - K + r.  The
+// y-neighbours come from the adjacent lanes through DPP wave shifts, the
+// x-neighbours are the adjacent register slots, so a step needs neither LDS
+// nor a barrier.  Every step the window's outer ring becomes stale (its
+// neighbours are outside the window), so after K steps the inner
+// (R - 2K) x (64 - 2K) cells are exact and are the only ones written back;
+// neighbouring wavefronts recompute the overlap.  Traffic per cell-step drops
+// from one read + one write of the state to about (R*64/((R-2K)(64-2K)) + 1)/K.
+//
+// Cells outside the E interior carry ca = cb = k1 = k2 = 0 and cells outside
+// the box are loaded as 0, so the uniform update leaves them at zero (before
+// a wall source is added, as in run_cpu) without per-cell masks.
+#include "fdtd_common.h"
+#include "fdtd2d_kernels.h"
+
+namespace ceep {
+namespace {
+
+// lane l receives lane l+1 (0 for lane 63): the value at column c+1
+__device__ __forceinline__ float from_next_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+// lane l receives lane l-1 (0 for lane 0): the value at column c-1
+__device__ __forceinline__ float from_prev_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+
+}  // namespace
+
+template <int R, int K, int POLES>
+__global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
+  const int lane = threadIdx.x & 63;
+  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (task >= p.ntasks) return;                       // uniform per wavefront
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = task % tiles;
+  const int chunk = task / tiles;
+  const int ty = tile % p.tiles_y;
+  const int tx = tile / p.tiles_y;
+  const int a0 = tx * (R - 2 * K) - K;                // row of register slot 0
+  const int cb0 = ty * (64 - 2 * K) - K;              // column of lane 0
+  const int c = cb0 + lane;
+  const bool col_ok = c >= 0 && c < p.My;
+
+  float mca[R], mcb[R], mk1[R], mk2[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int a = a0 + r;
+    const bool in = col_ok && a >= 0 && a < p.Mx;
+    const long long cell = (long long)a * p.pitch + c;
+    mca[r] = in ? p.ca[cell] : 0.f;
+    mcb[r] = in ? p.cb[cell] : 0.f;
+    if (POLES & 1) {
+      mk1[r] = in ? p.d_k1[cell] : 0.f;
+      mk2[r] = in ? p.d_k2dt[cell] : 0.f;
+    }
+  }
+  const float ch = p.ch, idx = p.idx;
+  const bool out_lane = lane >= K && lane < 64 - K && col_ok;
+
+  const int b_begin = chunk * p.bpw;
+  const int b_end = min(b_begin + p.bpw, p.B);
+  for (int bt = b_begin; bt < b_end; ++bt) {
+    const long long base = (long long)bt * p.bstride;
+    float ez[R], hx[R], hy[R], jd[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int a = a0 + r;
+      const bool in = col_ok && a >= 0 && a < p.Mx;
+      const long long o = base + (long long)a * p.pitch + c;
+      ez[r] = in ? p.ez_in[o] : 0.f;
+      hx[r] = in ? p.hx_in[o] : 0.f;
+      hy[r] = in ? p.hy_in[o] : 0.f;
+      if (POLES & 1) jd[r] = in ? p.jd_in[o] : 0.f;
+    }
+    // this TX's source, if it falls inside the window (halo included)
+    const int sc = p.src_cell[bt];
+    const int s_row = sc / p.pitch - a0;
+    const int s_lane = sc % p.pitch - cb0;
+    const bool src_here = s_row >= 0 && s_row < R && s_lane >= 0 && s_lane < 64;
+
+    for (int s = 0; s < K; ++s) {
+      if (s >= p.ksteps) break;
+      const int step = p.step0 + s;
+      // H half step (run_cpu form; the stale outer ring reads zeros)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float eR = from_next_lane(ez[r]);
+        const float eD = (r + 1 < R) ? ez[r + 1] : 0.f;
+        hx[r] = hx[r] - ch * (eR - ez[r]);
+        hy[r] = hy[r] + ch * (eD - ez[r]);
+      }
+      // E step (+ Debye ADE)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float hxl = from_prev_lane(hx[r]);
+        const float hyu = (r > 0) ? hy[r - 1] : 0.f;
+        const float curl = (hy[r] - hyu) - (hx[r] - hxl);
+        const float e0 = ez[r];
+        if (POLES == 0) {
+          ez[r] = mca[r] * e0 + mcb[r] * (curl * idx);
+        } else {
+          const float sgj = 0.5f * (1.f + mk1[r]) * jd[r];
+          const float en = mca[r] * e0 + mcb[r] * (curl * idx) - mcb[r] * sgj;
+          jd[r] = mk1[r] * jd[r] + mk2[r] * (en - e0);
+          ez[r] = en;
+        }
+      }
+      if (src_here) {
+        const float w = p.wave[(long long)bt * p.wave_stride_b + step];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r == s_row && lane == s_lane) ez[r] += w;
+      }
+      // probes inside this wavefront's output window
+      for (int q0 = 0; q0 < p.P; q0 += 64) {
+        const int q = q0 + lane;
+        const int pc = q < p.P ? p.probe_cell[q] : -1;
+        int pr = -1, pl = -1;
+        if (pc >= 0) {
+          pr = pc / p.pitch - a0;
+          pl = pc % p.pitch - cb0;
+        }
+        const bool mine = pc >= 0 && pr >= K && pr < R - K && pl >= K && pl < 64 - K;
+        unsigned long long m = __ballot(mine);
+        while (m) {
+          const int l = __ffsll((long long)m) - 1;
+          m &= m - 1;
+          const int qr = __shfl(pr, l);
+          const int ql = __shfl(pl, l);
+          float v = 0.f;
+#pragma unroll
+          for (int r = 0; r < R; ++r) v = (r == qr) ? ez[r] : v;
+          if (lane == ql) p.traces[((long long)bt * p.P + q0 + l) * p.T + step] = v;
+        }
+      }
+    }
+    // write back the exact inner window
+#pragma unroll
+    for (int r = K; r < R - K; ++r) {
+      const int a = a0 + r;
+      if (out_lane && a < p.Mx) {
+        const long long o = base + (long long)a * p.pitch + c;
+        p.ez_out[o] = ez[r];
+        p.hx_out[o] = hx[r];
+        p.hy_out[o] = hy[r];
+        if (POLES & 1) p.jd_out[o] = jd[r];
+      }
+    }
+  }
+}
+
+template <int R, int K>
+static hipError_t launch_rk(const TB2DArgs& a, int poles, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((a.ntasks + 3) / 4, 1, 1);
+  if (poles == 0) hipLaunchKernelGGL((step2d_tb_kernel<R, K, 0>), grd, blk, 0, s, a);
+  else if (poles == 1) hipLaunchKernelGGL((step2d_tb_kernel<R, K, 1>), grd, blk, 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int R>
+static hipError_t launch_r(const TB2DArgs& a, int poles, int K, hipStream_t s) {
+  switch (K) {
+    case 1: return launch_rk<R, 1>(a, poles, s);
+    case 2: return launch_rk<R, 2>(a, poles, s);
+    case 3: return launch_rk<R, 3>(a, poles, s);
+    case 4: return launch_rk<R, 4>(a, poles, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_step2d_tb(const TB2DArgs& a, int poles, int R, int K, hipStream_t s) {
+  switch (R) {
+    case 16: return launch_r<16>(a, poles, K, s);
+    case 24: return launch_r<24>(a, poles, K, s);
+    case 32: return launch_r<32>(a, poles, K, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace ceep

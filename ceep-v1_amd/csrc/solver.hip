@@ -91,6 +91,7 @@ struct ceep_fdtd {
   long long wave_stride = 0;
   short* probe_head = nullptr;
   short* probe_next = nullptr;
+  int* probe_cell = nullptr;   // [P] 2D box-linear cell of each probe, -1 outside
   float* traces = nullptr;
   float* partial = nullptr;
   float2* tw = nullptr;        // DFT twiddles [T][F] for tw_freqs
@@ -104,10 +105,16 @@ struct ceep_fdtd {
   bool has_run = false;
   int bb = 4;
   int use_graph = 1;
+  int tb_k = 0;                // 2D temporal depth (0: one step per launch)
+  int tb_r = 16;               // 2D rows per wavefront window
+  int tb_bpw = 1;              // 2D batch planes per wavefront
   float ch = 0.f, idx = 0.f;
 
   long long box_floats() const { return (long long)B * plane; }
   int poles() const { return (has_debye ? 1 : 0) | (has_lorentz ? 2 : 0); }
+  // the temporally blocked 2D kernel covers the plain and Debye cases
+  bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz; }
+  int launches_per_run() const { return use_tb() ? (T + tb_k - 1) / tb_k : T; }
   long long full_index(int64_t x, int64_t y, int64_t z) const {
     return ndim == 2 ? x * N[1] + y : (x * N[1] + y) * N[2] + z;
   }
@@ -176,8 +183,16 @@ int commit_materials(ceep_fdtd* h) {
         double eps = h->eps_r[g] * kEps0;
         if (h->has_debye || h->has_lorentz) eps = eps + eps_add;
         const double s = h->sigma_e[g] * dt / (2.0 * eps);
-        ca[o] = (float)((1.0 - s) / (1.0 + s));
-        cb[o] = (float)((dt / eps) / (1.0 + s));
+        // E is stepped on the interior only; elsewhere every E-side
+        // coefficient is 0 so the kernels' uniform update keeps E at 0
+        const bool interior = a >= 1 && a <= h->M[0] - 2 && b >= 1 && b <= h->M[1] - 2 &&
+                              (h->ndim == 2 || (c >= 1 && c <= mz - 2));
+        ca[o] = interior ? (float)((1.0 - s) / (1.0 + s)) : 0.f;
+        cb[o] = interior ? (float)((dt / eps) / (1.0 + s)) : 0.f;
+        if (!interior) {
+          if (h->has_debye) { dk1[o] = 0.f; dk2[o] = 0.f; }
+          if (h->has_lorentz) { lk1[o] = 0.f; lk2[o] = 0.f; lk3[o] = 0.f; }
+        }
         if (!da.empty()) {
           const double mu = h->mu_r[g] * kMu0;
           const double sm = h->sigma_m[g] * dt / (2.0 * mu);
@@ -219,6 +234,10 @@ int commit_materials(ceep_fdtd* h) {
       if ((rc = dalloc(&h->jd[k], h->box_floats()))) return rc;
       CEEP_HIP(hipMemsetAsync(h->jd[k], 0, jbytes, s));
     }
+    if (h->has_debye && h->ndim == 2) {   // ping-pong partner for the blocked kernel
+      if ((rc = dalloc(&h->jd[1], h->box_floats()))) return rc;
+      CEEP_HIP(hipMemsetAsync(h->jd[1], 0, jbytes, s));
+    }
     if (h->has_lorentz) {
       for (int i = 0; i < 2; ++i) {
         if ((rc = dalloc(&h->jl[i][k], h->box_floats()))) return rc;
@@ -253,13 +272,35 @@ int zero_state(ceep_fdtd* h) {
 int launch_steps(ceep_fdtd* h, int start) {
   hipStream_t s = h->stream;
   const int poles = h->poles();
-  if (h->ndim == 2) {
+  if (h->use_tb()) {
+    ceep::TB2DArgs a{};
+    a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
+    a.ca = h->ca; a.cb = h->cb; a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
+    a.ch = h->ch; a.idx = h->idx;
+    a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+    a.T = h->T; a.P = h->P; a.probe_cell = h->probe_cell; a.traces = h->traces;
+    const int K = h->tb_k, R = h->tb_r;
+    a.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
+    a.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
+    a.bpw = h->tb_bpw;
+    a.ntasks = a.tiles_x * a.tiles_y * ((h->B + a.bpw - 1) / a.bpw);
+    int cur = start;
+    for (int st = 0; st < h->T; st += K) {
+      const int o = cur ^ 1;
+      a.ez_in = h->f[cur][0]; a.hx_in = h->f[cur][1]; a.hy_in = h->f[cur][2];
+      a.ez_out = h->f[o][0]; a.hx_out = h->f[o][1]; a.hy_out = h->f[o][2];
+      a.jd_in = h->jd[cur]; a.jd_out = h->jd[o];
+      a.step0 = st;
+      a.ksteps = std::min(K, h->T - st);
+      CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
+      cur = o;
+    }
+  } else if (h->ndim == 2) {
     ceep::Step2DArgs a{};
     a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
     a.ca = h->ca; a.cb = h->cb;
     a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
     a.l_k1 = h->lk1; a.l_k2dt = h->lk2; a.l_k3 = h->lk3;
-    a.jd = h->jd[0];
     a.ch = h->ch; a.idx = h->idx;
     a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
     a.T = h->T; a.P = h->P;
@@ -269,6 +310,7 @@ int launch_steps(ceep_fdtd* h, int start) {
       a.ez_in = h->f[i][0]; a.hx_in = h->f[i][1]; a.hy_in = h->f[i][2];
       a.ez_out = h->f[o][0]; a.hx_out = h->f[o][1]; a.hy_out = h->f[o][2];
       a.jl_cur = h->jl[i][0]; a.jl_prev = h->jl[o][0];
+      a.jd = h->jd[start];       // updated in place; lives where the run started
       a.step = st;
       CEEP_HIP(ceep::launch_step2d(a, poles, h->bb, s));
     }
@@ -332,7 +374,12 @@ int run_device(ceep_fdtd* h) {
     CEEP_HIP(hipGraphLaunch(h->gexec[start], h->stream));
   }
   if (rc) return rc;
-  h->parity = start ^ (h->T & 1);
+  h->parity = start ^ (h->launches_per_run() & 1);
+  if (h->ndim == 2 && h->has_debye && !h->use_tb() && h->parity != start) {
+    // the per-step kernel updates J in place in jd[start]; keep J with the fields
+    CEEP_HIP(hipMemcpyAsync(h->jd[h->parity], h->jd[start],
+                            (size_t)h->box_floats() * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+  }
   return CEEP_OK;
 }
 
@@ -415,6 +462,7 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
   if (!rc) rc = dalloc(&h->traces, (size_t)batch * (n_probes > 0 ? n_probes : 1) * total_steps);
   if (!rc) rc = dalloc(&h->probe_head, h->plane);
   if (!rc) rc = dalloc(&h->probe_next, n_probes > 0 ? n_probes : 1);
+  if (!rc) rc = dalloc(&h->probe_cell, n_probes > 0 ? n_probes : 1);
   if (!rc) rc = dalloc(&h->partial, 1024);
   if (!rc && ndim == 2) rc = dalloc(&h->src2, batch);
   if (!rc && ndim == 3) rc = dalloc(&h->src3, batch);
@@ -447,7 +495,7 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   for (int k = 0; k < 3; ++k) { dfree(h->jd[k]); dfree(h->jl[0][k]); dfree(h->jl[1][k]); }
   dfree(h->ca); dfree(h->cb); dfree(h->da); dfree(h->dbi);
   dfree(h->dk1); dfree(h->dk2); dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
-  dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next);
+  dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next); dfree(h->probe_cell);
   dfree(h->traces); dfree(h->partial); dfree(h->tw); dfree(h->S);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -564,6 +612,15 @@ int ceep_fdtd_set_probes(ceep_fdtd* h, const int64_t* pos) {
     head[o] = (short)q;
   }
   h->prb.assign(pos, pos + (size_t)h->P * h->ndim);
+  if (h->ndim == 2 && h->P > 0) {
+    std::vector<int> pc(h->P, -1);
+    for (int q = 0; q < h->P; ++q) {
+      const int64_t* p = pos + (size_t)q * 2;
+      if (in_box(h, p, false))
+        pc[q] = (int)h->box_index((int)(p[0] - h->org), (int)(p[1] - h->org), 0);
+    }
+    CEEP_HIP(hipMemcpyAsync(h->probe_cell, pc.data(), pc.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  }
   CEEP_HIP(hipMemcpyAsync(h->probe_head, head.data(), head.size() * sizeof(short), hipMemcpyHostToDevice, h->stream));
   CEEP_HIP(hipMemcpyAsync(h->probe_next, next.data(), next.size() * sizeof(short), hipMemcpyHostToDevice, h->stream));
   CEEP_HIP(hipStreamSynchronize(h->stream));
@@ -706,6 +763,16 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     h->bb = value;
   } else if (key == 1) {
     h->use_graph = value ? 1 : 0;
+  } else if (key == 2) {
+    if (value < 0 || value > 4) return fail(CEEP_EINVAL, "temporal depth must be in [0, 4]");
+    h->tb_k = value;
+  } else if (key == 3) {
+    if (value != 16 && value != 24 && value != 32)
+      return fail(CEEP_EINVAL, "rows per wavefront must be 16, 24 or 32");
+    h->tb_r = value;
+  } else if (key == 4) {
+    if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
+    h->tb_bpw = value;
   } else {
     return fail(CEEP_EINVAL, "unknown tuning key");
   }

@@ -122,8 +122,12 @@ int ceep_fdtd_energy(ceep_fdtd* h, double* out);
  * mean elapsed ms per run.  Leaves the state of the last run. */
 int ceep_fdtd_time_runs(ceep_fdtd* h, int reps, double* ms_per_run);
 
-/* Tuning knobs: key 0 = batch planes per thread (1,2,4,8), key 1 = use a
- * captured hipGraph for the time loop (0/1). */
+/* Tuning knobs (results are the same to FP32 rounding for every setting):
+ *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8)
+ *   key 1: replay the time loop as a captured hipGraph (0/1)
+ *   key 2: 2D temporal depth K = steps per launch (0 = one-step kernel, 1..4)
+ *   key 3: 2D rows per wavefront window of the blocked kernel (16, 24, 32)
+ *   key 4: 2D batch planes per wavefront of the blocked kernel (>= 1) */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* Geometry of the stepped box: out[0..2] = box size per axis, out[3] =

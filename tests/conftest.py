@@ -10,6 +10,13 @@ for p in (ROOT, os.path.join(ROOT, "ceep-v1_amd")):
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+try:
+    # import torch before libceep_fdtd.so is loaded so both share one HIP
+    # runtime (torch's libs name the runtime by file name, ours by SONAME)
+    import torch  # noqa: F401
+except Exception:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built library")

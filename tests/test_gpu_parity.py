@@ -129,16 +129,25 @@ def test_2d_batch_independence_bitwise(amd):
 
 def test_2d_tuning_variants_bitwise(amd):
     s, _ = _config1(amd, 4, T=150)
+    s.tuning = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4, "graph": 1}
     s._build()
     ref = s.run()
     base = s.probe_data.copy()
-    for bb in (1, 2, 8):
+    for bb in (1, 2, 8):        # one-step kernel: planes per thread, graph on/off
         for graph in (0, 1):
+            s.tuning = {"temporal_depth": 0, "batch_per_thread": bb, "graph": graph}
             s._build()
-            s._handle.set_tuning(0, bb)
-            s._handle.set_tuning(1, graph)
             s.run()
-            assert np.array_equal(s.probe_data, base), (bb, graph)
+            if bb == 1 and graph == 0:
+                one_step = s.probe_data.copy()
+            assert np.array_equal(s.probe_data, one_step), (bb, graph)
+    for graph in (0, 1):          # blocked kernel: graph on/off
+        s.tuning = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4,
+                    "graph": graph}
+        s._build()
+        s.run()
+        assert np.array_equal(s.probe_data, base), graph
+    assert rel_l2(base, one_step) < 1e-6
     assert ref[0][0] is not None
 
 

@@ -54,6 +54,8 @@ SIGNATURES = {
     "ceep_fdtd_get_field": (_I, [_P, _I, _PD]),
     "ceep_fdtd_energy": (_I, [_P, _PD]),
     "ceep_fdtd_time_runs": (_I, [_P, _I, _PD]),
+    "ceep_fdtd_run_checkpointed": (_I, [_P, _I, _PD]),
+    "ceep_fdtd_adjoint": (_I, [_P, _PD, _PD, _PD]),
     "ceep_fdtd_set_tuning": (_I, [_P, _I, _I]),
     "ceep_fdtd_box_info": (_I, [_P, _PI64]),
 }
@@ -203,6 +205,20 @@ class Handle:
         out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)
         check(self._lib.ceep_fdtd_run(self._h, dptr(out)))
         return out
+
+    def run_checkpointed(self, every):
+        out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)
+        check(self._lib.ceep_fdtd_run_checkpointed(self._h, int(every), dptr(out)))
+        return out
+
+    def adjoint(self, dtraces):
+        g = np.ascontiguousarray(np.asarray(dtraces, dtype=np.float64))
+        if g.shape != (self.batch, self.n_probes, self.total_steps):
+            raise ValueError(f"dL/dtrace must have shape {(self.batch, self.n_probes, self.total_steps)}")
+        ge = np.zeros(self.shape, dtype=np.float64)
+        gs = np.zeros(self.shape, dtype=np.float64)
+        check(self._lib.ceep_fdtd_adjoint(self._h, dptr(g), dptr(ge), dptr(gs)))
+        return ge, gs
 
     def traces(self):
         out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)

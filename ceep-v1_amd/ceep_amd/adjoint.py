@@ -1,0 +1,70 @@
+"""Gradients of S-parameter misfits with respect to the eps_r and sigma maps.
+
+``BatchedTraces`` is a ``torch.autograd.Function`` whose forward runs the
+batched 2D solver on the GPU (checkpointing the full state every ``every``
+steps) and whose backward runs the hand-written time-reversed adjoint kernel
+(``ceep_fdtd_adjoint``: each segment is recomputed from its checkpoint, then
+stepped backward) to return dL/d eps_r and dL/d sigma per cell.  Everything
+downstream of the traces -- the running DFT to the S-matrix
+(src/ceep/solvers/dft.py:75-90, layout of s_params.py:99) and any loss -- is
+ordinary torch on the host, so autograd supplies dL/dtrace.
+
+There is no counterpart in the reference (src/ceep/ai/__init__.py is a
+docstring; "Differentiable FDTD" is an unchecked roadmap item).
+
+Import torch before the first solver is built (see ``_native``).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class BatchedTraces(torch.autograd.Function):
+    """traces = F(eps_r, sigma) for a built-or-buildable ``BatchedFDTD2D``."""
+
+    @staticmethod
+    def forward(ctx, eps_r, sigma_e, solver, every):
+        solver._eps_r[:] = eps_r.detach().cpu().numpy()
+        solver._sigma_e[:] = sigma_e.detach().cpu().numpy()
+        solver._build()
+        tr = solver._handle.run_checkpointed(int(every))
+        solver.probe_data = tr
+        ctx.solver = solver
+        return torch.from_numpy(tr.copy())
+
+    @staticmethod
+    def backward(ctx, g):
+        ge, gs = ctx.solver._handle.adjoint(g.detach().cpu().numpy())
+        return torch.from_numpy(ge), torch.from_numpy(gs), None, None
+
+
+def traces(solver, eps_r: torch.Tensor, sigma_e: torch.Tensor, every: int = 50) -> torch.Tensor:
+    """(B, P, T) float64 traces, differentiable in eps_r and sigma_e ((nx, ny) float64)."""
+    return BatchedTraces.apply(eps_r, sigma_e, solver, every)
+
+
+def s_matrix(traces_: torch.Tensor, freqs, dt: float) -> torch.Tensor:
+    """(F, RX, TX) complex S-matrix: dt * sum_s trace[tx, rx, s] exp(-i w_f s dt)."""
+    T = traces_.shape[-1]
+    f = torch.as_tensor(np.asarray(freqs, dtype=np.float64))
+    ph = 2.0 * np.pi * f[:, None] * (torch.arange(T, dtype=torch.float64) * dt)[None, :]
+    re = torch.einsum("bpt,ft->fpb", traces_, torch.cos(ph)) * dt
+    im = torch.einsum("bpt,ft->fpb", traces_, -torch.sin(ph)) * dt
+    return torch.complex(re, im)
+
+
+def s_misfit(traces_: torch.Tensor, freqs, dt: float, s_target: torch.Tensor) -> torch.Tensor:
+    """L = sum over (f, rx, tx) of |S - S_target|^2."""
+    d = s_matrix(traces_, freqs, dt) - s_target
+    return (d.real ** 2 + d.imag ** 2).sum()
+
+
+def gradient(solver, eps_r, sigma_e, freqs, s_target, every: int = 50):
+    """(L, dL/d eps_r, dL/d sigma) as NumPy for the S misfit at ``freqs``."""
+    e = torch.as_tensor(np.asarray(eps_r, dtype=np.float64)).clone().requires_grad_(True)
+    s = torch.as_tensor(np.asarray(sigma_e, dtype=np.float64)).clone().requires_grad_(True)
+    tr = traces(solver, e, s, every)
+    loss = s_misfit(tr, freqs, solver.dt, torch.as_tensor(s_target))
+    loss.backward()
+    return float(loss), e.grad.numpy(), s.grad.numpy()

@@ -32,7 +32,45 @@ struct Step2DArgs {
   const short* probe_head;       // per box cell: first probe id here, or -1
   const short* probe_next;       // [P]: next probe id on the same cell, or -1
   float* traces;                 // [B][P][T]
+  float* e_save;                 // if set: E after this step, [B][plane] (adjoint store)
 };
+
+// One adjoint (time-reversed) step of the non-dispersive 2D update.  With
+// q = cbi * lam_e (cbi = cb / dx, zero off the E interior):
+//   mu_x' = mu_x + q(a, c+1) - q(a, c)      mu_y' = mu_y + q(a, c) - q(a+1, c)
+//   lam_e' = ca lam_e + ch [(mu_y'(a-1) - mu_y'(a)) - (mu_x'(c-1) - mu_x'(c))]
+//            + dL/dtrace(step-1) at the probes
+// and, summed over the batch, g_ca += lam_e e^s,
+// g_cbi += lam_e (e^{s+1} - ca e^s - source(s)).
+struct Adj2DArgs {
+  int Mx, My, pitch, B;
+  long long bstride;
+  const float* le_in;
+  const float* mx_in;
+  const float* my_in;
+  float* le_out;
+  float* mx_out;
+  float* my_out;
+  const float* ca;
+  const float* cb;
+  float ch, idx;
+  const float* e_s;              // forward E at step s   [B][plane]
+  const float* e_s1;             // forward E at step s+1 [B][plane]
+  const int* src_cell;
+  const float* wave;
+  long long wave_stride_b;
+  int step, T, P;                // step = s (this launch computes lam_e^s)
+  const float* dtr;              // [B][P][T] dL/dtrace
+  const short* probe_head;
+  const short* probe_next;
+  float* g_ca;                   // [plane] accumulated over steps and TX
+  float* g_cbi;
+};
+
+hipError_t launch_adj2d(const Adj2DArgs& a, hipStream_t s);
+// lam_e[b][probe q] += dtr[b][q][step] for every TX and probe
+hipError_t launch_adj_seed(float* le, const int* probe_cell, const float* dtr, int B, int P,
+                           int T, int step, long long bstride, hipStream_t s);
 
 // poles: bit 0 = Debye pole present, bit 1 = Lorentz pole present.
 // bb: batch planes per thread (1, 2, 4, 8).

@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void step2d_kernel(Step2DArgs p) {
     }
     if (cell == p.src_cell[bt]) en += p.wave[(long long)bt * p.wave_stride_b + p.step];
     p.ez_out[o] = en;
+    if (p.e_save) p.e_save[o] = en;
     for (int q = probe0; q >= 0; q = p.probe_next[q])
       p.traces[((long long)bt * p.P + q) * p.T + p.step] = en;
   }

@@ -101,6 +101,17 @@ struct ceep_fdtd {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one per starting buffer parity
   int parity = 0;              // ping-pong buffer holding the current state
 
+  // adjoint: forward checkpoints every ck_every steps, e store of one segment,
+  // adjoint state (ping-pong), gradient sums
+  int ck_every = 0, ck_count = 0;
+  std::vector<float*> ck;      // ck_count x 3 fields (ez, hx, hy) [B][plane]
+  float* estore = nullptr;     // (ck_every + 1) x [B][plane]
+  float* adj[2][3] = {{nullptr}};
+  float* g_ca = nullptr;
+  float* g_cbi = nullptr;
+  float* d_dtr = nullptr;
+  bool ck_valid = false;
+
   bool mats_dirty = true;
   bool has_run = false;
   int bb = 4;
@@ -121,6 +132,20 @@ struct ceep_fdtd {
   long long box_index(int a, int b, int c) const {
     return ndim == 2 ? (long long)a * pitch + b
                      : (long long)a * row_block + (long long)b * pitch + c;
+  }
+  void free_adjoint() {
+    for (auto& p : ck) if (p) (void)hipFree(p);
+    ck.clear();
+    ck_count = 0;
+    ck_valid = false;
+    if (estore) (void)hipFree(estore);
+    estore = nullptr;
+    for (auto& row : adj)
+      for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
+    if (g_ca) (void)hipFree(g_ca);
+    if (g_cbi) (void)hipFree(g_cbi);
+    if (d_dtr) (void)hipFree(d_dtr);
+    g_ca = g_cbi = d_dtr = nullptr;
   }
   void drop_graph() {
     for (auto& g : gexec) {
@@ -268,11 +293,21 @@ int zero_state(ceep_fdtd* h) {
   return CEEP_OK;
 }
 
+int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save);
+
 // Enqueue total_steps steps starting from ping-pong buffer `start`.
 int launch_steps(ceep_fdtd* h, int start) {
+  return launch_range(h, start, 0, h->T, nullptr);
+}
+
+// Enqueue steps [step0, step0 + nsteps) from ping-pong buffer `start`.  With
+// e_save (2D one-step kernel only), E after step s is written to slot
+// s + 1 - step0 of the [slot][B][plane] store.
+int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) {
+  const int step_end = step0 + nsteps;
   hipStream_t s = h->stream;
   const int poles = h->poles();
-  if (h->use_tb()) {
+  if (h->use_tb() && !e_save) {
     ceep::TB2DArgs a{};
     a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
     a.ca = h->ca; a.cb = h->cb; a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
@@ -285,13 +320,13 @@ int launch_steps(ceep_fdtd* h, int start) {
     a.bpw = h->tb_bpw;
     a.ntasks = a.tiles_x * a.tiles_y * ((h->B + a.bpw - 1) / a.bpw);
     int cur = start;
-    for (int st = 0; st < h->T; st += K) {
+    for (int st = step0; st < step_end; st += K) {
       const int o = cur ^ 1;
       a.ez_in = h->f[cur][0]; a.hx_in = h->f[cur][1]; a.hy_in = h->f[cur][2];
       a.ez_out = h->f[o][0]; a.hx_out = h->f[o][1]; a.hy_out = h->f[o][2];
       a.jd_in = h->jd[cur]; a.jd_out = h->jd[o];
       a.step0 = st;
-      a.ksteps = std::min(K, h->T - st);
+      a.ksteps = std::min(K, step_end - st);
       CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
       cur = o;
     }
@@ -305,13 +340,14 @@ int launch_steps(ceep_fdtd* h, int start) {
     a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
     a.T = h->T; a.P = h->P;
     a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
-    for (int st = 0; st < h->T; ++st) {
-      const int i = (start + st) & 1, o = i ^ 1;
+    for (int st = step0; st < step_end; ++st) {
+      const int i = (start + st - step0) & 1, o = i ^ 1;
       a.ez_in = h->f[i][0]; a.hx_in = h->f[i][1]; a.hy_in = h->f[i][2];
       a.ez_out = h->f[o][0]; a.hx_out = h->f[o][1]; a.hy_out = h->f[o][2];
       a.jl_cur = h->jl[i][0]; a.jl_prev = h->jl[o][0];
       a.jd = h->jd[start];       // updated in place; lives where the run started
       a.step = st;
+      a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
       CEEP_HIP(ceep::launch_step2d(a, poles, h->bb, s));
     }
   } else {
@@ -329,8 +365,8 @@ int launch_steps(ceep_fdtd* h, int start) {
     a.T = h->T; a.P = h->P;
     a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
     const int bb = h->bb > 4 ? 4 : h->bb;
-    for (int st = 0; st < h->T; ++st) {
-      const int i = (start + st) & 1, o = i ^ 1;
+    for (int st = step0; st < step_end; ++st) {
+      const int i = (start + st - step0) & 1, o = i ^ 1;
       for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
       a.step = st;
       CEEP_HIP(ceep::launch_h3d(a, bb, s));
@@ -490,6 +526,7 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   if (!h) return CEEP_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   h->drop_graph();
+  h->free_adjoint();
   for (int i = 0; i < 2; ++i)
     for (int k = 0; k < 6; ++k) dfree(h->f[i][k]);
   for (int k = 0; k < 3; ++k) { dfree(h->jd[k]); dfree(h->jl[0][k]); dfree(h->jl[1][k]); }
@@ -752,6 +789,129 @@ int ceep_fdtd_time_runs(ceep_fdtd* h, int reps, double* ms_per_run) {
   if (rc) return rc;
   h->has_run = true;
   *ms_per_run = total / reps;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
+  int rc = ready_check(h);
+  if (rc) return rc;
+  if (h->ndim != 2 || h->has_debye || h->has_lorentz)
+    return fail(CEEP_EUNSUPPORTED, "the adjoint covers 2D non-dispersive media");
+  if (every < 1) return fail(CEEP_EINVAL, "checkpoint interval must be >= 1");
+  h->free_adjoint();
+  const int nck = (h->T + every - 1) / every;
+  const size_t bytes = (size_t)h->box_floats() * sizeof(float);
+  h->ck.assign((size_t)nck * 3, nullptr);
+  for (auto& p : h->ck)
+    if ((rc = dalloc(&p, h->box_floats()))) { h->free_adjoint(); return rc; }
+  h->ck_every = every;
+  h->ck_count = nck;
+  if ((rc = zero_state(h))) return rc;
+  CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), h->stream));
+  int cur = 0;
+  for (int k = 0; k < nck; ++k) {
+    for (int f = 0; f < 3; ++f)
+      CEEP_HIP(hipMemcpyAsync(h->ck[(size_t)k * 3 + f], h->f[cur][f], bytes,
+                              hipMemcpyDeviceToDevice, h->stream));
+    const int s0 = k * every, n = std::min(every, h->T - s0);
+    if ((rc = launch_range(h, cur, s0, n, nullptr))) return rc;
+    const int launches = h->use_tb() ? (n + h->tb_k - 1) / h->tb_k : n;
+    cur ^= launches & 1;
+  }
+  h->parity = cur;
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  h->has_run = true;
+  h->ck_valid = true;
+  if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
+  return CEEP_OK;
+}
+
+int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
+                      double* grad_sigma) {
+  if (!h || !dtraces || !grad_eps_r || !grad_sigma) return fail(CEEP_EINVAL, "null argument");
+  if (!h->ck_valid) return fail(CEEP_ESTATE, "no checkpointed forward run (ceep_fdtd_run_checkpointed)");
+  int rc;
+  const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
+  const size_t nbox = (size_t)h->box_floats(), bytes = nbox * sizeof(float);
+  // dL/dtrace is linear in the adjoint: scale it by a power of two so the
+  // FP32 adjoint fields stay far from the subnormal range, unscale at the end
+  double mx = 0.0;
+  const size_t ntr = (size_t)B * P * T;
+  for (size_t i = 0; i < ntr; ++i) mx = std::max(mx, std::fabs(dtraces[i]));
+  std::memset(grad_eps_r, 0, (size_t)h->full_cells * sizeof(double));
+  std::memset(grad_sigma, 0, (size_t)h->full_cells * sizeof(double));
+  if (mx == 0.0 || P == 0) return CEEP_OK;
+  const double scale = std::ldexp(1.0, -std::ilogb(mx));
+  std::vector<float> dtr(ntr);
+  for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
+  if (!h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * nbox))) return rc;
+  for (int i = 0; i < 2; ++i)
+    for (int f = 0; f < 3; ++f)
+      if (!h->adj[i][f] && (rc = dalloc(&h->adj[i][f], nbox))) return rc;
+  if (!h->g_ca && (rc = dalloc(&h->g_ca, h->plane))) return rc;
+  if (!h->g_cbi && (rc = dalloc(&h->g_cbi, h->plane))) return rc;
+  if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
+  hipStream_t st = h->stream;
+  CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr.data(), ntr * sizeof(float), hipMemcpyHostToDevice, st));
+  for (int f = 0; f < 3; ++f) CEEP_HIP(hipMemsetAsync(h->adj[0][f], 0, bytes, st));
+  CEEP_HIP(hipMemsetAsync(h->g_ca, 0, (size_t)h->plane * sizeof(float), st));
+  CEEP_HIP(hipMemsetAsync(h->g_cbi, 0, (size_t)h->plane * sizeof(float), st));
+  // lam_e^T = dL/dtrace(T-1) at the probes
+  CEEP_HIP(ceep::launch_adj_seed(h->adj[0][0], h->probe_cell, h->d_dtr, B, P, T, T - 1,
+                                 h->plane, st));
+  ceep::Adj2DArgs a{};
+  a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = B; a.bstride = h->plane;
+  a.ca = h->ca; a.cb = h->cb; a.ch = h->ch; a.idx = h->idx;
+  a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+  a.T = T; a.P = P; a.dtr = h->d_dtr;
+  a.probe_head = h->probe_head; a.probe_next = h->probe_next;
+  a.g_ca = h->g_ca; a.g_cbi = h->g_cbi;
+  int acur = 0;
+  for (int k = h->ck_count - 1; k >= 0; --k) {
+    const int s0 = k * every, n = std::min(every, T - s0);
+    // recompute the segment from its checkpoint (one-step kernel, storing E)
+    for (int f = 0; f < 3; ++f)
+      CEEP_HIP(hipMemcpyAsync(h->f[0][f], h->ck[(size_t)k * 3 + f], bytes,
+                              hipMemcpyDeviceToDevice, st));
+    CEEP_HIP(hipMemcpyAsync(h->estore, h->ck[(size_t)k * 3], bytes, hipMemcpyDeviceToDevice, st));
+    if ((rc = launch_range(h, 0, s0, n, h->estore))) return rc;
+    // adjoint steps s = s0 + n - 1 ... s0
+    for (int s = s0 + n - 1; s >= s0; --s) {
+      const int o = acur ^ 1;
+      a.le_in = h->adj[acur][0]; a.mx_in = h->adj[acur][1]; a.my_in = h->adj[acur][2];
+      a.le_out = h->adj[o][0]; a.mx_out = h->adj[o][1]; a.my_out = h->adj[o][2];
+      a.e_s = h->estore + (size_t)(s - s0) * nbox;
+      a.e_s1 = h->estore + (size_t)(s + 1 - s0) * nbox;
+      a.step = s;
+      CEEP_HIP(ceep::launch_adj2d(a, st));
+      acur = o;
+    }
+  }
+  std::vector<float> gca(h->plane), gcb(h->plane);
+  CEEP_HIP(hipMemcpyAsync(gca.data(), h->g_ca, gca.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipMemcpyAsync(gcb.data(), h->g_cbi, gcb.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  // the recomputation overwrote the fields: leave a defined (zero) state
+  if ((rc = zero_state(h))) return rc;
+  CEEP_HIP(hipStreamSynchronize(st));
+  // chain rule to eps_r and sigma in FP64 (ca, cb of fdtd_2d_batched.py:153-157)
+  const double dt = h->dt;
+  for (int x = 1; x < h->M[0] - 1; ++x)
+    for (int y = 1; y < h->M[1] - 1; ++y) {
+      const long long o = h->box_index(x, y, 0);
+      const long long g = h->full_index(x + h->org, y + h->org, 0);
+      const double eps = h->eps_r[g] * kEps0, sg = h->sigma_e[g];
+      const double sv = sg * dt / (2.0 * eps);
+      const double cb = (dt / eps) / (1.0 + sv);
+      const double dL_dca = gca[o] / scale;
+      const double dL_dcb = gcb[o] / scale / cb;   // g_cbi sums lam * cbi * curl
+      const double dca_ds = -2.0 / ((1.0 + sv) * (1.0 + sv));
+      const double ds_deps = -sv / eps, ds_dsig = dt / (2.0 * eps);
+      const double dcb_deps = -(dt / (eps * eps)) / (1.0 + sv) +
+                              (dt / eps) * (-1.0 / ((1.0 + sv) * (1.0 + sv))) * ds_deps;
+      const double dcb_dsig = (dt / eps) * (-1.0 / ((1.0 + sv) * (1.0 + sv))) * ds_dsig;
+      grad_eps_r[g] = kEps0 * (dL_dca * dca_ds * ds_deps + dL_dcb * dcb_deps);
+      grad_sigma[g] = dL_dca * dca_ds * ds_dsig + dL_dcb * dcb_dsig;
+    }
   return CEEP_OK;
 }
 

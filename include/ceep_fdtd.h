@@ -122,6 +122,22 @@ int ceep_fdtd_energy(ceep_fdtd* h, double* out);
  * mean elapsed ms per run.  Leaves the state of the last run. */
 int ceep_fdtd_time_runs(ceep_fdtd* h, int reps, double* ms_per_run);
 
+/* Forward run for the adjoint (2D, non-dispersive): zeroes the state, runs
+ * total_steps steps like ceep_fdtd_run and keeps the full state every `every`
+ * steps as a checkpoint.  traces_out as in ceep_fdtd_run.  No counterpart in
+ * the reference (it has no adjoint; src/ceep/ai/__init__.py is a docstring). */
+int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out);
+
+/* Discrete adjoint of the last checkpointed run.  dtraces = dL/dtrace,
+ * (batch, n_probes, total_steps) float64, for any loss L of the traces; on
+ * return grad_eps_r and grad_sigma (full grid, float64) hold dL/d eps_r and
+ * dL/d sigma per cell (0 outside the stepped E interior, where the reference's
+ * result does not depend on them).  Each checkpoint segment is recomputed
+ * forward (storing E) and then stepped backward.  Leaves the fields zeroed;
+ * the checkpoints stay valid for further adjoint calls. */
+int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
+                      double* grad_sigma);
+
 /* Tuning knobs (results are the same to FP32 rounding for every setting):
  *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8)
  *   key 1: replay the time loop as a captured hipGraph (0/1)

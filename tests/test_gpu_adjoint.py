@@ -1,0 +1,111 @@
+"""Hand-written adjoint (GPU) vs PyTorch autograd through the FP64 loop and
+FP64 central finite differences, on a 96 x 96 crop of the configuration-1
+geometry (disk phantom, ring of TX/RX)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import dft, fdtd2d  # noqa: E402
+from oracle import waveforms as W  # noqa: E402
+from oracle.torch_loop import run_cavity_torch, s_matrix_torch  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+NX = NY = 96
+N, DX, T = 8, 0.5e-3, 240
+FREQS = np.linspace(10e9, 25e9, 16)
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import ceep_amd
+    return ceep_amd
+
+
+def _case(amd):
+    dt = W.dt_2d_courant(DX, 0.5)
+    w = W.modulated_gaussian(T, dt, 18e9, 20e9, delay_factor=3.0)
+    pos = amd.phantoms.ring_positions((48, 48), 30, 4)
+    ph = amd.phantoms.DiskPhantom2D(NX, NY, 0, r=(8, 14), max_offset=8)
+    tg = amd.phantoms.DiskPhantom2D(NX, NY, 1, r=(8, 14), max_offset=8)
+    eps, sig = ph.get_eps_map(2e9)
+    eps_t, sig_t = tg.get_eps_map(2e9)
+    St = dft.running_dft(fdtd2d.run_cavity(NX, NY, DX, dt, N, pos, pos, w, eps_t, sig_t), FREQS, dt)
+    return dt, w, pos, eps, sig, St
+
+
+def _solver(amd, dt, w, pos):
+    return amd.BatchedFDTD2D(nx=NX, ny=NY, dx=DX, total_steps=T, cpml_thickness=N,
+                             source_positions=pos, probe_positions=pos, frequency=18e9,
+                             courant=0.5, waveform=w)
+
+
+@pytest.fixture(scope="module")
+def reference(amd):
+    dt, w, pos, eps, sig, St = _case(amd)
+    e = torch.tensor(eps, requires_grad=True)
+    s = torch.tensor(sig, requires_grad=True)
+    S = s_matrix_torch(run_cavity_torch(NX, NY, DX, dt, N, pos, pos, w, e, s), FREQS, dt)
+    d = S - torch.tensor(St)
+    loss = (d.real ** 2 + d.imag ** 2).sum()
+    loss.backward()
+    return dict(dt=dt, w=w, pos=pos, eps=eps, sig=sig, St=St, loss=float(loss),
+                ge=e.grad.numpy().copy(), gs=s.grad.numpy().copy())
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("every", [50, 1, T])
+def test_adjoint_matches_autograd(amd, reference, every):
+    from ceep_amd import adjoint
+    r = reference
+    s = _solver(amd, r["dt"], r["w"], r["pos"])
+    loss, ge, gs = adjoint.gradient(s, r["eps"], r["sig"], FREQS, r["St"], every=every)
+    assert loss == pytest.approx(r["loss"], rel=1e-4)
+    ee, es = rel_l2(ge, r["ge"]), rel_l2(gs, r["gs"])
+    print(f"every={every}: loss {loss:.4e} vs {r['loss']:.4e}; dL/deps rel-L2 {ee:.2e}, "
+          f"dL/dsigma rel-L2 {es:.2e}")
+    assert ee < 1e-2 and es < 1e-2
+    assert np.all(ge[:N + 1] == 0) and np.all(gs[:, :N + 1] == 0)   # CPML and wall cells
+
+
+def test_adjoint_matches_finite_differences(amd, reference):
+    """FP64 central differences (h = 1e-4) on 32 seeded pixels where the
+    gradient is not negligible; agreement within 1%."""
+    from ceep_amd import adjoint
+    r = reference
+    s = _solver(amd, r["dt"], r["w"], r["pos"])
+    _, ge, gs = adjoint.gradient(s, r["eps"], r["sig"], FREQS, r["St"], every=50)
+
+    def loss(eps, sig):
+        tr = fdtd2d.run_cavity(NX, NY, DX, r["dt"], N, r["pos"], r["pos"], r["w"], eps, sig)
+        return float(np.sum(np.abs(dft.running_dft(tr, FREQS, r["dt"]) - r["St"]) ** 2))
+
+    rng = np.random.default_rng(3)
+
+    def pick(g):
+        # 16 seeded pixels among the 200 largest |gradient| values
+        top = np.argsort(np.abs(g).ravel())[-200:]
+        sel = rng.choice(top, size=16, replace=False)
+        return [np.unravel_index(i, g.shape) for i in sel]
+
+    picks, picks_s = pick(r["ge"]), pick(r["gs"])
+    h = 1e-4
+    worst = 0.0
+    for (a, b) in picks:
+        p, m = r["eps"].copy(), r["eps"].copy()
+        p[a, b] += h
+        m[a, b] -= h
+        fd = (loss(p, r["sig"]) - loss(m, r["sig"])) / (2 * h)
+        worst = max(worst, abs(ge[a, b] - fd) / abs(fd))
+    for (a, b) in picks_s:
+        p, m = r["sig"].copy(), r["sig"].copy()
+        p[a, b] += h
+        m[a, b] -= h
+        fd = (loss(r["eps"], p) - loss(r["eps"], m)) / (2 * h)
+        worst = max(worst, abs(gs[a, b] - fd) / abs(fd))
+    print(f"worst relative deviation from FD over 32 pixels: {worst:.2e}")
+    assert worst < 1e-2

@@ -98,6 +98,101 @@ def _cpu_baseline(ph, pos, w, dt, steps):
     }
 
 
+def _adjoint_ratio(every=50):
+    """fwd+adjoint / fwd on the gradient workload: 600x600, 16 TX/RX ring, 800
+    steps, seeded lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells),
+    loss = |S - S_target|^2 at 16 frequencies in 1-2.5 GHz against a second
+    seed, checkpoints every `every` steps with recomputation."""
+    import ceep_amd
+    from ceep_amd import adjoint
+    from ceep_amd.constants import cfl_timestep_2d
+    from ceep_amd.waveforms import modulated_gaussian
+
+    dt = cfl_timestep_2d(DX, DX, COURANT)
+    pos = ceep_amd.phantoms.ring_positions((300, 300), 200, B)
+    w = modulated_gaussian(T, dt, 2e9, 1.5e9)
+    freqs = np.linspace(1e9, 2.5e9, 16)
+    kw = dict(r=(30.0, 90.0), max_offset=120.0)
+    eps, sig = ceep_amd.phantoms.DiskPhantom2D(NX, NY, 0, **kw).get_eps_map(2e9)
+    eps_t, sig_t = ceep_amd.phantoms.DiskPhantom2D(NX, NY, 1, **kw).get_eps_map(2e9)
+
+    def solver(e, sg):
+        s = ceep_amd.BatchedFDTD2D(nx=NX, ny=NY, dx=DX, total_steps=T, cpml_thickness=N_CPML,
+                                   source_positions=pos, probe_positions=pos, frequency=2e9,
+                                   courant=COURANT, waveform=w, cpml_R=1e-6)
+        s._eps_r[:] = e
+        s._sigma_e[:] = sg
+        return s
+
+    tgt = solver(eps_t, sig_t)
+    tgt.run()
+    S_t = tgt.s_parameters(freqs).s_matrix
+    s = solver(eps, sig)
+    s._build()
+    for _ in range(2):
+        s._handle.reset()
+        s._handle.run(copy=False)
+        s._handle.dft(freqs)
+    _device_sync()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        s._handle.reset()
+        s._handle.run(copy=False)
+        s._handle.dft(freqs)
+    _device_sync()
+    t_fwd = (time.perf_counter() - t0) / reps
+    def timed(ev):
+        adjoint.gradient(s, eps, sig, freqs, S_t, every=ev)        # warm
+        _device_sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            adjoint.gradient(s, eps, sig, freqs, S_t, every=ev)
+        _device_sync()
+        return (time.perf_counter() - t0) / reps
+
+    t_grad = timed(every)
+    t_whole = timed(T)
+    return {"ratio": round(t_grad / t_fwd, 3), "ms_forward": round(t_fwd * 1e3, 3),
+            "ms_forward_plus_adjoint": round(t_grad * 1e3, 3), "checkpoint_every": every,
+            "ratio_single_segment": round(t_whole / t_fwd, 3),
+            "note": ("ratio: checkpoints every 50 steps, each earlier segment recomputed; "
+                     "ratio_single_segment: E of all 800 steps kept from the forward pass "
+                     "(no recomputation, 16.7 GB of HBM)"),
+            "workload": ("2D 600x600, 16 TX/RX ring r=200, 800 steps, lossy disk phantom; "
+                         "L = sum |S - S_target|^2 at 16 freqs in 1-2.5 GHz; dL/deps_r, dL/dsigma")}
+
+
+def _bench_3d(steps_timed=2, T3=1000):
+    """3D 200^3, dx = 1 mm, 10-cell CPML, 16 z-directed sources = probes on a
+    70-cell ring at z = 100, Debye + Lorentz head (seed 0), 1000 steps."""
+    import ceep_amd
+    from ceep_amd.constants import cfl_timestep_3d
+    from ceep_amd.waveforms import modulated_gaussian
+
+    n3, dx3 = 200, 1e-3
+    dt3 = cfl_timestep_3d(dx3, dx3, dx3, 0.5) * 0.99
+    ring = ceep_amd.phantoms.ring_positions((100, 100), 70, 16)
+    pos = [(x, y, 100) for x, y in ring]
+    w = modulated_gaussian(T3, dt3, 2e9, 1.5e9)
+    ph = ceep_amd.phantoms.DebyeHeadPhantom3D((n3, n3, n3), 0)
+    s = ceep_amd.BatchedFDTD3D(nx=n3, ny=n3, nz=n3, dx=dx3, total_steps=T3, cpml_thickness=10,
+                               source_positions=pos, probe_positions=pos, frequency=2e9,
+                               waveform=w)
+    s.set_phantom(ph)
+    s._build()
+    h = s._handle
+    box = h.box_info()
+    cells = int(box[0] * box[1] * box[2])
+    h.time_runs(1)
+    ms = h.time_runs(steps_timed)
+    return {"value": round(cells * 16 * T3 / (ms * 1e-3) / 1e9, 3), "unit": "GCell-steps/s",
+            "ms_per_run": round(ms, 2), "cells_counted": f"stepped box {box[0]}x{box[1]}x{box[2]}",
+            "value_full_grid_convention": round(n3 ** 3 * 16 * T3 / (ms * 1e-3) / 1e9, 3),
+            "workload": ("3D 200^3 dx=1mm, 10-cell CPML, 16 Ez TX=RX on r=70 ring at z=100, "
+                         "1000 steps, Debye+Lorentz head seed 0 (H and E kernels per step)")}
+
+
 def _traffic():
     path = os.path.join(ROOT, "profiles", "step2d_debye_600x600x16_traffic.json")
     if not os.path.exists(path):
@@ -114,6 +209,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--single-run", action="store_true",
                     help="one simulation + DFT with the default tuning, no output (for rocprofv3 --pmc)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -182,6 +279,9 @@ def main():
     bytes_launch = cells * (B * BYTES_PER_CELL_PER_TX + BYTES_PER_CELL_SHARED) * steps_per_launch
     achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
 
+    adj = None if args.no_adjoint else _adjoint_ratio()
+    b3 = None if args.no_3d else _bench_3d()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = _cpu_baseline(ph, pos, w, dt, args.cpu_steps)
@@ -219,7 +319,9 @@ def main():
             "algorithmic_bytes_per_launch": bytes_launch,
             "launch_ms": round(ms_launch, 5),
         },
-        "fwd_adjoint_ratio": None,
+        "fwd_adjoint_ratio": adj["ratio"] if adj else None,
+        "adjoint": adj,
+        "bench_3d": b3,
         "cpu_baseline": cpu,
         "s_matrix_shape": list(S.shape),
     }

@@ -25,9 +25,7 @@ class BatchedTraces(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, eps_r, sigma_e, solver, every):
-        solver._eps_r[:] = eps_r.detach().cpu().numpy()
-        solver._sigma_e[:] = sigma_e.detach().cpu().numpy()
-        solver._build()
+        solver.update_materials(eps_r.detach().cpu().numpy(), sigma_e.detach().cpu().numpy())
         tr = solver._handle.run_checkpointed(int(every))
         solver.probe_data = tr
         ctx.solver = solver

@@ -182,9 +182,6 @@ class BatchedFDTD2D:
         h.set_probes(prb)
         h.reset()
         self._handle = h
-        ca, cb = self._coefficients()
-        self.ca = ca[None]
-        self.cb = cb[None]
         self.da = np.ones((1, self.nx, self.ny))
         self.db = np.full((1, self.nx, self.ny), self.dt / MU_0)
         self.inv_dx = 1.0 / self.dx
@@ -198,6 +195,16 @@ class BatchedFDTD2D:
         self.cpml_b_y, self.cpml_c_y = self.cpml_b_x, self.cpml_c_x
         self.probe_data = np.zeros((self.batch, len(self.probe_positions), self.total_steps))
         self._built = True
+
+    def update_materials(self, eps_r, sigma_e) -> None:
+        """Replace the eps_r / sigma maps of a built solver in place (the device
+        state, buffers and launch setup are kept; fields are not reset)."""
+        self._eps_r[:] = eps_r
+        self._sigma_e[:] = sigma_e
+        if not self._built:
+            self._build()
+            return
+        self._handle.set_materials(self._eps_r, self._sigma_e)
 
     @staticmethod
     def _as_dict(traces: np.ndarray) -> Dict[int, Dict[int, np.ndarray]]:
@@ -237,6 +244,10 @@ class BatchedFDTD2D:
         f = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
         S = self._handle.dft(f)
         return MultistaticSParameters(f, S, [tuple(p) for p in self.probe_positions])
+
+    # the (1, nx, ny) coefficient maps of _build (:162-165), formed on access
+    ca = property(lambda self: self._coefficients()[0][None])
+    cb = property(lambda self: self._coefficients()[1][None])
 
     def _field(self, comp):
         if not self._built:

@@ -80,6 +80,23 @@ __global__ void adj_seed_kernel(float* le, const int* probe_cell, const float* d
   atomicAdd(le + (long long)bt * bstride + pc, dtr[((long long)bt * P + q) * T + step]);
 }
 
+__global__ __launch_bounds__(256) void sum_parts_kernel(const float* __restrict__ src, int nparts,
+                                                        long long stride, long long n,
+                                                        float* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int k = 0; k < nparts; ++k) acc += src[k * stride + i];
+  dst[i] = acc;
+}
+
+hipError_t launch_sum_parts(const float* src, int nparts, long long stride, long long n,
+                            float* dst, hipStream_t s) {
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src,
+                     nparts, stride, n, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_adj_seed(float* le, const int* probe_cell, const float* dtr, int B, int P,
                            int T, int step, long long bstride, hipStream_t s) {
   const int n = B * P;

@@ -109,9 +109,46 @@ struct TB2DArgs {
   int tiles_x, tiles_y;           // output tiles over rows / columns
   int bpw;                        // batch planes per wavefront
   int ntasks;                     // tiles_x * tiles_y * ceil(B / bpw)
+  float* e_save;                  // if set: E after local step s -> e_save + s * slot ([B][plane])
+  long long slot;                 // floats per e_save slot
 };
 
 // poles: 0 or 1 (Debye); R in {16, 24, 32}; K in {1, 2, 3, 4}.
 hipError_t launch_step2d_tb(const TB2DArgs& a, int poles, int R, int K, hipStream_t s);
+
+// Temporally blocked adjoint: K adjoint steps (step_hi-1 down to step_hi-ksteps)
+// per launch on R x 64 register windows, the transpose of step2d_tb_kernel's
+// update; gradient sums go to one partial plane per TX chunk (no atomics).
+struct AdjTB2DArgs {
+  int Mx, My, pitch, B;
+  long long bstride;
+  const float* le_in;
+  const float* mx_in;
+  const float* my_in;
+  float* le_out;
+  float* mx_out;
+  float* my_out;
+  const float* ca;
+  const float* cb;
+  float ch, idx;
+  const float* e_store;           // forward E by slot: E^s at e_store + (s - seg0) * slot
+  long long slot;
+  int seg0;
+  const int* src_cell;
+  const float* wave;
+  long long wave_stride_b;
+  int step_hi, ksteps, T, P;
+  const float* dtr;               // [B][P][T] dL/dtrace
+  const int* probe_cell;
+  float* g_ca;                    // [nchunks][gstride] partial sums, one plane per TX chunk
+  float* g_cbi;
+  long long gstride;
+  int tiles_x, tiles_y, bpw, ntasks;
+};
+
+hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
+// dst[i] = sum over k < nparts of src[k * stride + i], i < n
+hipError_t launch_sum_parts(const float* src, int nparts, long long stride, long long n,
+                            float* dst, hipStream_t s);
 
 }  // namespace ceep

@@ -62,6 +62,19 @@ __global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
   }
   const float ch = p.ch, idx = p.idx;
   const bool out_lane = lane >= K && lane < 64 - K && col_ok;
+  // probes on this wavefront's output window (lane q describes probe q when
+  // P <= 64; larger probe sets are scanned every step)
+  const bool few_probes = p.P <= 64;
+  int pr_h = -1, pl_h = -1;
+  unsigned long long pmask = 0;
+  if (few_probes) {
+    const int pc = lane < p.P ? p.probe_cell[lane] : -1;
+    if (pc >= 0) {
+      pr_h = pc / p.pitch - a0;
+      pl_h = pc % p.pitch - cb0;
+    }
+    pmask = __ballot(pc >= 0 && pr_h >= K && pr_h < R - K && pl_h >= K && pl_h < 64 - K);
+  }
 
   const int b_begin = chunk * p.bpw;
   const int b_end = min(b_begin + p.bpw, p.B);
@@ -117,8 +130,26 @@ __global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
         for (int r = 0; r < R; ++r)
           if (r == s_row && lane == s_lane) ez[r] += w;
       }
+      if (p.e_save) {   // E after this step on the output window (adjoint store)
+        float* dst = p.e_save + (long long)s * p.slot + base;
+#pragma unroll
+        for (int r = K; r < R - K; ++r) {
+          const int a = a0 + r;
+          if (out_lane && a < p.Mx) dst[(long long)a * p.pitch + c] = ez[r];
+        }
+      }
       // probes inside this wavefront's output window
-      for (int q0 = 0; q0 < p.P; q0 += 64) {
+      for (unsigned long long m = pmask; m;) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int qr = __shfl(pr_h, l);
+        const int ql = __shfl(pl_h, l);
+        float v = 0.f;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v = (r == qr) ? ez[r] : v;
+        if (lane == ql) p.traces[((long long)bt * p.P + l) * p.T + step] = v;
+      }
+      for (int q0 = 0; !few_probes && q0 < p.P; q0 += 64) {
         const int q = q0 + lane;
         const int pc = q < p.P ? p.probe_cell[q] : -1;
         int pr = -1, pl = -1;
@@ -155,6 +186,188 @@ __global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
   }
 }
 
+// Transpose of step2d_tb_kernel's non-dispersive update.  Per adjoint step,
+// with q = cbi * lam_e:
+//   mu_x += q(c+1) - q(c);  mu_y += q(a) - q(a+1);
+//   lam_e = ca lam_e + ch [(mu_y(a-1) - mu_y(a)) - (mu_x(c-1) - mu_x(c))] + dL/dtrace(s-1)
+// The window shrinks by one cell per step exactly as in the forward kernel.
+template <int R, int K>
+__global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
+  constexpr int NO = R - 2 * K;                        // output rows
+  const int lane = threadIdx.x & 63;
+  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (task >= p.ntasks) return;
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = task % tiles;
+  const int chunk = task / tiles;
+  const int ty = tile % p.tiles_y;
+  const int tx = tile / p.tiles_y;
+  const int a0 = tx * NO - K;
+  const int cb0 = ty * (64 - 2 * K) - K;
+  const int c = cb0 + lane;
+  const bool col_ok = c >= 0 && c < p.My;
+  const bool out_lane = lane >= K && lane < 64 - K && col_ok;
+
+  float mca[R], mcbi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int a = a0 + r;
+    const bool in = col_ok && a >= 0 && a < p.Mx;
+    const long long cell = (long long)a * p.pitch + c;
+    mca[r] = in ? p.ca[cell] : 0.f;
+    mcbi[r] = in ? p.cb[cell] * p.idx : 0.f;
+  }
+  float gca[NO], gcb[NO];
+#pragma unroll
+  for (int r = 0; r < NO; ++r) { gca[r] = 0.f; gcb[r] = 0.f; }
+  const float ch = p.ch;
+  // probes anywhere in the window (halo included) receive dL/dtrace; lane q
+  // describes probe q when P <= 64
+  const bool few_probes = p.P <= 64;
+  int pr_h = -1, pl_h = -1;
+  unsigned long long pmask = 0;
+  if (few_probes) {
+    const int pc = lane < p.P ? p.probe_cell[lane] : -1;
+    if (pc >= 0) {
+      pr_h = pc / p.pitch - a0;
+      pl_h = pc % p.pitch - cb0;
+    }
+    pmask = __ballot(pc >= 0 && pr_h >= 0 && pr_h < R && pl_h >= 0 && pl_h < 64);
+  }
+
+  const int b_begin = chunk * p.bpw;
+  const int b_end = min(b_begin + p.bpw, p.B);
+  for (int bt = b_begin; bt < b_end; ++bt) {
+    const long long base = (long long)bt * p.bstride;
+    float le[R], mx[R], my[R], e1[NO];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int a = a0 + r;
+      const bool in = col_ok && a >= 0 && a < p.Mx;
+      const long long o = base + (long long)a * p.pitch + c;
+      le[r] = in ? p.le_in[o] : 0.f;
+      mx[r] = in ? p.mx_in[o] : 0.f;
+      my[r] = in ? p.my_in[o] : 0.f;
+    }
+    const float* e_hi = p.e_store + (long long)(p.step_hi - p.seg0) * p.slot + base;
+#pragma unroll
+    for (int r = 0; r < NO; ++r) {
+      const int a = a0 + K + r;
+      e1[r] = (out_lane && a < p.Mx) ? e_hi[(long long)a * p.pitch + c] : 0.f;
+    }
+    const int sc = p.src_cell[bt];
+    const int s_row = sc / p.pitch - a0;
+    const int s_lane = sc % p.pitch - cb0;
+    // everything the K steps read from memory, issued before the first step
+    float es_all[K][NO], wsrc[K], gq[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int step = p.step_hi - 1 - k;
+      const bool live = k < p.ksteps;
+      const float* e_s = p.e_store + (long long)(step - p.seg0) * p.slot + base;
+#pragma unroll
+      for (int r = 0; r < NO; ++r) {
+        const int a = a0 + K + r;
+        es_all[k][r] = (live && out_lane && a < p.Mx) ? e_s[(long long)a * p.pitch + c] : 0.f;
+      }
+      wsrc[k] = live ? p.wave[(long long)bt * p.wave_stride_b + step] : 0.f;
+      gq[k] = (live && step >= 1 && few_probes && ((pmask >> lane) & 1ull))
+                  ? p.dtr[((long long)bt * p.P + lane) * p.T + (step - 1)] : 0.f;
+    }
+
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (k >= p.ksteps) break;
+      const int step = p.step_hi - 1 - k;
+      // gradient sums on the output window (lam_e here is lam_e^{step+1})
+#pragma unroll
+      for (int r = 0; r < NO; ++r) {
+        const float es = es_all[k][r];
+        float d = e1[r] - mca[K + r] * es;
+        if (K + r == s_row && lane == s_lane) d -= wsrc[k];
+        gca[r] += le[K + r] * es;
+        gcb[r] += le[K + r] * d;
+        e1[r] = es;
+      }
+      // adjoint H half step
+      float q[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) q[r] = mcbi[r] * le[r];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float qR = from_next_lane(q[r]);
+        const float qD = (r + 1 < R) ? q[r + 1] : 0.f;
+        mx[r] = mx[r] + (qR - q[r]);
+        my[r] = my[r] + (q[r] - qD);
+      }
+      // adjoint E step
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float mxL = from_prev_lane(mx[r]);
+        const float myU = (r > 0) ? my[r - 1] : 0.f;
+        le[r] = mca[r] * le[r] + ch * ((myU - my[r]) - (mxL - mx[r]));
+      }
+      // dL/dtrace(step - 1) enters lam_e^step at the probes (halo included)
+      for (unsigned long long m = pmask; m;) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int qr = __shfl(pr_h, l);
+        const int ql = __shfl(pl_h, l);
+        const float g = __shfl(gq[k], l);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r == qr && lane == ql) le[r] += g;
+      }
+      if (!few_probes && step >= 1) {
+        for (int q0 = 0; q0 < p.P; q0 += 64) {
+          const int qi = q0 + lane;
+          const int pc = qi < p.P ? p.probe_cell[qi] : -1;
+          int pr = -1, pl = -1;
+          if (pc >= 0) {
+            pr = pc / p.pitch - a0;
+            pl = pc % p.pitch - cb0;
+          }
+          const bool here = pc >= 0 && pr >= 0 && pr < R && pl >= 0 && pl < 64;
+          unsigned long long m = __ballot(here);
+          while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const int qr = __shfl(pr, l);
+            const int ql = __shfl(pl, l);
+            const float g = p.dtr[((long long)bt * p.P + q0 + l) * p.T + (step - 1)];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (r == qr && lane == ql) le[r] += g;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = K; r < R - K; ++r) {
+      const int a = a0 + r;
+      if (out_lane && a < p.Mx) {
+        const long long o = base + (long long)a * p.pitch + c;
+        p.le_out[o] = le[r];
+        p.mx_out[o] = mx[r];
+        p.my_out[o] = my[r];
+      }
+    }
+  }
+  // this wavefront is the only writer of its output cells in its chunk's
+  // partial-sum plane during this launch: plain read-modify-write
+  float* gca_p = p.g_ca + (long long)chunk * p.gstride;
+  float* gcb_p = p.g_cbi + (long long)chunk * p.gstride;
+#pragma unroll
+  for (int r = 0; r < NO; ++r) {
+    const int a = a0 + K + r;
+    if (out_lane && a < p.Mx && mcbi[K + r] != 0.f) {
+      const long long cell = (long long)a * p.pitch + c;
+      gca_p[cell] += gca[r];
+      gcb_p[cell] += gcb[r];
+    }
+  }
+}
+
 template <int R, int K>
 static hipError_t launch_rk(const TB2DArgs& a, int poles, hipStream_t s) {
   const dim3 blk(256, 1, 1), grd((a.ntasks + 3) / 4, 1, 1);
@@ -171,6 +384,28 @@ static hipError_t launch_r(const TB2DArgs& a, int poles, int K, hipStream_t s) {
     case 2: return launch_rk<R, 2>(a, poles, s);
     case 3: return launch_rk<R, 3>(a, poles, s);
     case 4: return launch_rk<R, 4>(a, poles, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int R>
+static hipError_t launch_adj_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((a.ntasks + 3) / 4, 1, 1);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((adj2d_tb_kernel<R, 1>), grd, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((adj2d_tb_kernel<R, 2>), grd, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((adj2d_tb_kernel<R, 3>), grd, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((adj2d_tb_kernel<R, 4>), grd, blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
+  switch (R) {
+    case 16: return launch_adj_r<16>(a, K, s);
+    case 24: return launch_adj_r<24>(a, K, s);
+    case 32: return launch_adj_r<32>(a, K, s);
   }
   return hipErrorInvalidValue;
 }

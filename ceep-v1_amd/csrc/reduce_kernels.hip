@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include "reduce_kernels.h"
 
+#include <algorithm>
+
 namespace ceep {
 
 __global__ __launch_bounds__(256) void dft_kernel(const float* __restrict__ traces,
@@ -63,6 +65,24 @@ __global__ __launch_bounds__(256) void sumsq_kernel(const float* __restrict__ x,
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Device-to-device copy of n floats (both 16-byte aligned, as every buffer of
+// the engine is); a kernel keeps the copy on the shader engines at HBM speed.
+__global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ src,
+                                                   float4* __restrict__ dst, long long n4) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy(const float* src, float* dst, long long n, hipStream_t s) {
+  if (n % 4 != 0) return hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+  const long long n4 = n / 4;
+  const int blocks = (int)std::min<long long>((n4 + 255) / 256, 8192);
+  hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
+  return hipGetLastError();
 }
 
 hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,

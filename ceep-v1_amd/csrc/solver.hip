@@ -111,6 +111,7 @@ struct ceep_fdtd {
   float* g_cbi = nullptr;
   float* d_dtr = nullptr;
   bool ck_valid = false;
+  int estore_seg = -1;         // segment whose forward E the store holds, or -1
 
   bool mats_dirty = true;
   bool has_run = false;
@@ -140,6 +141,7 @@ struct ceep_fdtd {
     ck_valid = false;
     if (estore) (void)hipFree(estore);
     estore = nullptr;
+    estore_seg = -1;
     for (auto& row : adj)
       for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
     if (g_ca) (void)hipFree(g_ca);
@@ -301,13 +303,14 @@ int launch_steps(ceep_fdtd* h, int start) {
 }
 
 // Enqueue steps [step0, step0 + nsteps) from ping-pong buffer `start`.  With
-// e_save (2D one-step kernel only), E after step s is written to slot
-// s + 1 - step0 of the [slot][B][plane] store.
+// e_save (2D), E after step s is written to slot s + 1 - step0 of the
+// [slot][B][plane] store (by the blocked kernel on its output windows only,
+// i.e. on the stepped box).
 int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) {
   const int step_end = step0 + nsteps;
   hipStream_t s = h->stream;
   const int poles = h->poles();
-  if (h->use_tb() && !e_save) {
+  if (h->use_tb()) {
     ceep::TB2DArgs a{};
     a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
     a.ca = h->ca; a.cb = h->cb; a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
@@ -327,6 +330,8 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       a.jd_in = h->jd[cur]; a.jd_out = h->jd[o];
       a.step0 = st;
       a.ksteps = std::min(K, step_end - st);
+      a.slot = h->box_floats();
+      a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
       CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
       cur = o;
     }
@@ -413,8 +418,7 @@ int run_device(ceep_fdtd* h) {
   h->parity = start ^ (h->launches_per_run() & 1);
   if (h->ndim == 2 && h->has_debye && !h->use_tb() && h->parity != start) {
     // the per-step kernel updates J in place in jd[start]; keep J with the fields
-    CEEP_HIP(hipMemcpyAsync(h->jd[h->parity], h->jd[start],
-                            (size_t)h->box_floats() * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    CEEP_HIP(ceep::launch_copy(h->jd[start], h->jd[h->parity], h->box_floats(), h->stream));
   }
   return CEEP_OK;
 }
@@ -798,23 +802,34 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   if (h->ndim != 2 || h->has_debye || h->has_lorentz)
     return fail(CEEP_EUNSUPPORTED, "the adjoint covers 2D non-dispersive media");
   if (every < 1) return fail(CEEP_EINVAL, "checkpoint interval must be >= 1");
-  h->free_adjoint();
   const int nck = (h->T + every - 1) / every;
-  const size_t bytes = (size_t)h->box_floats() * sizeof(float);
-  h->ck.assign((size_t)nck * 3, nullptr);
-  for (auto& p : h->ck)
-    if ((rc = dalloc(&p, h->box_floats()))) { h->free_adjoint(); return rc; }
+  h->ck_valid = false;
+  if (nck != h->ck_count) {     // checkpoint buffers are kept across calls
+    for (auto& p : h->ck) dfree(p);
+    h->ck.assign((size_t)nck * 3, nullptr);
+    for (auto& p : h->ck)
+      if ((rc = dalloc(&p, h->box_floats()))) { h->free_adjoint(); return rc; }
+    h->ck_count = nck;
+  }
+  if (every != h->ck_every) dfree(h->estore);   // its slot count follows the interval
   h->ck_every = every;
-  h->ck_count = nck;
+  h->estore_seg = -1;
+  // the adjoint starts with the last segment: store its E during this pass so
+  // that segment needs no recomputation
+  if (!h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * h->box_floats()))) return rc;
   if ((rc = zero_state(h))) return rc;
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), h->stream));
   int cur = 0;
   for (int k = 0; k < nck; ++k) {
     for (int f = 0; f < 3; ++f)
-      CEEP_HIP(hipMemcpyAsync(h->ck[(size_t)k * 3 + f], h->f[cur][f], bytes,
-                              hipMemcpyDeviceToDevice, h->stream));
+      CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ck[(size_t)k * 3 + f], h->box_floats(), h->stream));
     const int s0 = k * every, n = std::min(every, h->T - s0);
-    if ((rc = launch_range(h, cur, s0, n, nullptr))) return rc;
+    float* es = nullptr;
+    if (k == nck - 1) {
+      CEEP_HIP(ceep::launch_copy(h->f[cur][0], h->estore, h->box_floats(), h->stream));
+      es = h->estore;
+    }
+    if ((rc = launch_range(h, cur, s0, n, es))) return rc;
     const int launches = h->use_tb() ? (n + h->tb_k - 1) / h->tb_k : n;
     cur ^= launches & 1;
   }
@@ -822,6 +837,7 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   CEEP_HIP(hipStreamSynchronize(h->stream));
   h->has_run = true;
   h->ck_valid = true;
+  h->estore_seg = nck - 1;
   if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
   return CEEP_OK;
 }
@@ -848,14 +864,16 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   for (int i = 0; i < 2; ++i)
     for (int f = 0; f < 3; ++f)
       if (!h->adj[i][f] && (rc = dalloc(&h->adj[i][f], nbox))) return rc;
-  if (!h->g_ca && (rc = dalloc(&h->g_ca, h->plane))) return rc;
-  if (!h->g_cbi && (rc = dalloc(&h->g_cbi, h->plane))) return rc;
+  // one partial-sum plane per TX chunk of the blocked kernel (at most B)
+  const int nchunks = h->use_tb() ? (B + h->tb_bpw - 1) / h->tb_bpw : 1;
+  if (!h->g_ca && (rc = dalloc(&h->g_ca, (size_t)B * h->plane))) return rc;
+  if (!h->g_cbi && (rc = dalloc(&h->g_cbi, (size_t)B * h->plane))) return rc;
   if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
   hipStream_t st = h->stream;
   CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr.data(), ntr * sizeof(float), hipMemcpyHostToDevice, st));
   for (int f = 0; f < 3; ++f) CEEP_HIP(hipMemsetAsync(h->adj[0][f], 0, bytes, st));
-  CEEP_HIP(hipMemsetAsync(h->g_ca, 0, (size_t)h->plane * sizeof(float), st));
-  CEEP_HIP(hipMemsetAsync(h->g_cbi, 0, (size_t)h->plane * sizeof(float), st));
+  CEEP_HIP(hipMemsetAsync(h->g_ca, 0, (size_t)nchunks * h->plane * sizeof(float), st));
+  CEEP_HIP(hipMemsetAsync(h->g_cbi, 0, (size_t)nchunks * h->plane * sizeof(float), st));
   // lam_e^T = dL/dtrace(T-1) at the probes
   CEEP_HIP(ceep::launch_adj_seed(h->adj[0][0], h->probe_cell, h->d_dtr, B, P, T, T - 1,
                                  h->plane, st));
@@ -869,12 +887,39 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   int acur = 0;
   for (int k = h->ck_count - 1; k >= 0; --k) {
     const int s0 = k * every, n = std::min(every, T - s0);
-    // recompute the segment from its checkpoint (one-step kernel, storing E)
-    for (int f = 0; f < 3; ++f)
-      CEEP_HIP(hipMemcpyAsync(h->f[0][f], h->ck[(size_t)k * 3 + f], bytes,
-                              hipMemcpyDeviceToDevice, st));
-    CEEP_HIP(hipMemcpyAsync(h->estore, h->ck[(size_t)k * 3], bytes, hipMemcpyDeviceToDevice, st));
-    if ((rc = launch_range(h, 0, s0, n, h->estore))) return rc;
+    // recompute the segment from its checkpoint, storing E (unless the
+    // store already holds this segment)
+    if (k != h->estore_seg) {
+      for (int f = 0; f < 3; ++f)
+        CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * 3 + f], h->f[0][f], (long long)nbox, st));
+      CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * 3], h->estore, (long long)nbox, st));
+      if ((rc = launch_range(h, 0, s0, n, h->estore))) return rc;
+      h->estore_seg = k;
+    }
+    if (h->use_tb()) {
+      const int K = h->tb_k, R = h->tb_r;
+      ceep::AdjTB2DArgs t{};
+      t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
+      t.ca = h->ca; t.cb = h->cb; t.ch = h->ch; t.idx = h->idx;
+      t.e_store = h->estore; t.slot = (long long)nbox; t.seg0 = s0;
+      t.src_cell = h->src2; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
+      t.T = T; t.P = P; t.dtr = h->d_dtr; t.probe_cell = h->probe_cell;
+      t.g_ca = h->g_ca; t.g_cbi = h->g_cbi; t.gstride = h->plane;
+      t.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
+      t.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
+      t.bpw = h->tb_bpw;
+      t.ntasks = t.tiles_x * t.tiles_y * ((B + t.bpw - 1) / t.bpw);
+      for (int hi = s0 + n; hi > s0; hi -= K) {
+        const int o = acur ^ 1;
+        t.le_in = h->adj[acur][0]; t.mx_in = h->adj[acur][1]; t.my_in = h->adj[acur][2];
+        t.le_out = h->adj[o][0]; t.mx_out = h->adj[o][1]; t.my_out = h->adj[o][2];
+        t.step_hi = hi;
+        t.ksteps = std::min(K, hi - s0);
+        CEEP_HIP(ceep::launch_adj2d_tb(t, R, K, st));
+        acur = o;
+      }
+      continue;
+    }
     // adjoint steps s = s0 + n - 1 ... s0
     for (int s = s0 + n - 1; s >= s0; --s) {
       const int o = acur ^ 1;
@@ -886,6 +931,10 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       CEEP_HIP(ceep::launch_adj2d(a, st));
       acur = o;
     }
+  }
+  if (nchunks > 1) {   // fold the chunk partials into plane 0 (in place: plane 0 is read first)
+    CEEP_HIP(ceep::launch_sum_parts(h->g_ca, nchunks, h->plane, h->plane, h->g_ca, st));
+    CEEP_HIP(ceep::launch_sum_parts(h->g_cbi, nchunks, h->plane, h->plane, h->g_cbi, st));
   }
   std::vector<float> gca(h->plane), gcb(h->plane);
   CEEP_HIP(hipMemcpyAsync(gca.data(), h->g_ca, gca.size() * sizeof(float), hipMemcpyDeviceToHost, st));

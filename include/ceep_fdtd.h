@@ -133,8 +133,10 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out);
  * return grad_eps_r and grad_sigma (full grid, float64) hold dL/d eps_r and
  * dL/d sigma per cell (0 outside the stepped E interior, where the reference's
  * result does not depend on them).  Each checkpoint segment is recomputed
- * forward (storing E) and then stepped backward.  Leaves the fields zeroed;
- * the checkpoints stay valid for further adjoint calls. */
+ * forward (storing E) and then stepped backward; the last segment's E was
+ * stored by the checkpointed run itself, so with every >= total_steps nothing
+ * is recomputed.  Leaves the fields zeroed; the checkpoints stay valid for
+ * further adjoint calls. */
 int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
                       double* grad_sigma);
 

@@ -72,6 +72,26 @@ def test_adjoint_matches_autograd(amd, reference, every):
     assert np.all(ge[:N + 1] == 0) and np.all(gs[:, :N + 1] == 0)   # CPML and wall cells
 
 
+@pytest.mark.parametrize("tuning", [
+    {"temporal_depth": 0},
+    {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4},
+    {"temporal_depth": 3, "rows_per_wave": 24, "batch_per_wave": 1},
+    {"temporal_depth": 2, "rows_per_wave": 32, "batch_per_wave": 3},
+    {"temporal_depth": 1, "rows_per_wave": 16, "batch_per_wave": 2},
+])
+def test_adjoint_kernel_variants(amd, reference, tuning):
+    """One-step and blocked (recompute + adjoint) kernels, segment lengths that
+    are not multiples of K, batch not a multiple of the planes per wavefront."""
+    from ceep_amd import adjoint
+    r = reference
+    s = _solver(amd, r["dt"], r["w"], r["pos"])
+    s.tuning = tuning
+    _, ge, gs = adjoint.gradient(s, r["eps"], r["sig"], FREQS, r["St"], every=45)
+    ee, es = rel_l2(ge, r["ge"]), rel_l2(gs, r["gs"])
+    print(f"{tuning}: dL/deps rel-L2 {ee:.2e}, dL/dsigma rel-L2 {es:.2e}")
+    assert ee < 1e-3 and es < 1e-3
+
+
 def test_adjoint_matches_finite_differences(amd, reference):
     """FP64 central differences (h = 1e-4) on 32 seeded pixels where the
     gradient is not negligible; agreement within 1%."""

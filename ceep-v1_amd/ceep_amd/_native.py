@@ -57,6 +57,11 @@ SIGNATURES = {
     "ceep_fdtd_run_checkpointed": (_I, [_P, _I, _PD]),
     "ceep_fdtd_adjoint": (_I, [_P, _PD, _PD, _PD]),
     "ceep_fdtd_set_tuning": (_I, [_P, _I, _I]),
+    "ceep_fdtd_multi_create": (_I, [_I, ctypes.POINTER(_P), _I, _I, _I, ctypes.POINTER(_P)]),
+    "ceep_fdtd_multi_run": (_I, [_P, _I]),
+    "ceep_fdtd_multi_time": (_I, [_P, _I, _I, _PD]),
+    "ceep_fdtd_multi_info": (_I, [_P, ctypes.POINTER(ctypes.c_longlong)]),
+    "ceep_fdtd_multi_destroy": (_I, [_P]),
     "ceep_fdtd_box_info": (_I, [_P, _PI64]),
 }
 
@@ -259,3 +264,42 @@ class Handle:
         out = np.zeros(5, dtype=np.int64)
         check(self._lib.ceep_fdtd_box_info(self._h, iptr(out)))
         return out
+
+
+class MultiHandle:
+    """Owns one ``ceep_fdtd_multi*`` grouping several 2D ``Handle``s."""
+
+    def __init__(self, handles, K, R, bpw):
+        self._lib = load()
+        self._m = _P()
+        self._handles = list(handles)           # keep the members alive
+        arr = (_P * len(handles))(*[h._h for h in handles])
+        check(self._lib.ceep_fdtd_multi_create(len(handles), arr, int(K), int(R), int(bpw),
+                                               ctypes.byref(self._m)))
+
+    PERSISTENT = 0
+    PER_ROUND = 1
+
+    def run(self, mode=PERSISTENT):
+        check(self._lib.ceep_fdtd_multi_run(self._m, int(mode)))
+
+    def time(self, reps, mode=PERSISTENT):
+        v = ctypes.c_double()
+        check(self._lib.ceep_fdtd_multi_time(self._m, int(mode), int(reps), ctypes.byref(v)))
+        return v.value
+
+    def info(self):
+        out = (ctypes.c_longlong * 3)()
+        check(self._lib.ceep_fdtd_multi_info(self._m, out))
+        return {"items": out[0], "segments": out[1], "workgroups": out[2]}
+
+    def close(self):
+        if getattr(self, "_m", None) and self._m.value:
+            self._lib.ceep_fdtd_multi_destroy(self._m)
+            self._m = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -147,6 +147,37 @@ struct AdjTB2DArgs {
 };
 
 hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
+
+// One problem of an uneven batch: its blocked-kernel arguments (buffers set
+// per round from the ping-pong pairs) and the number of K-step rounds.
+struct MultiDesc {
+  TB2DArgs a;
+  float* f[2][3];
+  float* jd[2];
+  int nrounds;
+};
+
+// The persistent uneven-batch launch: items are (round, problem, task),
+// enumerated as segments of one problem's tasks in one round.
+struct MultiArgs {
+  const MultiDesc* probs;
+  int nprob;
+  const long long* seg_start;     // [nseg] first item of each segment (ascending)
+  const int* seg_prob;
+  const int* seg_round;
+  int nseg;
+  unsigned long long total;       // number of items
+  unsigned long long* queue;      // next item (zeroed before the launch)
+  unsigned* done;                 // [nprob] finished items (zeroed before the launch)
+  unsigned* status;               // set to 1 if a dependency wait ran out
+  unsigned long long wait_ticks;  // real-time clock ticks before a wait gives up
+};
+
+hipError_t launch_multi_tb(const MultiArgs& m, int poles, int R, int K, int blocks, hipStream_t s);
+// One round's items [first, first + count) (segments [seg0, seg1)) as one launch.
+hipError_t launch_round_tb(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
+                           long long first, long long count, hipStream_t s);
+int multi_tb_blocks_per_cu(int poles, int R, int K);
 // dst[i] = sum over k < nparts of src[k * stride + i], i < n
 hipError_t launch_sum_parts(const float* src, int nparts, long long stride, long long n,
                             float* dst, hipStream_t s);

@@ -32,11 +32,10 @@ __device__ __forceinline__ float from_prev_lane(float v) {
 
 }  // namespace
 
+// One task = one (output tile, batch chunk) window advanced p.ksteps steps by
+// the calling wavefront.
 template <int R, int K, int POLES>
-__global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
-  const int lane = threadIdx.x & 63;
-  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (task >= p.ntasks) return;                       // uniform per wavefront
+__device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = task % tiles;
   const int chunk = task / tiles;
@@ -366,6 +365,187 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
       gcb_p[cell] += gcb[r];
     }
   }
+}
+
+template <int R, int K, int POLES>
+__global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
+  const int lane = threadIdx.x & 63;
+  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (task >= p.ntasks) return;                       // uniform per wavefront
+  tb_task<R, K, POLES>(p, task, lane);
+}
+
+// NOTE: switched off (ceep_fdtd_multi_run refuses mode 0).  On MI355X the
+// first version (acquire load per poll) and this one (relaxed polls, one
+// acquire fence, real-time-bounded waits, global abort flag) both ran without
+// terminating, although a CPU model of the queue drains for every worker
+// count; the cause is not established -- the leading suspects are the
+// visibility of the done counters across XCDs to the polling loads and the
+// real-time bound -- so the path stays disabled until it is.
+//
+// Uneven batch in one persistent launch.  Every wavefront is a worker that
+// takes items from one queue ordered round by round (a round = K time steps
+// of every problem still running), so all items an item depends on -- the
+// previous round of its own problem, which read the buffer it will write and
+// wrote the buffer it will read -- were taken earlier and are running or
+// done; the oldest unfinished item never waits, so the queue always drains.
+// A worker publishes a finished item with an agent-scope release (its stores
+// reach the memory shared by all XCDs); a waiting worker polls the counter
+// with relaxed loads (an acquire per poll would invalidate the L2 each time)
+// and issues one agent-scope acquire fence once the count is reached.
+// Every wait is bounded in real time (m.wait_ticks of the 100 MHz constant
+// clock); a wait that runs out sets *status, after which every wait and every
+// queue take gives up at once, so the launch always drains.
+template <int R, int K, int POLES>
+__global__ __launch_bounds__(256) void multi_tb_kernel(MultiArgs m) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    unsigned long long i = 0;
+    if (lane == 0) {
+      i = __hip_atomic_load(m.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+              ? m.total : atomicAdd(m.queue, 1ull);
+    }
+    i = __shfl(i, 0);
+    if (i >= m.total) break;
+    int lo = 0, hi = m.nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((unsigned long long)m.seg_start[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    const int pidx = m.seg_prob[lo];
+    const int round = m.seg_round[lo];
+    const int task = (int)(i - (unsigned long long)m.seg_start[lo]);
+    const MultiDesc& d = m.probs[pidx];
+    if (round > 0) {
+      const unsigned need = (unsigned)round * (unsigned)d.a.ntasks;
+      int ok = 1;
+      if (lane == 0) {
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        while (__hip_atomic_load(m.done + pidx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (__hip_atomic_load(m.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+              __builtin_amdgcn_s_memrealtime() - t0 > m.wait_ticks) {
+            ok = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(32);
+        }
+        if (!ok) atomicOr(m.status, 1u);
+      }
+      ok = __shfl(ok, 0);
+      if (!ok) continue;                      // abandoned: the host reports the failure
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    TB2DArgs a = d.a;
+    const int cur = round & 1, nxt = cur ^ 1;
+    a.ez_in = d.f[cur][0]; a.hx_in = d.f[cur][1]; a.hy_in = d.f[cur][2];
+    a.ez_out = d.f[nxt][0]; a.hx_out = d.f[nxt][1]; a.hy_out = d.f[nxt][2];
+    a.jd_in = d.jd[cur]; a.jd_out = d.jd[nxt];
+    a.step0 = round * K;
+    a.ksteps = min(K, a.T - a.step0);
+    a.e_save = nullptr;
+    tb_task<R, K, POLES>(a, task, lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0)
+      __hip_atomic_fetch_add(m.done + pidx, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The same work items without in-kernel waits: one launch per round covers
+// that round's segments [seg0, seg1) and the launch boundary orders rounds.
+template <int R, int K, int POLES>
+__global__ __launch_bounds__(256) void round_tb_kernel(MultiArgs m, int seg0, int seg1,
+                                                       long long first, long long count) {
+  const int lane = threadIdx.x & 63;
+  const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= count) return;
+  const long long i = first + w;
+  int lo = seg0, hi = seg1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.seg_start[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const int pidx = m.seg_prob[lo];
+  const int round = m.seg_round[lo];
+  const int task = (int)(i - m.seg_start[lo]);
+  const MultiDesc& d = m.probs[pidx];
+  TB2DArgs a = d.a;
+  const int cur = round & 1, nxt = cur ^ 1;
+  a.ez_in = d.f[cur][0]; a.hx_in = d.f[cur][1]; a.hy_in = d.f[cur][2];
+  a.ez_out = d.f[nxt][0]; a.hx_out = d.f[nxt][1]; a.hy_out = d.f[nxt][2];
+  a.jd_in = d.jd[cur]; a.jd_out = d.jd[nxt];
+  a.step0 = round * K;
+  a.ksteps = min(K, a.T - a.step0);
+  a.e_save = nullptr;
+  tb_task<R, K, POLES>(a, task, lane);
+}
+
+template <int R, int K>
+static hipError_t launch_round_rk(const MultiArgs& m, int poles, int seg0, int seg1,
+                                  long long first, long long count, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((unsigned)((count + 3) / 4), 1, 1);
+  if (poles == 0) hipLaunchKernelGGL((round_tb_kernel<R, K, 0>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  else if (poles == 1) hipLaunchKernelGGL((round_tb_kernel<R, K, 1>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_round_tb(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
+                           long long first, long long count, hipStream_t s) {
+#define CEEP_RR(RR)                                                                        \
+  switch (K) {                                                                             \
+    case 1: return launch_round_rk<RR, 1>(m, poles, seg0, seg1, first, count, s);           \
+    case 2: return launch_round_rk<RR, 2>(m, poles, seg0, seg1, first, count, s);           \
+    case 3: return launch_round_rk<RR, 3>(m, poles, seg0, seg1, first, count, s);           \
+    case 4: return launch_round_rk<RR, 4>(m, poles, seg0, seg1, first, count, s);           \
+  }                                                                                        \
+  return hipErrorInvalidValue;
+  if (R == 16) { CEEP_RR(16) }
+  if (R == 24) { CEEP_RR(24) }
+  if (R == 32) { CEEP_RR(32) }
+#undef CEEP_RR
+  return hipErrorInvalidValue;
+}
+
+template <int R, int K>
+static hipError_t launch_multi_rk(const MultiArgs& m, int poles, int blocks, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd(blocks, 1, 1);
+  if (poles == 0) hipLaunchKernelGGL((multi_tb_kernel<R, K, 0>), grd, blk, 0, s, m);
+  else if (poles == 1) hipLaunchKernelGGL((multi_tb_kernel<R, K, 1>), grd, blk, 0, s, m);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int R>
+static hipError_t launch_multi_r(const MultiArgs& m, int poles, int K, int blocks, hipStream_t s) {
+  switch (K) {
+    case 1: return launch_multi_rk<R, 1>(m, poles, blocks, s);
+    case 2: return launch_multi_rk<R, 2>(m, poles, blocks, s);
+    case 3: return launch_multi_rk<R, 3>(m, poles, blocks, s);
+    case 4: return launch_multi_rk<R, 4>(m, poles, blocks, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_multi_tb(const MultiArgs& m, int poles, int R, int K, int blocks, hipStream_t s) {
+  switch (R) {
+    case 16: return launch_multi_r<16>(m, poles, K, blocks, s);
+    case 24: return launch_multi_r<24>(m, poles, K, blocks, s);
+    case 32: return launch_multi_r<32>(m, poles, K, blocks, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+int multi_tb_blocks_per_cu(int poles, int R, int K) {
+  int n = 0;
+  const void* f = nullptr;
+#define CEEP_MF(RR, KK) \
+  f = poles ? (const void*)multi_tb_kernel<RR, KK, 1> : (const void*)multi_tb_kernel<RR, KK, 0>
+  if (R == 16) { if (K == 1) CEEP_MF(16, 1); else if (K == 2) CEEP_MF(16, 2); else if (K == 3) CEEP_MF(16, 3); else CEEP_MF(16, 4); }
+  else if (R == 24) { if (K == 1) CEEP_MF(24, 1); else if (K == 2) CEEP_MF(24, 2); else if (K == 3) CEEP_MF(24, 3); else CEEP_MF(24, 4); }
+  else { if (K == 1) CEEP_MF(32, 1); else if (K == 2) CEEP_MF(32, 2); else if (K == 3) CEEP_MF(32, 3); else CEEP_MF(32, 4); }
+#undef CEEP_MF
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, 0) != hipSuccess) return 0;
+  return n;
 }
 
 template <int R, int K>

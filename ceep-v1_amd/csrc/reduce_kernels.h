@@ -10,6 +10,10 @@ hipError_t launch_dft(const float* traces, const float2* tw, int B, int P, int T
 // dst[0:n) = src[0:n) on stream s (float4 kernel when n % 4 == 0).
 hipError_t launch_copy(const float* src, float* dst, long long n, hipStream_t s);
 
+// Zero `bytes` (a multiple of 4) with a kernel, so the clear can sit in a
+// captured graph without a memset node.
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+
 // partial[nblocks] = per-workgroup sums of x[i]^2 over n floats.
 hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,
                         hipStream_t s);

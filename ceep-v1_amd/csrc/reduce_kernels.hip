@@ -85,6 +85,20 @@ hipError_t launch_copy(const float* src, float* dst, long long n, hipStream_t s)
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void zero_kernel(unsigned* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+  const long long n = (long long)(bytes / 4);
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, s, static_cast<unsigned*>(p), n);
+  return hipGetLastError();
+}
+
 hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,
                         hipStream_t s) {
   hipLaunchKernelGGL(sumsq_kernel, dim3(nblocks), dim3(256), 0, s, x, n, partial);

@@ -997,3 +997,229 @@ int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out) {
 }
 
 }  // extern "C"
+
+// ---- uneven batch: many 2D problems in one persistent launch -----------------
+
+struct ceep_fdtd_multi {
+  std::vector<ceep_fdtd*> hs;
+  int K = 4, R = 16, bpw = 1, poles = 0, blocks = 0;
+  ceep::MultiDesc* d_desc = nullptr;
+  long long* d_seg_start = nullptr;
+  int* d_seg_prob = nullptr;
+  int* d_seg_round = nullptr;
+  int nseg = 0;
+  unsigned long long total = 0;
+  unsigned long long* d_queue = nullptr;
+  unsigned* d_done = nullptr;
+  unsigned* d_status = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};   // per mode
+  std::vector<int> rounds;
+  std::vector<int> round_seg;     // [max_rounds + 1] first segment of each round
+  std::vector<long long> seg_start_h;
+};
+
+namespace {
+
+void multi_free(ceep_fdtd_multi* m) {
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (auto& g : m->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
+  dfree(m->d_desc); dfree(m->d_seg_start); dfree(m->d_seg_prob); dfree(m->d_seg_round);
+  dfree(m->d_queue); dfree(m->d_done); dfree(m->d_status);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+// zero every problem's state and traces and the queue counters, then the
+// persistent launch (mode 0) or one launch per round (mode 1)
+int multi_enqueue(ceep_fdtd_multi* m, int mode) {
+  hipStream_t s = m->stream;
+  for (ceep_fdtd* h : m->hs) {
+    const size_t bytes = (size_t)h->box_floats() * sizeof(float);
+    for (int i = 0; i < 2; ++i)
+      for (int f = 0; f < 3; ++f) CEEP_HIP(ceep::launch_zero(h->f[i][f], bytes, s));
+    for (int i = 0; i < 2; ++i)
+      if (h->jd[i]) CEEP_HIP(ceep::launch_zero(h->jd[i], bytes, s));
+    CEEP_HIP(ceep::launch_zero(h->traces, (size_t)h->B * h->P * h->T * sizeof(float), s));
+  }
+  CEEP_HIP(ceep::launch_zero(m->d_queue, sizeof(unsigned long long), s));
+  CEEP_HIP(ceep::launch_zero(m->d_done, m->hs.size() * sizeof(unsigned), s));
+  CEEP_HIP(ceep::launch_zero(m->d_status, sizeof(unsigned), s));
+  ceep::MultiArgs a{};
+  a.probs = m->d_desc; a.nprob = (int)m->hs.size();
+  a.seg_start = m->d_seg_start; a.seg_prob = m->d_seg_prob; a.seg_round = m->d_seg_round;
+  a.nseg = m->nseg; a.total = m->total;
+  a.queue = m->d_queue; a.done = m->d_done; a.status = m->d_status;
+  a.wait_ticks = 25000000ull;      // 0.25 s of the 100 MHz constant clock
+  if (mode == 0) {
+    CEEP_HIP(ceep::launch_multi_tb(a, m->poles, m->R, m->K, m->blocks, s));
+    return CEEP_OK;
+  }
+  const int nrounds = (int)m->round_seg.size() - 1;
+  for (int r = 0; r < nrounds; ++r) {
+    const int s0 = m->round_seg[r], s1 = m->round_seg[r + 1];
+    const long long first = m->seg_start_h[s0];
+    const long long last = s1 < m->nseg ? m->seg_start_h[s1] : (long long)m->total;
+    CEEP_HIP(ceep::launch_round_tb(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
+  }
+  return CEEP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw,
+                           ceep_fdtd_multi** out) {
+  if (!out || !handles || n < 1) return fail(CEEP_EINVAL, "bad argument");
+  *out = nullptr;
+  if (K < 1 || K > 4) return fail(CEEP_EINVAL, "temporal depth must be in [1, 4]");
+  if (R != 16 && R != 24 && R != 32) return fail(CEEP_EINVAL, "rows per wavefront must be 16, 24 or 32");
+  if (bpw < 1) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
+  int rc;
+  int poles = -1;
+  for (int i = 0; i < n; ++i) {
+    ceep_fdtd* h = handles[i];
+    if ((rc = ready_check(h))) return rc;
+    if (h->ndim != 2 || h->has_lorentz)
+      return fail(CEEP_EUNSUPPORTED, "uneven batches take 2D problems without Lorentz poles");
+    if (poles >= 0 && poles != h->poles())
+      return fail(CEEP_EUNSUPPORTED, "every problem of an uneven batch needs the same pole set");
+    poles = h->poles();
+  }
+  ceep_fdtd_multi* m = new ceep_fdtd_multi();
+  m->hs.assign(handles, handles + n);
+  m->K = K; m->R = R; m->bpw = bpw; m->poles = poles;
+  std::vector<ceep::MultiDesc> desc(n);
+  std::vector<int> ntasks(n);
+  int max_rounds = 0;
+  for (int i = 0; i < n; ++i) {
+    ceep_fdtd* h = handles[i];
+    ceep::TB2DArgs& a = desc[i].a;
+    a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
+    a.ca = h->ca; a.cb = h->cb; a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
+    a.ch = h->ch; a.idx = h->idx;
+    a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+    a.T = h->T; a.P = h->P; a.probe_cell = h->probe_cell; a.traces = h->traces;
+    a.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
+    a.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
+    a.bpw = bpw;
+    a.ntasks = a.tiles_x * a.tiles_y * ((h->B + bpw - 1) / bpw);
+    for (int b = 0; b < 2; ++b) {
+      for (int f = 0; f < 3; ++f) desc[i].f[b][f] = h->f[b][f];
+      desc[i].jd[b] = h->jd[b];
+    }
+    desc[i].nrounds = (h->T + K - 1) / K;
+    ntasks[i] = a.ntasks;
+    m->rounds.push_back(desc[i].nrounds);
+    max_rounds = std::max(max_rounds, desc[i].nrounds);
+  }
+  std::vector<long long> seg_start;
+  std::vector<int> seg_prob, seg_round;
+  unsigned long long cum = 0;
+  for (int r = 0; r < max_rounds; ++r)
+    for (int i = 0; i < n; ++i)
+      if (r < desc[i].nrounds) {
+        seg_start.push_back((long long)cum);
+        seg_prob.push_back(i);
+        seg_round.push_back(r);
+        cum += (unsigned long long)ntasks[i];
+      }
+  m->nseg = (int)seg_start.size();
+  m->total = cum;
+  m->seg_start_h = seg_start;
+  for (int k = 0, r = -1; k < m->nseg; ++k)
+    while (r < seg_round[k]) { m->round_seg.push_back(k); ++r; }
+  m->round_seg.push_back(m->nseg);
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) { multi_free(m); return fail(CEEP_EHIP, std::string("device query: ") + hipGetErrorString(e)); }
+  const int per_cu = ceep::multi_tb_blocks_per_cu(poles, R, K);
+  m->blocks = std::max(1, per_cu) * cus;
+  rc = CEEP_OK;
+  if (!rc) rc = dalloc(&m->d_desc, (size_t)n);
+  if (!rc) rc = dalloc(&m->d_seg_start, seg_start.size());
+  if (!rc) rc = dalloc(&m->d_seg_prob, seg_prob.size());
+  if (!rc) rc = dalloc(&m->d_seg_round, seg_round.size());
+  if (!rc) rc = dalloc(&m->d_queue, 1);
+  if (!rc) rc = dalloc(&m->d_done, (size_t)n);
+  if (!rc) rc = dalloc(&m->d_status, 1);
+  if (rc) { multi_free(m); return rc; }
+  e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemcpy(m->d_desc, desc.data(), desc.size() * sizeof(desc[0]), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->d_seg_start, seg_start.data(), seg_start.size() * sizeof(long long), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->d_seg_prob, seg_prob.data(), seg_prob.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->d_seg_round, seg_round.data(), seg_round.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { multi_free(m); return fail(CEEP_EHIP, std::string("multi setup: ") + hipGetErrorString(e)); }
+  *out = m;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode) {
+  if (!m) return fail(CEEP_EINVAL, "null handle");
+  if (mode != 0 && mode != 1) return fail(CEEP_EINVAL, "mode must be 0 (persistent) or 1 (per round)");
+  if (mode == 0)   // kept in the source, switched off: see the note at multi_tb_kernel
+    return fail(CEEP_EUNSUPPORTED,
+                "the persistent single-launch mode is switched off in this build (its runs on "
+                "MI355X did not terminate); use mode 1, one launch per round in one graph");
+  int rc;
+  if (!m->gexec[mode]) {
+    hipGraph_t g = nullptr;
+    CEEP_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    rc = multi_enqueue(m, mode);
+    hipError_t e = hipStreamEndCapture(m->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(CEEP_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&m->gexec[mode], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { m->gexec[mode] = nullptr; return fail(CEEP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+  }
+  CEEP_HIP(hipGraphLaunch(m->gexec[mode], m->stream));
+  unsigned status = 0;
+  CEEP_HIP(hipMemcpyAsync(&status, m->d_status, sizeof(unsigned), hipMemcpyDeviceToHost, m->stream));
+  CEEP_HIP(hipStreamSynchronize(m->stream));
+  if (status)
+    return fail(CEEP_EHIP, "uneven-batch launch: a dependency wait ran out (results invalid)");
+  for (size_t i = 0; i < m->hs.size(); ++i) {
+    m->hs[i]->parity = m->rounds[i] & 1;
+    m->hs[i]->has_run = true;
+  }
+  return CEEP_OK;
+}
+
+int ceep_fdtd_multi_time(ceep_fdtd_multi* m, int mode, int reps, double* ms) {
+  if (!m || !ms || reps < 1) return fail(CEEP_EINVAL, "bad argument");
+  int rc = ceep_fdtd_multi_run(m, mode);       // capture + warm
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  CEEP_HIP(hipEventCreate(&e0));
+  CEEP_HIP(hipEventCreate(&e1));
+  CEEP_HIP(hipEventRecord(e0, m->stream));
+  for (int r = 0; r < reps; ++r) CEEP_HIP(hipGraphLaunch(m->gexec[mode], m->stream));
+  CEEP_HIP(hipEventRecord(e1, m->stream));
+  CEEP_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  CEEP_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  unsigned status = 0;
+  CEEP_HIP(hipMemcpy(&status, m->d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (status) return fail(CEEP_EHIP, "uneven-batch launch: a dependency wait ran out (results invalid)");
+  *ms = t / reps;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_multi_info(ceep_fdtd_multi* m, long long* out) {
+  if (!m || !out) return fail(CEEP_EINVAL, "null argument");
+  out[0] = (long long)m->total; out[1] = m->nseg; out[2] = m->blocks;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_multi_destroy(ceep_fdtd_multi* m) {
+  if (m) multi_free(m);
+  return CEEP_OK;
+}
+
+}  // extern "C"

@@ -152,6 +152,36 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
  * row pitch in floats, out[4] = box origin (grid index of box cell 0). */
 int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out);
 
+/* ---- uneven batches (no counterpart in the reference) ----------------------
+ * Many 2D problems of different grid sizes and step counts, each a fully set
+ * up handle (materials, sources, probes; no Lorentz poles; one pole set for
+ * all), advanced together in ONE persistent kernel launch on a work queue of
+ * (K-step round, problem, window) items; the launch sequence is captured as a
+ * hipGraph on the first run.  Traces land in each handle (ceep_fdtd_get_traces,
+ * ceep_fdtd_dft). */
+typedef struct ceep_fdtd_multi ceep_fdtd_multi;
+
+/* K = steps per round (1..4), R = rows per wavefront window (16, 24, 32),
+ * bpw = TX planes per wavefront.  The handles must outlive the group. */
+int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw,
+                           ceep_fdtd_multi** out);
+
+/* Zero every problem and run all of them to their own total_steps, as a
+ * captured graph (captured on the first call).  mode 1: every round (K steps
+ * of every problem still running) is one launch over all problems' windows;
+ * the graph holds the clears and every round.  mode 0, one persistent launch
+ * over a dependency-ordered work queue, is switched off in this build and
+ * returns CEEP_EUNSUPPORTED. */
+int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode);
+
+/* `reps` graph replays of `mode` bracketed by HIP events; *ms = mean ms per replay. */
+int ceep_fdtd_multi_time(ceep_fdtd_multi* m, int mode, int reps, double* ms);
+
+/* out[0] = queue items, out[1] = (round, problem) segments, out[2] = workgroups. */
+int ceep_fdtd_multi_info(ceep_fdtd_multi* m, long long* out);
+
+int ceep_fdtd_multi_destroy(ceep_fdtd_multi* m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,98 @@
+"""Uneven batch in one persistent launch (GPU): each problem bit-identical to
+its own blocked run and within 1e-5 of the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import ade, fdtd2d
+from oracle import waveforms as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import ceep_amd
+    return ceep_amd
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+SPECS = [  # (nx, ny, n, T, B, seed)
+    (96, 96, 8, 90, 3, 0),
+    (128, 110, 10, 131, 5, 1),
+    (80, 120, 6, 70, 4, 2),
+    (150, 150, 12, 203, 3, 3),
+    (64, 64, 5, 57, 2, 4),
+]
+
+
+def _make(amd, spec, debye, tuning):
+    nx, ny, n, T, B, seed = spec
+    dx = 0.5e-3
+    dt = W.dt_2d_courant(dx, 0.5)
+    w = W.modulated_gaussian(T, dt, 6e9 + seed * 1e9, 8e9, delay_factor=3.0)
+    pos = amd.phantoms.ring_positions((nx / 2, ny / 2), min(nx, ny) / 2 - n - 4, B)
+    s = amd.BatchedFDTD2D(nx=nx, ny=ny, dx=dx, total_steps=T, cpml_thickness=n,
+                          source_positions=pos, probe_positions=pos + [(n + 2, n + 2)],
+                          frequency=6e9, courant=0.5, waveform=w)
+    ph = amd.phantoms.DebyeHeadPhantom2D(nx, ny, seed)
+    m = ph.get_debye_maps()
+    if debye:
+        s.set_phantom(ph)
+    else:
+        s._eps_r[:] = m["eps_inf"] + m["delta_eps"]
+        s._sigma_e[:] = m["sigma"]
+    s.tuning = tuning
+    return s, dt, w, m, pos
+
+
+def _check(amd, debye, K, R, bpw, mode):
+    from ceep_amd.uneven import UnevenBatch
+    tun = {"temporal_depth": K, "rows_per_wave": R, "batch_per_wave": bpw}
+    group = [_make(amd, sp, debye, tun) for sp in SPECS]
+    ub = UnevenBatch([g[0] for g in group], K, R, bpw)
+    print(f"uneven batch mode={mode} K={K} R={R} bpw={bpw} debye={debye} {ub.info()}", flush=True)
+    ub.run(mode)
+    multi = [g[0].probe_data.copy() for g in group]
+    ub.run(mode)                                  # graph replay
+    for g, first in zip(group, multi):
+        assert np.array_equal(g[0].probe_data, first)
+    print(ub.info())
+    for sp, (s, dt, w, m, pos), tr in zip(SPECS, group, multi):
+        alone, *_ = _make(amd, sp, debye, tun)
+        alone.run()
+        assert np.array_equal(tr, alone.probe_data), sp
+        nx, ny, n = sp[0], sp[1], sp[2]
+        poles = [ade.debye_pole(m["delta_eps"], m["tau"], dt)] if debye else None
+        ref = fdtd2d.run_cavity(nx, ny, 0.5e-3, dt, n, pos, pos + [(n + 2, n + 2)], w,
+                                s._eps_r, s._sigma_e, poles=poles)
+        assert rel_l2(tr, ref) < 1e-5, sp
+    ub.close()
+
+
+@pytest.mark.parametrize("debye", [True, False])
+@pytest.mark.parametrize("K,R,bpw", [(4, 16, 1), (3, 24, 2), (2, 16, 4)])
+def test_uneven_per_round_launches(amd, debye, K, R, bpw):
+    from ceep_amd.uneven import UnevenBatch
+    _check(amd, debye, K, R, bpw, UnevenBatch.PER_ROUND)
+
+
+@pytest.mark.skip(reason="persistent single-launch mode switched off: its MI355X runs did not "
+                         "terminate and the cause is not established (the entry point refuses it)")
+@pytest.mark.parametrize("debye", [True, False])
+@pytest.mark.parametrize("K,R,bpw", [(4, 16, 1), (3, 24, 2), (2, 16, 4)])
+def test_uneven_persistent_launch(amd, debye, K, R, bpw):
+    from ceep_amd.uneven import UnevenBatch
+    _check(amd, debye, K, R, bpw, UnevenBatch.PERSISTENT)
+
+
+def test_uneven_persistent_mode_refused(amd):
+    from ceep_amd.uneven import UnevenBatch
+    tun = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 1}
+    group = [_make(amd, sp, True, tun) for sp in SPECS[:2]]
+    ub = UnevenBatch([g[0] for g in group], 4, 16, 1)
+    with pytest.raises(NotImplementedError):
+        ub.run(UnevenBatch.PERSISTENT)
+    ub.close()

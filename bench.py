@@ -64,6 +64,10 @@ def _workload():
     return s, ph, pos, w, dt
 
 
+def _progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def _device_sync():
     # every library call already returns after its stream has drained; the
     # device-wide synchronize is the contract's barrier on top of that
@@ -193,6 +197,88 @@ def _bench_3d(steps_timed=2, T3=1000):
                          "1000 steps, Debye+Lorentz head seed 0 (H and E kernels per step)")}
 
 
+def _bench_uneven(n_problems=64, seed=4, autotune=True):
+    """64 seeded Debye heads (the head generator above scaled to each grid),
+    grids drawn from {256, 384, 512, 600}^2, steps U[600, 1200], 16 TX = RX on a
+    ring of radius 200/600 of the grid, CPML 12-15 cells, each followed by its
+    301-frequency DFT.  The batch runs as one captured graph of per-round
+    launches (launch shape from the coordinate autotuner); the comparison runs
+    the 64 problems one after another with the one-step kernel."""
+    import ceep_amd
+    from ceep_amd.autotune import autotune_uneven
+    from ceep_amd.constants import cfl_timestep_2d
+    from ceep_amd.uneven import UnevenBatch
+    from ceep_amd.waveforms import modulated_gaussian
+
+    rng = np.random.default_rng(seed)
+    dt = cfl_timestep_2d(DX, DX, COURANT)
+    solvers, grids = [], []
+    for i in range(n_problems):
+        g = int(rng.choice([256, 384, 512, 600]))
+        steps = int(rng.integers(600, 1201))
+        ncp = int(rng.integers(12, 16))
+        pos = ceep_amd.phantoms.ring_positions((g / 2, g / 2), 200.0 * g / 600.0, B)
+        w = modulated_gaussian(steps, dt, 2e9, 1.5e9)
+        s = ceep_amd.BatchedFDTD2D(nx=g, ny=g, dx=DX, total_steps=steps, cpml_thickness=ncp,
+                                   source_positions=pos, probe_positions=pos, frequency=2e9,
+                                   courant=COURANT, waveform=w, cpml_R=1e-6)
+        s.set_phantom(ceep_amd.phantoms.DebyeHeadPhantom2D(g, g, 1000 + i))
+        s._build()
+        solvers.append(s)
+        grids.append(g)
+    shape, log = (4, 16, 2), []
+    if autotune:
+        shape, log = autotune_uneven(solvers)
+    ub = UnevenBatch(solvers, *shape)
+    cell_steps = ub.cell_steps
+    full_cells = sum(g * g * B * s.total_steps for g, s in zip(grids, solvers))
+
+    def batched():
+        ub._multi.run(UnevenBatch.PER_ROUND)
+        for s in solvers:
+            s._handle.dft(FREQS)
+
+    def sequential():
+        for s in solvers:
+            s._handle.reset()
+            s._handle.run(copy=False)
+            s._handle.dft(FREQS)
+
+    batched()
+    _device_sync()
+    t0 = time.perf_counter()
+    batched()
+    _device_sync()
+    t_b = time.perf_counter() - t0
+    ms_graph = ub.time(2)
+    info = ub.info()
+    ub.close()
+    for s in solvers:
+        s._handle.apply_tuning({"temporal_depth": 0, "batch_per_thread": 4, "graph": 1})
+    sequential()
+    _device_sync()
+    t0 = time.perf_counter()
+    sequential()
+    _device_sync()
+    t_s = time.perf_counter() - t0
+    return {"value": round(cell_steps / t_b / 1e9, 3), "unit": "GCell-steps/s",
+            "ms_per_batch": round(t_b * 1e3, 2), "ms_graph_only": round(ms_graph, 2),
+            "value_full_grid_convention": round(full_cells / t_b / 1e9, 3),
+            "sequential_one_step_value": round(cell_steps / t_s / 1e9, 3),
+            "sequential_one_step_ms": round(t_s * 1e3, 2),
+            "speedup_vs_sequential": round(t_s / t_b, 3),
+            "launch_shape": {"temporal_depth": shape[0], "rows_per_wave": shape[1],
+                             "batch_per_wave": shape[2]},
+            "autotune_log_ms": [[round(ms, 2), list(sh)] for ms, sh in sorted(log)],
+            "work_items": info["items"], "segments": info["segments"],
+            "cell_steps": cell_steps,
+            "grids": {str(g): grids.count(g) for g in sorted(set(grids))},
+            "workload": (f"{n_problems} seeded Debye heads, grids from 256/384/512/600 squared, "
+                         "steps U[600,1200], 16 TX/RX, CPML 12-15, 301-frequency DFT each; "
+                         "one hipGraph of per-round launches over all problems vs the problems "
+                         "run one after another with the one-step kernel")}
+
+
 def _traffic():
     path = os.path.join(ROOT, "profiles", "step2d_debye_600x600x16_traffic.json")
     if not os.path.exists(path):
@@ -211,6 +297,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--single-run", action="store_true",
                     help="one simulation + DFT with the default tuning, no output (for rocprofv3 --pmc)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -241,6 +328,7 @@ def main():
     if not args.no_autotune:
         from ceep_amd.autotune import autotune_2d
         tuning = autotune_2d(s, reps=2)
+        _progress(f"autotuned 2D launch: {tuning}")
     h = s._handle
     box = h.box_info()
     cells = int(box[0] * box[1])
@@ -279,8 +367,18 @@ def main():
     bytes_launch = cells * (B * BYTES_PER_CELL_PER_TX + BYTES_PER_CELL_SHARED) * steps_per_launch
     achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
 
+    _progress(f"2D workload: {value:.1f} GCell-steps/s")
     adj = None if args.no_adjoint else _adjoint_ratio()
+    if adj:
+        _progress(f"gradient ratio {adj['ratio']}")
     b3 = None if args.no_3d else _bench_3d()
+    if b3:
+        _progress(f"3D: {b3['value']} GCell-steps/s")
+    bu = None if args.no_uneven else _bench_uneven(autotune=not args.no_autotune)
+    if bu:
+        _progress(f"uneven batch: {bu['value']} GCell-steps/s")
+    if not args.no_cpu_baseline:
+        _progress("timing the CPU baseline")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -322,6 +420,7 @@ def main():
         "fwd_adjoint_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
         "bench_3d": b3,
+        "bench_uneven": bu,
         "cpu_baseline": cpu,
         "s_matrix_shape": list(S.shape),
     }

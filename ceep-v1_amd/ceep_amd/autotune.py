@@ -94,3 +94,48 @@ def autotune_2d(solver, reps: int = 2, candidates=None, verbose: bool = False) -
     solver.tuning = best
     solver.tuning_log = log
     return best
+
+
+def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 32),
+                    batch_per_wave=(1, 2, 4), start=(4, 16, 2), verbose: bool = False):
+    """Launch shape (temporal depth K, rows per wavefront R, TX planes per
+    wavefront) for an ``UnevenBatch`` of ``solvers``, by coordinate search:
+    K over ``depths`` at the starting R and planes, then R at the best K, then
+    the planes at the best (K, R).  Each candidate is a captured graph of the
+    whole batch, timed with HIP events after one warm replay.
+
+    Returns ``(best_shape, log)`` with ``log`` a list of ``(ms, shape)``.
+    """
+    from .uneven import UnevenBatch
+
+    timed: Dict[tuple, float] = {}
+    log = []
+
+    def measure(shape):
+        if shape in timed:
+            return timed[shape]
+        k, r, b = shape
+        if r - 2 * k < 4:
+            return float("inf")
+        ub = UnevenBatch(solvers, k, r, b)
+        try:
+            ms = ub.time(reps)
+        finally:
+            ub.close()
+        timed[shape] = ms
+        log.append((ms, shape))
+        if verbose:
+            print(f"  {ms:9.3f} ms  K={k} R={r} planes/wave={b}", flush=True)
+        return ms
+
+    best = tuple(start)
+    for axis, values in ((0, depths), (1, rows), (2, batch_per_wave)):
+        for v in values:
+            cand = list(best)
+            cand[axis] = v
+            cand = tuple(cand)
+            if measure(cand) < measure(best):
+                best = cand
+    for s in solvers:
+        s._handle.reset()
+    return best, log

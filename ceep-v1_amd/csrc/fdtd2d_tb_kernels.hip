@@ -194,7 +194,7 @@ template <int R, int K>
 __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
   constexpr int NO = R - 2 * K;                        // output rows
   const int lane = threadIdx.x & 63;
-  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int task = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (task >= p.ntasks) return;
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = task % tiles;
@@ -370,7 +370,9 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
 template <int R, int K, int POLES>
 __global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
   const int lane = threadIdx.x & 63;
-  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // readfirstlane: the wavefront index is uniform, and saying so keeps the
+  // task decomposition and every derived address in scalar registers
+  const int task = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (task >= p.ntasks) return;                       // uniform per wavefront
   tb_task<R, K, POLES>(p, task, lane);
 }
@@ -456,7 +458,7 @@ template <int R, int K, int POLES>
 __global__ __launch_bounds__(256) void round_tb_kernel(MultiArgs m, int seg0, int seg1,
                                                        long long first, long long count) {
   const int lane = threadIdx.x & 63;
-  const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long w = (long long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (w >= count) return;
   const long long i = first + w;
   int lo = seg0, hi = seg1 - 1;

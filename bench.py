@@ -146,23 +146,27 @@ def _adjoint_ratio(every=50):
         s._handle.dft(freqs)
     _device_sync()
     t_fwd = (time.perf_counter() - t0) / reps
-    def timed(ev):
-        adjoint.gradient(s, eps, sig, freqs, S_t, every=ev)        # warm
+    def timed(ev, fp64=False):
+        adjoint.gradient(s, eps, sig, freqs, S_t, every=ev, fp64_forward=fp64)        # warm
         _device_sync()
         t0 = time.perf_counter()
         for _ in range(reps):
-            adjoint.gradient(s, eps, sig, freqs, S_t, every=ev)
+            adjoint.gradient(s, eps, sig, freqs, S_t, every=ev, fp64_forward=fp64)
         _device_sync()
         return (time.perf_counter() - t0) / reps
 
     t_grad = timed(every)
     t_whole = timed(T)
+    t_grad64 = timed(every, True)
     return {"ratio": round(t_grad / t_fwd, 3), "ms_forward": round(t_fwd * 1e3, 3),
             "ms_forward_plus_adjoint": round(t_grad * 1e3, 3), "checkpoint_every": every,
             "ratio_single_segment": round(t_whole / t_fwd, 3),
+            "ratio_fp64_forward": round(t_grad64 / t_fwd, 3),
             "note": ("ratio: checkpoints every 50 steps, each earlier segment recomputed; "
                      "ratio_single_segment: E of all 800 steps kept from the forward pass "
-                     "(no recomputation, 16.7 GB of HBM)"),
+                     "(no recomputation, 16.7 GB of HBM); ratio_fp64_forward: forward and "
+                     "recomputation on FP64 fields (what this misfit needs for an accurate "
+                     "gradient: its residual is 3e-10 of |S|), adjoint FP32, over the FP32 forward"),
             "workload": ("2D 600x600, 16 TX/RX ring r=200, 800 steps, lossy disk phantom; "
                          "L = sum |S - S_target|^2 at 16 freqs in 1-2.5 GHz; dL/deps_r, dL/dsigma")}
 

@@ -12,6 +12,14 @@ ordinary torch on the host, so autograd supplies dL/dtrace.
 There is no counterpart in the reference (src/ceep/ai/__init__.py is a
 docstring; "Differentiable FDTD" is an unchecked roadmap item).
 
+``fp64_forward=True`` runs the forward and the segment recomputation on FP64
+fields (the adjoint stays FP32).  It is for misfits whose residual
+``S - S_target`` is smaller than what FP32 traces resolve: with TX = RX probes
+the trace is dominated by the source pulse itself, and a scattered signal of
+1e-6 or less of it (the configuration-1 geometry at 400 steps: 5e-6; the
+600 x 600, 800-step workload: 3e-10) is below FP32 trace accuracy (~1e-6), so
+an FP32 forward returns a residual -- and a gradient -- dominated by rounding.
+
 Import torch before the first solver is built (see ``_native``).
 """
 from __future__ import annotations
@@ -24,8 +32,9 @@ class BatchedTraces(torch.autograd.Function):
     """traces = F(eps_r, sigma) for a built-or-buildable ``BatchedFDTD2D``."""
 
     @staticmethod
-    def forward(ctx, eps_r, sigma_e, solver, every):
+    def forward(ctx, eps_r, sigma_e, solver, every, fp64_forward=False):
         solver.update_materials(eps_r.detach().cpu().numpy(), sigma_e.detach().cpu().numpy())
+        solver._handle.set_tuning("fp64_forward", int(bool(fp64_forward)))
         tr = solver._handle.run_checkpointed(int(every))
         solver.probe_data = tr
         ctx.solver = solver
@@ -34,12 +43,13 @@ class BatchedTraces(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         ge, gs = ctx.solver._handle.adjoint(g.detach().cpu().numpy())
-        return torch.from_numpy(ge), torch.from_numpy(gs), None, None
+        return torch.from_numpy(ge), torch.from_numpy(gs), None, None, None
 
 
-def traces(solver, eps_r: torch.Tensor, sigma_e: torch.Tensor, every: int = 50) -> torch.Tensor:
+def traces(solver, eps_r: torch.Tensor, sigma_e: torch.Tensor, every: int = 50,
+           fp64_forward: bool = False) -> torch.Tensor:
     """(B, P, T) float64 traces, differentiable in eps_r and sigma_e ((nx, ny) float64)."""
-    return BatchedTraces.apply(eps_r, sigma_e, solver, every)
+    return BatchedTraces.apply(eps_r, sigma_e, solver, every, fp64_forward)
 
 
 def s_matrix(traces_: torch.Tensor, freqs, dt: float) -> torch.Tensor:
@@ -58,11 +68,12 @@ def s_misfit(traces_: torch.Tensor, freqs, dt: float, s_target: torch.Tensor) ->
     return (d.real ** 2 + d.imag ** 2).sum()
 
 
-def gradient(solver, eps_r, sigma_e, freqs, s_target, every: int = 50):
+def gradient(solver, eps_r, sigma_e, freqs, s_target, every: int = 50,
+             fp64_forward: bool = False):
     """(L, dL/d eps_r, dL/d sigma) as NumPy for the S misfit at ``freqs``."""
     e = torch.as_tensor(np.asarray(eps_r, dtype=np.float64)).clone().requires_grad_(True)
     s = torch.as_tensor(np.asarray(sigma_e, dtype=np.float64)).clone().requires_grad_(True)
-    tr = traces(solver, e, s, every)
+    tr = traces(solver, e, s, every, fp64_forward)
     loss = s_misfit(tr, freqs, solver.dt, torch.as_tensor(s_target))
     loss.backward()
-    return float(loss), e.grad.numpy(), s.grad.numpy()
+    return float(loss.detach()), e.grad.numpy(), s.grad.numpy()

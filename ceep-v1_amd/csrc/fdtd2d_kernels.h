@@ -72,6 +72,34 @@ hipError_t launch_adj2d(const Adj2DArgs& a, hipStream_t s);
 hipError_t launch_adj_seed(float* le, const int* probe_cell, const float* dtr, int B, int P,
                            int T, int step, long long bstride, hipStream_t s);
 
+// FP64 fields for the gradient path's forward and recomputation passes
+// (non-dispersive; same layout and pitch as the FP32 buffers, in doubles).
+struct Step2D64Args {
+  int Mx, My, pitch, B;
+  long long bstride;
+  const double* ez_in;
+  const double* hx_in;
+  const double* hy_in;
+  double* ez_out;
+  double* hx_out;
+  double* hy_out;
+  const double* ca;              // 0 off the E interior
+  const double* cb;
+  double ch;                     // (dt / mu0) * (1 / dx)
+  double idx;                    // 1 / dx
+  const int* src_cell;
+  const double* wave;            // [T] or [B][T]
+  long long wave_stride_b;
+  int step, T, P;
+  const short* probe_head;
+  const short* probe_next;
+  double* traces;                // [B][P][T]
+  float* traces32;               // the same samples rounded to FP32
+  float* e_save;                 // if set: E after this step (FP32) [B][plane]
+};
+
+hipError_t launch_step2d_f64(const Step2D64Args& a, hipStream_t s);
+
 // poles: bit 0 = Debye pole present, bit 1 = Lorentz pole present.
 // bb: batch planes per thread (1, 2, 4, 8).
 hipError_t launch_step2d(const Step2DArgs& a, int poles, int bb, hipStream_t s);

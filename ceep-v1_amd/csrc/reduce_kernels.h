@@ -14,6 +14,9 @@ hipError_t launch_copy(const float* src, float* dst, long long n, hipStream_t s)
 // captured graph without a memset node.
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 
+// dst[0:n) = (float)src[0:n) on stream s.
+hipError_t launch_to_f32(const double* src, float* dst, long long n, hipStream_t s);
+
 // partial[nblocks] = per-workgroup sums of x[i]^2 over n floats.
 hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,
                         hipStream_t s);

@@ -99,6 +99,20 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void to_f32_kernel(const double* __restrict__ src,
+                                                     float* __restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+hipError_t launch_to_f32(const double* src, float* dst, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(to_f32_kernel, dim3(blocks), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,
                         hipStream_t s) {
   hipLaunchKernelGGL(sumsq_kernel, dim3(nblocks), dim3(256), 0, s, x, n, partial);

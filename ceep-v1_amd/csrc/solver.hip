@@ -113,6 +113,16 @@ struct ceep_fdtd {
   bool ck_valid = false;
   int estore_seg = -1;         // segment whose forward E the store holds, or -1
 
+  // FP64 forward of the gradient path (tuning key 5): fields, maps, waveform,
+  // traces and checkpoints in doubles; the adjoint itself stays FP32
+  int fwd64 = 0;
+  double* f64[2][3] = {{nullptr}};
+  double *ca64 = nullptr, *cb64 = nullptr, *wave64 = nullptr, *traces64 = nullptr;
+  std::vector<double> wave_h;  // the waveform as given (FP64), [T] or [B][T]
+  std::vector<double*> ck64;   // ck_count x 3 FP64 checkpoints
+  bool maps64_dirty = true;
+  bool traces64_valid = false; // the last run was the FP64 forward
+
   bool mats_dirty = true;
   bool has_run = false;
   int bb = 4;
@@ -137,6 +147,8 @@ struct ceep_fdtd {
   void free_adjoint() {
     for (auto& p : ck) if (p) (void)hipFree(p);
     ck.clear();
+    for (auto& p : ck64) if (p) (void)hipFree(p);
+    ck64.clear();
     ck_count = 0;
     ck_valid = false;
     if (estore) (void)hipFree(estore);
@@ -381,6 +393,75 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
   return CEEP_OK;
 }
 
+// FP64 buffers of the gradient path's forward: fields (ping-pong), ca/cb
+// (fdtd_2d_batched.py:152-159, formed in FP64 and kept in FP64; 0 off the E
+// interior), the waveform and the traces.
+int prepare_f64(ceep_fdtd* h) {
+  int rc;
+  const size_t nbox = (size_t)h->box_floats();
+  for (int i = 0; i < 2; ++i)
+    for (int f = 0; f < 3; ++f)
+      if (!h->f64[i][f] && (rc = dalloc(&h->f64[i][f], nbox))) return rc;
+  if (!h->traces64 && (rc = dalloc(&h->traces64, (size_t)h->B * h->P * h->T))) return rc;
+  hipStream_t s = h->stream;
+  if (!h->wave64) {
+    if ((rc = dalloc(&h->wave64, h->wave_h.size()))) return rc;
+    CEEP_HIP(hipMemcpyAsync(h->wave64, h->wave_h.data(), h->wave_h.size() * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+  }
+  if (h->maps64_dirty || !h->ca64) {
+    const long long nmap = h->plane;
+    std::vector<double> ca(nmap, 0.0), cb(nmap, 0.0);
+    const double dt = h->dt;
+    for (int a = 1; a < h->M[0] - 1; ++a)
+      for (int b = 1; b < h->M[1] - 1; ++b) {
+        const long long g = h->full_index(a + h->org, b + h->org, 0);
+        const long long o = h->box_index(a, b, 0);
+        const double eps = h->eps_r[g] * kEps0;
+        const double sv = h->sigma_e[g] * dt / (2.0 * eps);
+        ca[o] = (1.0 - sv) / (1.0 + sv);
+        cb[o] = (dt / eps) / (1.0 + sv);
+      }
+    if (!h->ca64 && (rc = dalloc(&h->ca64, (size_t)nmap))) return rc;
+    if (!h->cb64 && (rc = dalloc(&h->cb64, (size_t)nmap))) return rc;
+    CEEP_HIP(hipMemcpyAsync(h->ca64, ca.data(), nmap * sizeof(double), hipMemcpyHostToDevice, s));
+    CEEP_HIP(hipMemcpyAsync(h->cb64, cb.data(), nmap * sizeof(double), hipMemcpyHostToDevice, s));
+    CEEP_HIP(hipStreamSynchronize(s));   // the host vectors go out of scope
+    h->maps64_dirty = false;
+  }
+  return CEEP_OK;
+}
+
+// Enqueue FP64 steps [step0, step0 + nsteps) from FP64 buffer `start`; with
+// e_save, E after step s goes (as FP32) to slot s + 1 - step0.
+int launch_range64(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) {
+  ceep::Step2D64Args a{};
+  a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
+  a.ca = h->ca64; a.cb = h->cb64;
+  a.ch = (h->dt / kMu0) * (1.0 / h->dx);
+  a.idx = 1.0 / h->dx;
+  a.src_cell = h->src2; a.wave = h->wave64; a.wave_stride_b = h->wave_stride;
+  a.T = h->T; a.P = h->P;
+  a.probe_head = h->probe_head; a.probe_next = h->probe_next;
+  a.traces = h->traces64; a.traces32 = h->traces;
+  for (int st = step0; st < step0 + nsteps; ++st) {
+    const int i = (start + st - step0) & 1, o = i ^ 1;
+    a.ez_in = h->f64[i][0]; a.hx_in = h->f64[i][1]; a.hy_in = h->f64[i][2];
+    a.ez_out = h->f64[o][0]; a.hx_out = h->f64[o][1]; a.hy_out = h->f64[o][2];
+    a.step = st;
+    a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
+    CEEP_HIP(ceep::launch_step2d_f64(a, h->stream));
+  }
+  return CEEP_OK;
+}
+
+// copy n doubles with the float4 copy kernel (every buffer is 16-byte aligned)
+int copy64(const double* src, double* dst, long long n, hipStream_t s) {
+  CEEP_HIP(ceep::launch_copy(reinterpret_cast<const float*>(src), reinterpret_cast<float*>(dst),
+                             2 * n, s));
+  return CEEP_OK;
+}
+
 int ready_check(ceep_fdtd* h) {
   if (!h) return fail(CEEP_EINVAL, "null handle");
   if (!h->have_sources) return fail(CEEP_ESTATE, "sources not set (ceep_fdtd_set_sources)");
@@ -392,6 +473,7 @@ int ready_check(ceep_fdtd* h) {
 // continues from the fields it finished with; _build starts them at zero).
 // Traces are cleared first.  Caller synchronizes.
 int run_device(ceep_fdtd* h) {
+  h->traces64_valid = false;
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), h->stream));
   const int start = h->parity;
   int rc = CEEP_OK;
@@ -538,6 +620,9 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   dfree(h->dk1); dfree(h->dk2); dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
   dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next); dfree(h->probe_cell);
   dfree(h->traces); dfree(h->partial); dfree(h->tw); dfree(h->S);
+  for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 3; ++k) dfree(h->f64[i][k]);
+  dfree(h->ca64); dfree(h->cb64); dfree(h->wave64); dfree(h->traces64);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CEEP_OK;
@@ -565,6 +650,7 @@ int ceep_fdtd_set_materials(ceep_fdtd* h, const double* eps_r, const double* sig
     return fail(CEEP_EINVAL, "2D TMz takes no magnetic maps (the reference fixes da=1, db=dt/mu0)");
   }
   h->mats_dirty = true;
+  h->maps64_dirty = true;
   return CEEP_OK;
 }
 
@@ -615,6 +701,8 @@ int ceep_fdtd_set_sources(ceep_fdtd* h, const int64_t* pos, const double* wavefo
   std::vector<float> w(nw);
   for (size_t i = 0; i < nw; ++i) w[i] = (float)waveform[i];
   h->wave_stride = per_batch ? h->T : 0;
+  h->wave_h.assign(waveform, waveform + nw);
+  dfree(h->wave64);
   dfree(h->wave);
   int rc = dalloc(&h->wave, nw);
   if (rc) return rc;
@@ -683,6 +771,10 @@ int ceep_fdtd_get_traces(ceep_fdtd* h, double* out) {
   if (!h || !out) return fail(CEEP_EINVAL, "null argument");
   if (!h->has_run) return fail(CEEP_ESTATE, "no run yet");
   const size_t n = (size_t)h->B * h->P * h->T;
+  if (h->traces64_valid) {
+    if (n) CEEP_HIP(hipMemcpy(out, h->traces64, n * sizeof(double), hipMemcpyDeviceToHost));
+    return CEEP_OK;
+  }
   std::vector<float> tmp(n);
   if (n) CEEP_HIP(hipMemcpy(tmp.data(), h->traces, n * sizeof(float), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n; ++i) out[i] = tmp[i];
@@ -796,6 +888,55 @@ int ceep_fdtd_time_runs(ceep_fdtd* h, int reps, double* ms_per_run) {
   return CEEP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The FP64 forward of ceep_fdtd_run_checkpointed: FP64 checkpoints every
+// ck_every steps, the last segment's E kept (as FP32) in the store, FP64
+// traces; the final state is also left, rounded, in the FP32 fields.
+int run_checkpointed64(ceep_fdtd* h, int nck, double* traces_out) {
+  int rc;
+  const long long nbox = h->box_floats();
+  const int every = h->ck_every;
+  hipStream_t st = h->stream;
+  if ((rc = prepare_f64(h))) return rc;
+  if ((int)h->ck64.size() != nck * 3) {
+    for (auto& p : h->ck64) dfree(p);
+    h->ck64.assign((size_t)nck * 3, nullptr);
+    for (auto& p : h->ck64)
+      if ((rc = dalloc(&p, (size_t)nbox))) { h->free_adjoint(); return rc; }
+  }
+  for (int f = 0; f < 3; ++f) CEEP_HIP(ceep::launch_zero(h->f64[0][f], nbox * sizeof(double), st));
+  CEEP_HIP(ceep::launch_zero(h->traces64, (size_t)h->B * h->P * h->T * sizeof(double), st));
+  int cur = 0;
+  for (int k = 0; k < nck; ++k) {
+    for (int f = 0; f < 3; ++f)
+      if ((rc = copy64(h->f64[cur][f], h->ck64[(size_t)k * 3 + f], nbox, st))) return rc;
+    const int s0 = k * every, n = std::min(every, h->T - s0);
+    float* es = nullptr;
+    if (k == nck - 1) {
+      CEEP_HIP(ceep::launch_to_f32(h->f64[cur][0], h->estore, nbox, st));
+      es = h->estore;
+    }
+    if ((rc = launch_range64(h, cur, s0, n, es))) return rc;
+    cur ^= n & 1;
+  }
+  for (int f = 0; f < 3; ++f) CEEP_HIP(ceep::launch_to_f32(h->f64[cur][f], h->f[0][f], nbox, st));
+  h->parity = 0;
+  CEEP_HIP(hipStreamSynchronize(st));
+  h->has_run = true;
+  h->traces64_valid = true;
+  h->ck_valid = true;
+  h->estore_seg = nck - 1;
+  if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
+  return CEEP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   int rc = ready_check(h);
   if (rc) return rc;
@@ -804,13 +945,14 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   if (every < 1) return fail(CEEP_EINVAL, "checkpoint interval must be >= 1");
   const int nck = (h->T + every - 1) / every;
   h->ck_valid = false;
-  if (nck != h->ck_count) {     // checkpoint buffers are kept across calls
+  // checkpoint buffers (FP32, or FP64 for the FP64 forward) are kept across calls
+  if (!h->fwd64 && h->ck.size() != (size_t)nck * 3) {
     for (auto& p : h->ck) dfree(p);
     h->ck.assign((size_t)nck * 3, nullptr);
     for (auto& p : h->ck)
       if ((rc = dalloc(&p, h->box_floats()))) { h->free_adjoint(); return rc; }
-    h->ck_count = nck;
   }
+  h->ck_count = nck;
   if (every != h->ck_every) dfree(h->estore);   // its slot count follows the interval
   h->ck_every = every;
   h->estore_seg = -1;
@@ -819,6 +961,8 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   if (!h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * h->box_floats()))) return rc;
   if ((rc = zero_state(h))) return rc;
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), h->stream));
+  if (h->fwd64) return run_checkpointed64(h, nck, traces_out);
+  h->traces64_valid = false;
   int cur = 0;
   for (int k = 0; k < nck; ++k) {
     for (int f = 0; f < 3; ++f)
@@ -889,7 +1033,13 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     const int s0 = k * every, n = std::min(every, T - s0);
     // recompute the segment from its checkpoint, storing E (unless the
     // store already holds this segment)
-    if (k != h->estore_seg) {
+    if (k != h->estore_seg && h->fwd64) {
+      for (int f = 0; f < 3; ++f)
+        if ((rc = copy64(h->ck64[(size_t)k * 3 + f], h->f64[0][f], (long long)nbox, st))) return rc;
+      CEEP_HIP(ceep::launch_to_f32(h->ck64[(size_t)k * 3], h->estore, (long long)nbox, st));
+      if ((rc = launch_range64(h, 0, s0, n, h->estore))) return rc;
+      h->estore_seg = k;
+    } else if (k != h->estore_seg) {
       for (int f = 0; f < 3; ++f)
         CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * 3 + f], h->f[0][f], (long long)nbox, st));
       CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * 3], h->estore, (long long)nbox, st));
@@ -982,6 +1132,11 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 4) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
     h->tb_bpw = value;
+  } else if (key == 5) {
+    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "forward precision must be 0 (FP32) or 1 (FP64)");
+    if (value && h->ndim != 2) return fail(CEEP_EUNSUPPORTED, "the FP64 forward covers the 2D gradient path");
+    if (value != h->fwd64) h->ck_valid = false;
+    h->fwd64 = value;
   } else {
     return fail(CEEP_EINVAL, "unknown tuning key");
   }

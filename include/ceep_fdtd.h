@@ -140,12 +140,17 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out);
 int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
                       double* grad_sigma);
 
-/* Tuning knobs (results are the same to FP32 rounding for every setting):
+/* Tuning knobs (keys 0-4 give the same results to FP32 rounding for every setting):
  *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8)
  *   key 1: replay the time loop as a captured hipGraph (0/1)
  *   key 2: 2D temporal depth K = steps per launch (0 = one-step kernel, 1..4)
  *   key 3: 2D rows per wavefront window of the blocked kernel (16, 24, 32)
- *   key 4: 2D batch planes per wavefront of the blocked kernel (>= 1) */
+ *   key 4: 2D batch planes per wavefront of the blocked kernel (>= 1)
+ *   key 5: precision of the gradient path's forward and recomputation passes
+ *          (0 = FP32, 1 = FP64; 2D non-dispersive): with 1,
+ *          ceep_fdtd_run_checkpointed steps FP64 fields and returns FP64
+ *          traces, for misfits whose residual lies below FP32 trace accuracy;
+ *          the adjoint stays FP32.  Not a launch-shape knob: results differ. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* Geometry of the stepped box: out[0..2] = box size per axis, out[3] =

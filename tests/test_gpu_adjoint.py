@@ -72,6 +72,25 @@ def test_adjoint_matches_autograd(amd, reference, every):
     assert np.all(ge[:N + 1] == 0) and np.all(gs[:, :N + 1] == 0)   # CPML and wall cells
 
 
+@pytest.mark.parametrize("every", [50, 7])
+def test_adjoint_fp64_forward_matches_autograd(amd, reference, every):
+    """The FP64 forward (bitwise the oracle's FP64 traces) with the FP32 adjoint."""
+    from ceep_amd import adjoint
+    r = reference
+    s = _solver(amd, r["dt"], r["w"], r["pos"])
+    loss, ge, gs = adjoint.gradient(s, r["eps"], r["sig"], FREQS, r["St"], every=every,
+                                    fp64_forward=True)
+    ref_tr = fdtd2d.run_cavity(NX, NY, DX, r["dt"], N, r["pos"], r["pos"], r["w"], r["eps"], r["sig"])
+    d = float(np.abs(s.probe_data - ref_tr).max() / np.abs(ref_tr).max())
+    ee, es = rel_l2(ge, r["ge"]), rel_l2(gs, r["gs"])
+    print(f"fp64 forward every={every}: traces max |diff| / max |ref| {d:.1e} "
+          f"(bitwise: {np.array_equal(s.probe_data, ref_tr)}); loss {loss:.6e} vs {r['loss']:.6e}; "
+          f"dL/deps rel-L2 {ee:.2e}, dL/dsigma rel-L2 {es:.2e}")
+    assert d <= 1e-13
+    assert loss == pytest.approx(r["loss"], rel=1e-10)
+    assert ee < 1e-4 and es < 1e-4
+
+
 @pytest.mark.parametrize("tuning", [
     {"temporal_depth": 0},
     {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4},

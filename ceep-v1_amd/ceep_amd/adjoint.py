@@ -1,10 +1,12 @@
 """Gradients of S-parameter misfits with respect to the eps_r and sigma maps.
 
-``BatchedTraces`` is a ``torch.autograd.Function`` whose forward runs the
-batched 2D solver on the GPU (checkpointing the full state every ``every``
-steps) and whose backward runs the hand-written time-reversed adjoint kernel
-(``ceep_fdtd_adjoint``: each segment is recomputed from its checkpoint, then
-stepped backward) to return dL/d eps_r and dL/d sigma per cell.  Everything
+``BatchedTraces`` is a ``torch.autograd.Function`` whose forward runs a
+batched solver on the GPU -- ``BatchedFDTD2D`` (non-dispersive) or
+``BatchedFDTD3D`` (with its Debye / Lorentz poles) -- checkpointing the full
+state every ``every`` steps, and whose backward runs the hand-written
+time-reversed adjoint kernels (``ceep_fdtd_adjoint``: each segment is
+recomputed from its checkpoint, then stepped backward) to return dL/d eps_r
+and dL/d sigma per cell.  Everything
 downstream of the traces -- the running DFT to the S-matrix
 (src/ceep/solvers/dft.py:75-90, layout of s_params.py:99) and any loss -- is
 ordinary torch on the host, so autograd supplies dL/dtrace.
@@ -48,7 +50,7 @@ class BatchedTraces(torch.autograd.Function):
 
 def traces(solver, eps_r: torch.Tensor, sigma_e: torch.Tensor, every: int = 50,
            fp64_forward: bool = False) -> torch.Tensor:
-    """(B, P, T) float64 traces, differentiable in eps_r and sigma_e ((nx, ny) float64)."""
+    """(B, P, T) float64 traces, differentiable in eps_r and sigma_e (grid-shaped float64)."""
     return BatchedTraces.apply(eps_r, sigma_e, solver, every, fp64_forward)
 
 

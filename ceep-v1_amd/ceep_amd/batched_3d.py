@@ -157,6 +157,19 @@ class BatchedFDTD3D:
         self.probe_data = np.zeros((self.batch, len(self.probe_positions), self.total_steps))
         self._built = True
 
+    def update_materials(self, eps_r, sigma_e) -> None:
+        """Replace the eps_r / sigma maps of a built solver in place (device
+        buffers, pole maps and launch setup are kept; fields are not reset)."""
+        self._eps_r[:] = eps_r
+        self._sigma_e[:] = sigma_e
+        if not self._built:
+            self._build()
+            return
+        magnetic = bool(np.any(self._mu_r != 1.0) or np.any(self._sigma_m != 0.0))
+        self._handle.set_materials(self._eps_r, self._sigma_e,
+                                   self._mu_r if magnetic else None,
+                                   self._sigma_m if magnetic else None)
+
     def run(self, verbose: bool = False) -> Dict[int, Dict[int, np.ndarray]]:
         """fdtd_3d_batched.py:886-985 on the GPU."""
         if not self._built:

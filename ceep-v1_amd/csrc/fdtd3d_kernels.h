@@ -31,7 +31,58 @@ struct Step3DArgs {
   const short* probe_head;
   const short* probe_next;
   float* traces;                  // [B][P][T]
+  float* e_save;                  // if set: E after this step, [3][B][vol] (adjoint store)
+  long long e_cstride;            // floats per component block of e_save (B * bstride)
 };
+
+// One adjoint (time-reversed) step of the 3D update with the Debye/Lorentz
+// ADE, as three kernels over the adjoint state (lam = adjoint of E, hbar =
+// adjoint of H, ad/al/bl = adjoints of J_Debye, J_Lorentz, J_Lorentz_prev):
+//   A (pointwise): G = k2D ad + k2L al;  Et = lam + G;  lam <- ca Et - G;
+//       q = cb/dx Et;  ad <- k1D ad - cb gammaD Et;  al <- k1L al + bl - cb Et;
+//       bl <- k3L al;  g_ca += Et e^s;  g_cbi += Et (e^{s+1} - source - ca e^s)
+//   B: hbar <- da hbar + curl^T q        (q taken at the +1 neighbours)
+//   C: lam  <- lam + grad^T (dbi hbar)   (hbar taken at the -1 neighbours),
+//       then lam_z += dL/dtrace(step - 1) at the probes
+// g_ca/g_cbi are per-batch-chunk partial planes (gstride floats apart).
+struct Adj3DArgs {
+  int Mx, My, Mz, pitch, B;
+  long long sx, bstride;
+  float* lam[3];
+  float* hbar[3];
+  float* q[3];
+  float* ad[3];
+  float* al[3];
+  float* bl[3];
+  const float* ca;
+  const float* cb;
+  const float* da;                // null -> da = 1, dbi = ch
+  const float* dbi;
+  const float* d_k1;
+  const float* d_k2dt;
+  const float* l_k1;
+  const float* l_k2dt;
+  const float* l_k3;
+  float ch, idx;
+  const float* e_s;               // E at step s   [3][B][vol]
+  const float* e_s1;              // E at step s+1 [3][B][vol]
+  long long e_cstride;
+  const long long* src_cell;
+  const float* wave;
+  long long wave_stride_b;
+  int step, T, P;
+  const float* dtr;               // [B][P][T]
+  const short* probe_head;
+  const short* probe_next;
+  float* g_ca;
+  float* g_cbi;
+  long long gstride;
+};
+
+hipError_t launch_adj3d(const Adj3DArgs& a, int poles, int bb, hipStream_t s);
+// lam_z[b][probe cell] += dtr[b][q][step] for every TX and probe
+hipError_t launch_adj3d_seed(float* lam_z, const long long* probe_cell, const float* dtr,
+                             int B, int P, int T, int step, long long bstride, hipStream_t s);
 
 hipError_t launch_h3d(const Step3DArgs& a, int bb, hipStream_t s);
 hipError_t launch_e3d(const Step3DArgs& a, int poles, int bb, hipStream_t s);

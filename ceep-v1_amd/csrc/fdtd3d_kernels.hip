@@ -105,6 +105,11 @@ __global__ __launch_bounds__(256) void e3d_kernel(Step3DArgs p) {
       p.ey[o] = eyn;
       if (cell == p.src_cell[bt]) ezn += p.wave[(long long)bt * p.wave_stride_b + p.step];
       p.ez[o] = ezn;
+      if (p.e_save) {
+        p.e_save[o] = exn;
+        p.e_save[p.e_cstride + o] = eyn;
+        p.e_save[2 * p.e_cstride + o] = ezn;
+      }
     }
     for (int q = probe0; q >= 0; q = p.probe_next[q])
       p.traces[((long long)bt * p.P + q) * p.T + p.step] = ezn;

@@ -92,6 +92,7 @@ struct ceep_fdtd {
   short* probe_head = nullptr;
   short* probe_next = nullptr;
   int* probe_cell = nullptr;   // [P] 2D box-linear cell of each probe, -1 outside
+  long long* probe_cell3 = nullptr;  // [P] 3D box-linear cell of each probe, -1 outside
   float* traces = nullptr;
   float* partial = nullptr;
   float2* tw = nullptr;        // DFT twiddles [T][F] for tw_freqs
@@ -110,6 +111,7 @@ struct ceep_fdtd {
   float* g_ca = nullptr;
   float* g_cbi = nullptr;
   float* d_dtr = nullptr;
+  float* adj3[6][3] = {{nullptr}};  // 3D: lam, hbar, q, Debye, Lorentz, Lorentz-prev adjoints
   bool ck_valid = false;
   int estore_seg = -1;         // segment whose forward E the store holds, or -1
 
@@ -155,6 +157,8 @@ struct ceep_fdtd {
     estore = nullptr;
     estore_seg = -1;
     for (auto& row : adj)
+      for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
+    for (auto& row : adj3)
       for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
     if (g_ca) (void)hipFree(g_ca);
     if (g_cbi) (void)hipFree(g_cbi);
@@ -382,10 +386,12 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     a.T = h->T; a.P = h->P;
     a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
     const int bb = h->bb > 4 ? 4 : h->bb;
+    a.e_cstride = h->box_floats();
     for (int st = step0; st < step_end; ++st) {
       const int i = (start + st - step0) & 1, o = i ^ 1;
       for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
       a.step = st;
+      a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * 3 * h->box_floats() : nullptr;
       CEEP_HIP(ceep::launch_h3d(a, bb, s));
       CEEP_HIP(ceep::launch_e3d(a, poles, bb, s));
     }
@@ -618,7 +624,7 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   for (int k = 0; k < 3; ++k) { dfree(h->jd[k]); dfree(h->jl[0][k]); dfree(h->jl[1][k]); }
   dfree(h->ca); dfree(h->cb); dfree(h->da); dfree(h->dbi);
   dfree(h->dk1); dfree(h->dk2); dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
-  dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next); dfree(h->probe_cell);
+  dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next); dfree(h->probe_cell); dfree(h->probe_cell3);
   dfree(h->traces); dfree(h->partial); dfree(h->tw); dfree(h->S);
   for (int i = 0; i < 2; ++i)
     for (int k = 0; k < 3; ++k) dfree(h->f64[i][k]);
@@ -749,6 +755,19 @@ int ceep_fdtd_set_probes(ceep_fdtd* h, const int64_t* pos) {
         pc[q] = (int)h->box_index((int)(p[0] - h->org), (int)(p[1] - h->org), 0);
     }
     CEEP_HIP(hipMemcpyAsync(h->probe_cell, pc.data(), pc.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  }
+  if (h->ndim == 3 && h->P > 0) {
+    std::vector<long long> pc(h->P, -1);
+    for (int q = 0; q < h->P; ++q) {
+      const int64_t* p = pos + (size_t)q * 3;
+      if (in_box(h, p, false))
+        pc[q] = h->box_index((int)(p[0] - h->org), (int)(p[1] - h->org), (int)(p[2] - h->org));
+    }
+    dfree(h->probe_cell3);
+    int rc = dalloc(&h->probe_cell3, pc.size());
+    if (rc) return rc;
+    CEEP_HIP(hipMemcpyAsync(h->probe_cell3, pc.data(), pc.size() * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+    CEEP_HIP(hipStreamSynchronize(h->stream));   // pc goes out of scope
   }
   CEEP_HIP(hipMemcpyAsync(h->probe_head, head.data(), head.size() * sizeof(short), hipMemcpyHostToDevice, h->stream));
   CEEP_HIP(hipMemcpyAsync(h->probe_next, next.data(), next.size() * sizeof(short), hipMemcpyHostToDevice, h->stream));
@@ -933,6 +952,189 @@ int run_checkpointed64(ceep_fdtd* h, int nck, double* traces_out) {
   return CEEP_OK;
 }
 
+// The live 3D state a checkpoint holds: six fields, then the ADE currents
+// (Debye J per component, both Lorentz buffers per component).
+std::vector<float*> state3d(ceep_fdtd* h) {
+  std::vector<float*> v;
+  for (int k = 0; k < 6; ++k) v.push_back(h->f[0][k]);
+  for (int k = 0; k < 3; ++k) {
+    if (h->has_debye) v.push_back(h->jd[k]);
+    if (h->has_lorentz) { v.push_back(h->jl[0][k]); v.push_back(h->jl[1][k]); }
+  }
+  return v;
+}
+
+// 3D forward with a checkpoint of the whole state every `every` steps and
+// the last segment's E (three components) kept in the store.
+int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
+  int rc;
+  const int nck = (h->T + every - 1) / every;
+  const long long nbox = h->box_floats();
+  hipStream_t st = h->stream;
+  std::vector<float*> live = state3d(h);
+  const size_t nstate = live.size();
+  h->ck_valid = false;
+  if (h->ck.size() != (size_t)nck * nstate) {
+    for (auto& p : h->ck) dfree(p);
+    h->ck.assign((size_t)nck * nstate, nullptr);
+    for (auto& p : h->ck)
+      if ((rc = dalloc(&p, (size_t)nbox))) { h->free_adjoint(); return rc; }
+  }
+  h->ck_count = nck;
+  if (every != h->ck_every) dfree(h->estore);
+  h->ck_every = every;
+  h->estore_seg = -1;
+  if (!h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * 3 * nbox))) {
+    h->free_adjoint();
+    return rc;
+  }
+  if ((rc = zero_state(h))) return rc;
+  CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), st));
+  h->traces64_valid = false;
+  for (int k = 0; k < nck; ++k) {
+    for (size_t i = 0; i < nstate; ++i)
+      CEEP_HIP(ceep::launch_copy(live[i], h->ck[(size_t)k * nstate + i], nbox, st));
+    const int s0 = k * every, n = std::min(every, h->T - s0);
+    float* es = nullptr;
+    if (k == nck - 1) {
+      for (int c = 0; c < 3; ++c)
+        CEEP_HIP(ceep::launch_copy(h->f[0][c], h->estore + (size_t)c * nbox, nbox, st));
+      es = h->estore;
+    }
+    if ((rc = launch_range(h, s0 & 1, s0, n, es))) return rc;
+  }
+  h->parity = h->T & 1;
+  CEEP_HIP(hipStreamSynchronize(st));
+  h->has_run = true;
+  h->ck_valid = true;
+  h->estore_seg = nck - 1;
+  if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
+  return CEEP_OK;
+}
+
+// sum over the poles of k2 / 2 (dispersive.py:253-299, grid.py:178-184), the
+// part of eps_eff that does not depend on eps_r
+double eps_add_at(const ceep_fdtd* h, long long g) {
+  const double dt = h->dt;
+  double add = 0.0;
+  if (h->has_debye && h->deb_tau[g] > 0)
+    add += ((2.0 * kEps0 * h->deb_de[g] * dt) / (2.0 * h->deb_tau[g] + dt)) / 2.0;
+  if (h->has_lorentz && h->lor_w0[g] > 0) {
+    const double d = h->lor_dl[g], w02 = h->lor_w0[g] * h->lor_w0[g], dt2 = dt * dt;
+    const double denom = 4.0 + 2.0 * d * dt + w02 * dt2;
+    add += (2.0 * kEps0 * h->lor_de[g] * w02 * dt2 / denom) / 2.0;
+  }
+  return add;
+}
+
+// dL/d eps_r and dL/d sigma at one cell from the sensitivities to ca and to
+// cb (gcb_over_cb = sum of lam * cb * (curl/dx - sum gJ), i.e. dL/dcb * cb),
+// through eps = eps_r eps0 + eps_add, s = sigma dt / (2 eps),
+// ca = (1 - s)/(1 + s), cb = (dt/eps)/(1 + s)   (fdtd_3d_batched.py:282-290)
+void chain_rule(double eps_r, double eps_add, double sigma, double dt, double dL_dca,
+                double gcb_over_cb, double* ge, double* gs) {
+  const double eps = eps_r * kEps0 + eps_add;
+  const double sv = sigma * dt / (2.0 * eps);
+  const double cb = (dt / eps) / (1.0 + sv);
+  const double dL_dcb = gcb_over_cb / cb;
+  const double dca_ds = -2.0 / ((1.0 + sv) * (1.0 + sv));
+  const double ds_deps = -sv / eps, ds_dsig = dt / (2.0 * eps);
+  const double dcb_deps = -(dt / (eps * eps)) / (1.0 + sv) +
+                          (dt / eps) * (-1.0 / ((1.0 + sv) * (1.0 + sv))) * ds_deps;
+  const double dcb_dsig = (dt / eps) * (-1.0 / ((1.0 + sv) * (1.0 + sv))) * ds_dsig;
+  *ge = kEps0 * (dL_dca * dca_ds * ds_deps + dL_dcb * dcb_deps);
+  *gs = dL_dca * dca_ds * ds_dsig + dL_dcb * dcb_dsig;
+}
+
+int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* grad_sigma) {
+  int rc;
+  const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
+  const long long nbox = h->box_floats();
+  const size_t ntr = (size_t)B * P * T;
+  double mx = 0.0;
+  for (size_t i = 0; i < ntr; ++i) mx = std::max(mx, std::fabs(dtraces[i]));
+  std::memset(grad_eps_r, 0, (size_t)h->full_cells * sizeof(double));
+  std::memset(grad_sigma, 0, (size_t)h->full_cells * sizeof(double));
+  if (mx == 0.0 || P == 0) return CEEP_OK;
+  // dL/dtrace is linear in the adjoint: a power-of-two scale keeps the FP32
+  // adjoint fields far from the subnormal range
+  const double scale = std::ldexp(1.0, -std::ilogb(mx));
+  std::vector<float> dtr(ntr);
+  for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
+  const int bb = h->bb >= 4 ? 4 : (h->bb >= 2 ? 2 : 1);
+  const int nchunks = (B + bb - 1) / bb;
+  const bool use[6] = {true, true, true, h->has_debye, h->has_lorentz, h->has_lorentz};
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 3; ++c)
+      if (use[r] && !h->adj3[r][c] && (rc = dalloc(&h->adj3[r][c], (size_t)nbox))) return rc;
+  if (!h->g_ca && (rc = dalloc(&h->g_ca, (size_t)nchunks * h->plane))) return rc;
+  if (!h->g_cbi && (rc = dalloc(&h->g_cbi, (size_t)nchunks * h->plane))) return rc;
+  if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
+  hipStream_t st = h->stream;
+  CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr.data(), ntr * sizeof(float), hipMemcpyHostToDevice, st));
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 3; ++c)
+      if (use[r] && r != 2) CEEP_HIP(ceep::launch_zero(h->adj3[r][c], (size_t)nbox * sizeof(float), st));
+  CEEP_HIP(ceep::launch_zero(h->g_ca, (size_t)nchunks * h->plane * sizeof(float), st));
+  CEEP_HIP(ceep::launch_zero(h->g_cbi, (size_t)nchunks * h->plane * sizeof(float), st));
+  // lam_z^T = dL/dtrace(T-1) at the probes
+  CEEP_HIP(ceep::launch_adj3d_seed(h->adj3[0][2], h->probe_cell3, h->d_dtr, B, P, T, T - 1,
+                                   h->plane, st));
+  ceep::Adj3DArgs a{};
+  a.Mx = h->M[0]; a.My = h->M[1]; a.Mz = h->M[2]; a.pitch = h->pitch; a.B = B;
+  a.sx = h->row_block; a.bstride = h->plane;
+  for (int c = 0; c < 3; ++c) {
+    a.lam[c] = h->adj3[0][c]; a.hbar[c] = h->adj3[1][c]; a.q[c] = h->adj3[2][c];
+    a.ad[c] = h->adj3[3][c]; a.al[c] = h->adj3[4][c]; a.bl[c] = h->adj3[5][c];
+  }
+  a.ca = h->ca; a.cb = h->cb; a.da = h->da; a.dbi = h->dbi;
+  a.d_k1 = h->dk1; a.d_k2dt = h->dk2; a.l_k1 = h->lk1; a.l_k2dt = h->lk2; a.l_k3 = h->lk3;
+  a.ch = h->ch; a.idx = h->idx;
+  a.e_cstride = nbox;
+  a.src_cell = h->src3; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+  a.T = T; a.P = P; a.dtr = h->d_dtr;
+  a.probe_head = h->probe_head; a.probe_next = h->probe_next;
+  a.g_ca = h->g_ca; a.g_cbi = h->g_cbi; a.gstride = h->plane;
+  std::vector<float*> live = state3d(h);
+  const size_t nstate = live.size();
+  const size_t slot = (size_t)3 * nbox;
+  for (int k = h->ck_count - 1; k >= 0; --k) {
+    const int s0 = k * every, n = std::min(every, T - s0);
+    if (k != h->estore_seg) {   // recompute the segment from its checkpoint, storing E
+      for (size_t i = 0; i < nstate; ++i)
+        CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + i], live[i], nbox, st));
+      for (int c = 0; c < 3; ++c)
+        CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + c], h->estore + (size_t)c * nbox, nbox, st));
+      if ((rc = launch_range(h, s0 & 1, s0, n, h->estore))) return rc;
+      h->estore_seg = k;
+    }
+    for (int s = s0 + n - 1; s >= s0; --s) {
+      a.e_s = h->estore + (size_t)(s - s0) * slot;
+      a.e_s1 = h->estore + (size_t)(s + 1 - s0) * slot;
+      a.step = s;
+      CEEP_HIP(ceep::launch_adj3d(a, h->poles(), bb, st));
+    }
+  }
+  if (nchunks > 1) {   // fold the chunk partials into plane 0 (in place: plane 0 is read first)
+    CEEP_HIP(ceep::launch_sum_parts(h->g_ca, nchunks, h->plane, h->plane, h->g_ca, st));
+    CEEP_HIP(ceep::launch_sum_parts(h->g_cbi, nchunks, h->plane, h->plane, h->g_cbi, st));
+  }
+  std::vector<float> gca(h->plane), gcb(h->plane);
+  CEEP_HIP(hipMemcpyAsync(gca.data(), h->g_ca, gca.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipMemcpyAsync(gcb.data(), h->g_cbi, gcb.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  if ((rc = zero_state(h))) return rc;   // the recomputation overwrote the fields
+  CEEP_HIP(hipStreamSynchronize(st));
+  for (int x = 1; x < h->M[0] - 1; ++x)
+    for (int y = 1; y < h->M[1] - 1; ++y)
+      for (int z = 1; z < h->M[2] - 1; ++z) {
+        const long long o = h->box_index(x, y, z);
+        const long long g = h->full_index(x + h->org, y + h->org, z + h->org);
+        chain_rule(h->eps_r[g], eps_add_at(h, g), h->sigma_e[g], h->dt, gca[o] / scale,
+                   gcb[o] / scale, &grad_eps_r[g], &grad_sigma[g]);
+      }
+  return CEEP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -940,9 +1142,10 @@ extern "C" {
 int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   int rc = ready_check(h);
   if (rc) return rc;
-  if (h->ndim != 2 || h->has_debye || h->has_lorentz)
-    return fail(CEEP_EUNSUPPORTED, "the adjoint covers 2D non-dispersive media");
   if (every < 1) return fail(CEEP_EINVAL, "checkpoint interval must be >= 1");
+  if (h->ndim == 3) return run_checkpointed3d(h, every, traces_out);
+  if (h->has_debye || h->has_lorentz)
+    return fail(CEEP_EUNSUPPORTED, "the 2D adjoint covers non-dispersive media (3D covers Debye/Lorentz)");
   const int nck = (h->T + every - 1) / every;
   h->ck_valid = false;
   // checkpoint buffers (FP32, or FP64 for the FP64 forward) are kept across calls
@@ -990,6 +1193,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
                       double* grad_sigma) {
   if (!h || !dtraces || !grad_eps_r || !grad_sigma) return fail(CEEP_EINVAL, "null argument");
   if (!h->ck_valid) return fail(CEEP_ESTATE, "no checkpointed forward run (ceep_fdtd_run_checkpointed)");
+  if (h->ndim == 3) return adjoint3d(h, dtraces, grad_eps_r, grad_sigma);
   int rc;
   const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
   const size_t nbox = (size_t)h->box_floats(), bytes = nbox * sizeof(float);

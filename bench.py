@@ -102,6 +102,26 @@ def _cpu_baseline(ph, pos, w, dt, steps):
     }
 
 
+def _cpu_openmp(ph, pos, w, dt, steps=100):
+    """The oracle's C restatement (FP64, OpenMP over every host core) on the
+    first `steps` steps of the same workload: a stronger CPU reference point
+    than the reference's own single-core NumPy path."""
+    from oracle import ade, c_ref
+
+    m = ph.get_debye_maps()
+    poles = [ade.debye_pole(m["delta_eps"], m["tau"], dt)]
+    c_ref.run_cavity2d(NX, NY, DX, dt, N_CPML, pos, pos, w[:5], m["eps_inf"], m["sigma"],
+                       poles=poles)
+    t0 = time.perf_counter()
+    c_ref.run_cavity2d(NX, NY, DX, dt, N_CPML, pos, pos, w[:steps], m["eps_inf"], m["sigma"],
+                       poles=poles)
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round((NX - 2 * N_CPML + 1) ** 2 * B * steps / el / 1e9, 4),
+            "unit": "GCell-steps/s (stepped box)", "cores": threads, "kind": "port",
+            "sample": f"first {steps} of {T} steps, FP64 C restatement, {threads} OpenMP threads, {el:.2f} s"}
+
+
 def _adjoint_ratio(every=50):
     """fwd+adjoint / fwd on the gradient workload: 600x600, 16 TX/RX ring, 800
     steps, seeded lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells),
@@ -194,6 +214,7 @@ def _bench_3d(steps_timed=2, T3=1000):
     cells = int(box[0] * box[1] * box[2])
     h.time_runs(1)
     ms = h.time_runs(steps_timed)
+    s.close()
     return {"value": round(cells * 16 * T3 / (ms * 1e-3) / 1e9, 3), "unit": "GCell-steps/s",
             "ms_per_run": round(ms, 2), "cells_counted": f"stepped box {box[0]}x{box[1]}x{box[2]}",
             "value_full_grid_convention": round(n3 ** 3 * 16 * T3 / (ms * 1e-3) / 1e9, 3),
@@ -230,7 +251,7 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
         s._build()
         solvers.append(s)
         grids.append(g)
-    shape, log = (4, 16, 2), []
+    shape, log = (4, 16, 4), []       # the tuner's usual pick on MI355X
     if autotune:
         shape, log = autotune_uneven(solvers)
     ub = UnevenBatch(solvers, *shape)
@@ -281,6 +302,73 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
                          "steps U[600,1200], 16 TX/RX, CPML 12-15, 301-frequency DFT each; "
                          "one hipGraph of per-round launches over all problems vs the problems "
                          "run one after another with the one-step kernel")}
+
+
+def _bench_3d_gradient(T3=1000, every=100):
+    """dL/d eps_r and dL/d sigma over the whole 200^3 volume of the 3D
+    workload (Debye + Lorentz head seed 0, 16 TX, 1000 steps), loss = sum
+    |S - S_target|^2 over the 101-frequency S-matrix against head seed 1,
+    checkpoints every 100 steps with recomputation.  Ratio = gradient wall
+    time over the forward's (reset, run, DFT)."""
+    import ceep_amd
+    from ceep_amd import adjoint
+    from ceep_amd.constants import cfl_timestep_3d
+    from ceep_amd.waveforms import modulated_gaussian
+
+    n3, dx3 = 200, 1e-3
+    dt3 = cfl_timestep_3d(dx3, dx3, dx3, 0.5) * 0.99
+    ring = ceep_amd.phantoms.ring_positions((100, 100), 70, 16)
+    pos = [(x, y, 100) for x, y in ring]
+    w = modulated_gaussian(T3, dt3, 2e9, 1.5e9)
+    freqs = np.linspace(0.5e9, 3e9, 101)
+
+    def solver(seed):
+        s = ceep_amd.BatchedFDTD3D(nx=n3, ny=n3, nz=n3, dx=dx3, total_steps=T3,
+                                   cpml_thickness=10, source_positions=pos, probe_positions=pos,
+                                   frequency=2e9, waveform=w)
+        s.set_phantom(ceep_amd.phantoms.DebyeHeadPhantom3D((n3, n3, n3), seed))
+        return s
+
+    tgt = solver(1)
+    tgt.run()
+    S_t = tgt.s_parameters(freqs).s_matrix
+    tgt.close()
+    s = solver(0)
+    s._build()
+    h = s._handle
+
+    def forward():
+        h.reset()
+        h.run(copy=False)
+        h.dft(freqs)
+
+    forward()
+    _device_sync()
+    t0 = time.perf_counter()
+    forward()
+    _device_sync()
+    t_fwd = time.perf_counter() - t0
+    eps, sig = s._eps_r.copy(), s._sigma_e.copy()
+    try:
+        adjoint.gradient(s, eps, sig, freqs, S_t, every=every)       # allocates the store
+        _device_sync()
+        t0 = time.perf_counter()
+        adjoint.gradient(s, eps, sig, freqs, S_t, every=every)
+        _device_sync()
+        t_grad = time.perf_counter() - t0
+    except Exception as exc:   # e.g. the E store does not fit
+        s.close()
+        return {"error": f"{type(exc).__name__}: {exc}"}
+    box = h.box_info()
+    cells = int(box[0] * box[1] * box[2])
+    s.close()
+    gb = (every + 1) * 3 * cells * 16 * 4 / 1e9
+    return {"ratio": round(t_grad / t_fwd, 3), "ms_forward": round(t_fwd * 1e3, 1),
+            "ms_forward_plus_adjoint": round(t_grad * 1e3, 1), "checkpoint_every": every,
+            "e_store_gb": round(gb, 1),
+            "workload": ("3D 200^3 Debye+Lorentz head seed 0, 16 TX/RX, 1000 steps; "
+                         "L = sum |S - S_target|^2 over 101 freqs 0.5-3 GHz vs head seed 1; "
+                         "dL/deps_r and dL/dsigma over the full volume")}
 
 
 def _traffic():
@@ -378,15 +466,19 @@ def main():
     b3 = None if args.no_3d else _bench_3d()
     if b3:
         _progress(f"3D: {b3['value']} GCell-steps/s")
+    g3 = None if (args.no_3d or args.no_adjoint) else _bench_3d_gradient()
+    if g3:
+        _progress(f"3D gradient: {g3}")
     bu = None if args.no_uneven else _bench_uneven(autotune=not args.no_autotune)
     if bu:
         _progress(f"uneven batch: {bu['value']} GCell-steps/s")
     if not args.no_cpu_baseline:
         _progress("timing the CPU baseline")
 
-    cpu = None
+    cpu = cpu_omp = None
     if not args.no_cpu_baseline:
         cpu = _cpu_baseline(ph, pos, w, dt, args.cpu_steps)
+        cpu_omp = _cpu_openmp(ph, pos, w, dt)
 
     line = {
         "metric": "GCell-steps/s (600²×16TX, 3D 200³); %HBM peak; fwd+adjoint / fwd time ratio",
@@ -424,8 +516,10 @@ def main():
         "fwd_adjoint_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
         "bench_3d": b3,
+        "adjoint_3d": g3,
         "bench_uneven": bu,
         "cpu_baseline": cpu,
+        "cpu_openmp_reference_point": cpu_omp,
         "s_matrix_shape": list(S.shape),
     }
     print(json.dumps(line), flush=True)

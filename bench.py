@@ -176,15 +176,22 @@ def _adjoint_ratio(every=50):
         return (time.perf_counter() - t0) / reps
 
     t_grad = timed(every)
-    t_whole = timed(T)
     t_grad64 = timed(every, True)
+    s._handle.set_tuning("adjoint_reversible", 0)
+    t_store = timed(every)
+    t_whole = timed(T)
+    s._handle.set_tuning("adjoint_reversible", 1)
     return {"ratio": round(t_grad / t_fwd, 3), "ms_forward": round(t_fwd * 1e3, 3),
             "ms_forward_plus_adjoint": round(t_grad * 1e3, 3), "checkpoint_every": every,
             "ratio_single_segment": round(t_whole / t_fwd, 3),
+            "ratio_recompute_and_e_store": round(t_store / t_fwd, 3),
             "ratio_fp64_forward": round(t_grad64 / t_fwd, 3),
-            "note": ("ratio: checkpoints every 50 steps, each earlier segment recomputed; "
-                     "ratio_single_segment: E of all 800 steps kept from the forward pass "
-                     "(no recomputation, 16.7 GB of HBM); ratio_fp64_forward: forward and "
+            "note": ("ratio: checkpoints every 50 steps; the adjoint launches rebuild the "
+                     "forward state backward from each segment's end checkpoint (reversible, "
+                     "no recomputation pass); ratio_recompute_and_e_store: each segment "
+                     "recomputed into an E store instead; ratio_single_segment: E of all 800 "
+                     "steps kept from the forward pass (16.7 GB, no recomputation); "
+                     "ratio_fp64_forward: forward and "
                      "recomputation on FP64 fields (what this misfit needs for an accurate "
                      "gradient: its residual is 3e-10 of |S|), adjoint FP32, over the FP32 forward"),
             "workload": ("2D 600x600, 16 TX/RX ring r=200, 800 steps, lossy disk phantom; "
@@ -392,6 +399,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--single-run", action="store_true",
                     help="one simulation + DFT with the default tuning, no output (for rocprofv3 --pmc)")
+    ap.add_argument("--single-run-3d", action="store_true",
+                    help="one 3D simulation (200^3, Debye + Lorentz, 100 steps), no output (for rocprofv3 --pmc)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of 256 MiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -401,6 +410,9 @@ def main():
         for _ in range(3):
             y.copy_(x)
         torch.cuda.synchronize()
+        return
+    if args.single_run_3d:
+        _bench_3d(steps_timed=1, T3=100)
         return
     if args.single_run:
         s, *_ = _workload()

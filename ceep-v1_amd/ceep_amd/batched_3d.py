@@ -54,6 +54,7 @@ class BatchedFDTD3D:
         self._sigma_m = np.zeros(shape, dtype=np.float64)
         self._debye = None
         self._lorentz = None
+        self.tuning = None       # launch settings (see autotune.tuning_for); None = defaults
         self._built = False
         self._handle = None
         self.xp = np
@@ -149,6 +150,8 @@ class BatchedFDTD3D:
                         self._mu_r if magnetic else None, self._sigma_m if magnetic else None)
         h.set_sources(src, self.waveform)
         h.set_probes(prb)
+        if self.tuning is not None:
+            h.apply_tuning(self.tuning)
         h.reset()
         self._handle = h
         self.inv_dx = 1.0 / self.dx

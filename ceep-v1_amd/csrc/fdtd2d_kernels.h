@@ -172,9 +172,22 @@ struct AdjTB2DArgs {
   float* g_cbi;
   long long gstride;
   int tiles_x, tiles_y, bpw, ntasks;
+  // reversible form: the forward state at step_hi (ez, hx, hy) in, and at
+  // step_hi - ksteps out; e_store is unused
+  const float* fe_in;
+  const float* fhx_in;
+  const float* fhy_in;
+  float* fe_out;
+  float* fhx_out;
+  float* fhy_out;
 };
 
 hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
+// The reversible blocked adjoint: the forward state is stepped backward
+// (E^s = (E^{s+1} - source - cb/dx curl H^{s+1/2}) / ca, then
+// H^{s-1/2} = H^{s+1/2} -+ ch dE^s) inside the same windows as the adjoint,
+// so no segment is recomputed and no E store is read.
+hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
 
 // One problem of an uneven batch: its blocked-kernel arguments (buffers set
 // per round from the ping-pong pairs) and the number of K-step rounds.

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
   const bool col_ok = c >= 0 && c < p.My;
   const bool out_lane = lane >= K && lane < 64 - K && col_ok;
 
-  float mca[R], mcbi[R];
+  float mca[R], mica[R], mcbi[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int a = a0 + r;
@@ -215,6 +215,7 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
     const long long cell = (long long)a * p.pitch + c;
     mca[r] = in ? p.ca[cell] : 0.f;
     mcbi[r] = in ? p.cb[cell] * p.idx : 0.f;
+    mica[r] = mca[r] != 0.f ? 1.f / mca[r] : 0.f;
   }
   float gca[NO], gcb[NO];
 #pragma unroll
@@ -354,6 +355,196 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
   }
   // this wavefront is the only writer of its output cells in its chunk's
   // partial-sum plane during this launch: plain read-modify-write
+  float* gca_p = p.g_ca + (long long)chunk * p.gstride;
+  float* gcb_p = p.g_cbi + (long long)chunk * p.gstride;
+#pragma unroll
+  for (int r = 0; r < NO; ++r) {
+    const int a = a0 + K + r;
+    if (out_lane && a < p.Mx && mcbi[K + r] != 0.f) {
+      const long long cell = (long long)a * p.pitch + c;
+      gca_p[cell] += gca[r];
+      gcb_p[cell] += gcb[r];
+    }
+  }
+}
+
+// Reversible form of adj2d_tb_kernel.  Each reversed step first rebuilds
+// the forward state one step back from the window's own registers
+//   E^s = (E^{s+1} - source(s) - cb/dx curl H^{s+1/2}) / ca      (E interior; 0 elsewhere)
+//   hx^{s-1/2} = hx^{s+1/2} + ch (E^s(c+1) - E^s(c)),  hy^{s-1/2} = hy^{s+1/2} - ch (E^s(a+1) - E^s(a))
+// (the forward H/E steps of step2d_tb_kernel solved for their inputs), takes
+// the sensitivities g_ca += lam E^s and g_cbi += lam cb/dx curl H^{s+1/2} --
+// the latter without the cancellation of E^{s+1} - ca E^s - source -- and
+// then applies the same adjoint step as adj2d_tb_kernel.  The forward
+// reconstruction shrinks the valid window like the forward step (left/top
+// for E, right/bottom for H), so the exact inner window is the same.  Every
+// segment restarts from an exact checkpoint, which bounds the rounding drift
+// of the reconstruction to one segment (the (1/ca)^n growth in lossy cells).
+template <int R, int K>
+__global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
+  constexpr int NO = R - 2 * K;
+  const int lane = threadIdx.x & 63;
+  const int task = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (task >= p.ntasks) return;
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = task % tiles;
+  const int chunk = task / tiles;
+  const int ty = tile % p.tiles_y;
+  const int tx = tile / p.tiles_y;
+  const int a0 = tx * NO - K;
+  const int cb0 = ty * (64 - 2 * K) - K;
+  const int c = cb0 + lane;
+  const bool col_ok = c >= 0 && c < p.My;
+  const bool out_lane = lane >= K && lane < 64 - K && col_ok;
+
+  float mca[R], mcbi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int a = a0 + r;
+    const bool in = col_ok && a >= 0 && a < p.Mx;
+    const long long cell = (long long)a * p.pitch + c;
+    mca[r] = in ? p.ca[cell] : 0.f;
+    mcbi[r] = in ? p.cb[cell] * p.idx : 0.f;
+  }
+  float gca[NO], gcb[NO];
+#pragma unroll
+  for (int r = 0; r < NO; ++r) { gca[r] = 0.f; gcb[r] = 0.f; }
+  const float ch = p.ch, idx = p.idx;
+  const bool few_probes = p.P <= 64;
+  int pr_h = -1, pl_h = -1;
+  unsigned long long pmask = 0;
+  if (few_probes) {
+    const int pc = lane < p.P ? p.probe_cell[lane] : -1;
+    if (pc >= 0) {
+      pr_h = pc / p.pitch - a0;
+      pl_h = pc % p.pitch - cb0;
+    }
+    pmask = __ballot(pc >= 0 && pr_h >= 0 && pr_h < R && pl_h >= 0 && pl_h < 64);
+  }
+
+  const int b_begin = chunk * p.bpw;
+  const int b_end = min(b_begin + p.bpw, p.B);
+  for (int bt = b_begin; bt < b_end; ++bt) {
+    const long long base = (long long)bt * p.bstride;
+    float le[R], mx[R], my[R], fe[R], fx[R], fy[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int a = a0 + r;
+      const bool in = col_ok && a >= 0 && a < p.Mx;
+      const long long o = base + (long long)a * p.pitch + c;
+      le[r] = in ? p.le_in[o] : 0.f;
+      mx[r] = in ? p.mx_in[o] : 0.f;
+      my[r] = in ? p.my_in[o] : 0.f;
+      fe[r] = in ? p.fe_in[o] : 0.f;
+      fx[r] = in ? p.fhx_in[o] : 0.f;
+      fy[r] = in ? p.fhy_in[o] : 0.f;
+    }
+    const int sc = p.src_cell[bt];
+    const int s_row = sc / p.pitch - a0;
+    const int s_lane = sc % p.pitch - cb0;
+    float wsrc[K], gq[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int step = p.step_hi - 1 - k;
+      const bool live = k < p.ksteps;
+      wsrc[k] = live ? p.wave[(long long)bt * p.wave_stride_b + step] : 0.f;
+      gq[k] = (live && step >= 1 && few_probes && ((pmask >> lane) & 1ull))
+                  ? p.dtr[((long long)bt * p.P + lane) * p.T + (step - 1)] : 0.f;
+    }
+
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (k >= p.ksteps) break;
+      const int step = p.step_hi - 1 - k;
+      // forward E one step back (in place: row r's E is only read by row r),
+      // and the sensitivities (lam here is lam^{step+1})
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float hxl = from_prev_lane(fx[r]);
+        const float hyu = (r > 0) ? fy[r - 1] : 0.f;
+        const float curl = (fy[r] - hyu) - (fx[r] - hxl);
+        const float dterm = mcbi[r] * curl;
+        const float inj = (r == s_row && lane == s_lane) ? wsrc[k] : 0.f;
+        const float es = ((fe[r] - inj) - dterm) * mica[r];
+        if (r >= K && r < R - K) {
+          gca[r - K] += le[r] * es;
+          gcb[r - K] += le[r] * dterm;
+        }
+        fe[r] = es;
+      }
+      // forward H one step back
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float eR = from_next_lane(fe[r]);
+        const float eD = (r + 1 < R) ? fe[r + 1] : 0.f;
+        fx[r] = fx[r] + ch * (eR - fe[r]);
+        fy[r] = fy[r] - ch * (eD - fe[r]);
+      }
+      // adjoint H half step and E step (as adj2d_tb_kernel)
+      float q[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) q[r] = mcbi[r] * le[r];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float qR = from_next_lane(q[r]);
+        const float qD = (r + 1 < R) ? q[r + 1] : 0.f;
+        mx[r] = mx[r] + (qR - q[r]);
+        my[r] = my[r] + (q[r] - qD);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float mxL = from_prev_lane(mx[r]);
+        const float myU = (r > 0) ? my[r - 1] : 0.f;
+        le[r] = mca[r] * le[r] + ch * ((myU - my[r]) - (mxL - mx[r]));
+      }
+      for (unsigned long long m = pmask; m;) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int qr = __shfl(pr_h, l);
+        const int ql = __shfl(pl_h, l);
+        const float g = __shfl(gq[k], l);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r == qr && lane == ql) le[r] += g;
+      }
+      if (!few_probes && step >= 1) {
+        for (int q0 = 0; q0 < p.P; q0 += 64) {
+          const int qi = q0 + lane;
+          const int pc = qi < p.P ? p.probe_cell[qi] : -1;
+          int pr = -1, pl = -1;
+          if (pc >= 0) {
+            pr = pc / p.pitch - a0;
+            pl = pc % p.pitch - cb0;
+          }
+          const bool here = pc >= 0 && pr >= 0 && pr < R && pl >= 0 && pl < 64;
+          unsigned long long m = __ballot(here);
+          while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const int qr = __shfl(pr, l);
+            const int ql = __shfl(pl, l);
+            const float g = p.dtr[((long long)bt * p.P + q0 + l) * p.T + (step - 1)];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (r == qr && lane == ql) le[r] += g;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = K; r < R - K; ++r) {
+      const int a = a0 + r;
+      if (out_lane && a < p.Mx) {
+        const long long o = base + (long long)a * p.pitch + c;
+        p.le_out[o] = le[r];
+        p.mx_out[o] = mx[r];
+        p.my_out[o] = my[r];
+        p.fe_out[o] = fe[r];
+        p.fhx_out[o] = fx[r];
+        p.fhy_out[o] = fy[r];
+      }
+    }
+  }
   float* gca_p = p.g_ca + (long long)chunk * p.gstride;
   float* gcb_p = p.g_cbi + (long long)chunk * p.gstride;
 #pragma unroll
@@ -581,6 +772,28 @@ static hipError_t launch_adj_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int R>
+static hipError_t launch_rev_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((a.ntasks + 3) / 4, 1, 1);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((adj2d_rev_tb_kernel<R, 1>), grd, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((adj2d_rev_tb_kernel<R, 2>), grd, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((adj2d_rev_tb_kernel<R, 3>), grd, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((adj2d_rev_tb_kernel<R, 4>), grd, blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
+  switch (R) {
+    case 16: return launch_rev_r<16>(a, K, s);
+    case 24: return launch_rev_r<24>(a, K, s);
+    case 32: return launch_rev_r<32>(a, K, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {

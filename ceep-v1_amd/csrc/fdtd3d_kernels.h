@@ -84,6 +84,21 @@ hipError_t launch_adj3d(const Adj3DArgs& a, int poles, int bb, hipStream_t s);
 hipError_t launch_adj3d_seed(float* lam_z, const long long* probe_cell, const float* dtr,
                              int B, int P, int T, int step, long long bstride, hipStream_t s);
 
+// One fused 3D step (H and E+ADE, inject, probes) from the `*_in` fields to
+// the `*_out` fields (ping-pong; ADE currents as in Step3DArgs).  Each E cell
+// recomputes the six neighbouring H values it needs from the input fields,
+// so H and E are streamed once per step instead of twice.
+struct Step3DFArgs {
+  Step3DArgs a;                   // geometry, maps, ADE, sources, probes, e_save
+  const float* e_in[3];
+  const float* h_in[3];
+  float* e_out[3];
+  float* h_out[3];
+  int xcd_order;                  // 1: renumber workgroups so each XCD walks contiguous x-slabs
+};
+
+hipError_t launch_step3d_fused(const Step3DFArgs& a, int poles, int bb, hipStream_t s);
+
 hipError_t launch_h3d(const Step3DArgs& a, int bb, hipStream_t s);
 hipError_t launch_e3d(const Step3DArgs& a, int poles, int bb, hipStream_t s);
 

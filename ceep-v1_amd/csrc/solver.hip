@@ -132,12 +132,20 @@ struct ceep_fdtd {
   int tb_k = 0;                // 2D temporal depth (0: one step per launch)
   int tb_r = 16;               // 2D rows per wavefront window
   int tb_bpw = 1;              // 2D batch planes per wavefront
+  int fused3d = 0;             // 3D: 1/2 = one fused H+E launch per step (ping-pong fields)
+  int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
+  bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
 
   long long box_floats() const { return (long long)B * plane; }
+  // field buffer holding the current state (3D fields stay in buffer 0 with
+  // the two-kernel in-place update)
+  int fbuf() const { return (ndim == 2 || fused3d) ? parity : 0; }
   int poles() const { return (has_debye ? 1 : 0) | (has_lorentz ? 2 : 0); }
   // the temporally blocked 2D kernel covers the plain and Debye cases
   bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz; }
+  // 2D gradient path: reversible reconstruction instead of recompute + E store
+  bool use_rev() const { return adj_rev && use_tb() && !fwd64; }
   int launches_per_run() const { return use_tb() ? (T + tb_k - 1) / tb_k : T; }
   long long full_index(int64_t x, int64_t y, int64_t z) const {
     return ndim == 2 ? x * N[1] + y : (x * N[1] + y) * N[2] + z;
@@ -300,7 +308,7 @@ int zero_state(ceep_fdtd* h) {
   const int nf = h->ndim == 2 ? 3 : 6;
   for (int k = 0; k < nf; ++k) {
     CEEP_HIP(hipMemsetAsync(h->f[0][k], 0, bytes, s));
-    if (h->ndim == 2) CEEP_HIP(hipMemsetAsync(h->f[1][k], 0, bytes, s));
+    CEEP_HIP(hipMemsetAsync(h->f[1][k], 0, bytes, s));
   }
   for (int k = 0; k < 3; ++k) {
     if (h->jd[k]) CEEP_HIP(hipMemsetAsync(h->jd[k], 0, bytes, s));
@@ -392,6 +400,17 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
       a.step = st;
       a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * 3 * h->box_floats() : nullptr;
+      if (h->fused3d) {
+        ceep::Step3DFArgs fa{};
+        fa.a = a;
+        fa.xcd_order = h->fused3d == 1;
+        for (int k = 0; k < 3; ++k) {
+          fa.e_in[k] = h->f[i][k]; fa.h_in[k] = h->f[i][3 + k];
+          fa.e_out[k] = h->f[o][k]; fa.h_out[k] = h->f[o][3 + k];
+        }
+        CEEP_HIP(ceep::launch_step3d_fused(fa, poles, bb, s));
+        continue;
+      }
       CEEP_HIP(ceep::launch_h3d(a, bb, s));
       CEEP_HIP(ceep::launch_e3d(a, poles, bb, s));
     }
@@ -583,7 +602,7 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
   const int nf = ndim == 2 ? 3 : 6;
   for (int k = 0; k < nf && !rc; ++k) {
     rc = dalloc(&h->f[0][k], h->box_floats());
-    if (!rc && ndim == 2) rc = dalloc(&h->f[1][k], h->box_floats());
+    if (!rc) rc = dalloc(&h->f[1][k], h->box_floats());
   }
   if (!rc) rc = dalloc(&h->ca, h->plane);
   if (!rc) rc = dalloc(&h->cb, h->plane);
@@ -840,7 +859,7 @@ int ceep_fdtd_get_field(ceep_fdtd* h, int comp, double* out) {
   if (comp < 0 || comp >= nf) return fail(CEEP_EINVAL, "bad field component");
   const size_t total = (size_t)h->B * h->full_cells;
   std::memset(out, 0, total * sizeof(double));
-  const int buf = h->ndim == 2 ? h->parity : 0;
+  const int buf = h->fbuf();
   std::vector<float> box(h->box_floats());
   CEEP_HIP(hipMemcpy(box.data(), h->f[buf][comp], box.size() * sizeof(float), hipMemcpyDeviceToHost));
   const int mz = h->ndim == 3 ? h->M[2] : 1;
@@ -857,7 +876,7 @@ int ceep_fdtd_get_field(ceep_fdtd* h, int comp, double* out) {
 int ceep_fdtd_energy(ceep_fdtd* h, double* out) {
   if (!h || !out) return fail(CEEP_EINVAL, "null argument");
   *out = 0.0;
-  const int buf = h->ndim == 2 ? h->parity : 0;
+  const int buf = h->fbuf();
   const int ne = h->ndim == 2 ? 1 : 3;
   const int nf = h->ndim == 2 ? 3 : 6;
   double se = 0.0, sh = 0.0;
@@ -954,9 +973,10 @@ int run_checkpointed64(ceep_fdtd* h, int nck, double* traces_out) {
 
 // The live 3D state a checkpoint holds: six fields, then the ADE currents
 // (Debye J per component, both Lorentz buffers per component).
-std::vector<float*> state3d(ceep_fdtd* h) {
+std::vector<float*> state3d(ceep_fdtd* h, int par) {
   std::vector<float*> v;
-  for (int k = 0; k < 6; ++k) v.push_back(h->f[0][k]);
+  const int fb = h->fused3d ? par : 0;
+  for (int k = 0; k < 6; ++k) v.push_back(h->f[fb][k]);
   for (int k = 0; k < 3; ++k) {
     if (h->has_debye) v.push_back(h->jd[k]);
     if (h->has_lorentz) { v.push_back(h->jl[0][k]); v.push_back(h->jl[1][k]); }
@@ -971,8 +991,7 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
   const int nck = (h->T + every - 1) / every;
   const long long nbox = h->box_floats();
   hipStream_t st = h->stream;
-  std::vector<float*> live = state3d(h);
-  const size_t nstate = live.size();
+  const size_t nstate = state3d(h, 0).size();
   h->ck_valid = false;
   if (h->ck.size() != (size_t)nck * nstate) {
     for (auto& p : h->ck) dfree(p);
@@ -992,13 +1011,14 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), st));
   h->traces64_valid = false;
   for (int k = 0; k < nck; ++k) {
+    const int s0 = k * every, n = std::min(every, h->T - s0);
+    const std::vector<float*> live = state3d(h, s0 & 1);
     for (size_t i = 0; i < nstate; ++i)
       CEEP_HIP(ceep::launch_copy(live[i], h->ck[(size_t)k * nstate + i], nbox, st));
-    const int s0 = k * every, n = std::min(every, h->T - s0);
     float* es = nullptr;
     if (k == nck - 1) {
       for (int c = 0; c < 3; ++c)
-        CEEP_HIP(ceep::launch_copy(h->f[0][c], h->estore + (size_t)c * nbox, nbox, st));
+        CEEP_HIP(ceep::launch_copy(live[c], h->estore + (size_t)c * nbox, nbox, st));
       es = h->estore;
     }
     if ((rc = launch_range(h, s0 & 1, s0, n, es))) return rc;
@@ -1095,11 +1115,11 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
   a.T = T; a.P = P; a.dtr = h->d_dtr;
   a.probe_head = h->probe_head; a.probe_next = h->probe_next;
   a.g_ca = h->g_ca; a.g_cbi = h->g_cbi; a.gstride = h->plane;
-  std::vector<float*> live = state3d(h);
-  const size_t nstate = live.size();
+  const size_t nstate = state3d(h, 0).size();
   const size_t slot = (size_t)3 * nbox;
   for (int k = h->ck_count - 1; k >= 0; --k) {
     const int s0 = k * every, n = std::min(every, T - s0);
+    const std::vector<float*> live = state3d(h, s0 & 1);
     if (k != h->estore_seg) {   // recompute the segment from its checkpoint, storing E
       for (size_t i = 0; i < nstate; ++i)
         CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + i], live[i], nbox, st));
@@ -1148,10 +1168,11 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
     return fail(CEEP_EUNSUPPORTED, "the 2D adjoint covers non-dispersive media (3D covers Debye/Lorentz)");
   const int nck = (h->T + every - 1) / every;
   h->ck_valid = false;
-  // checkpoint buffers (FP32, or FP64 for the FP64 forward) are kept across calls
-  if (!h->fwd64 && h->ck.size() != (size_t)nck * 3) {
+  // checkpoint buffers (FP32: one per segment start plus the final state; or
+  // FP64 for the FP64 forward) are kept across calls
+  if (!h->fwd64 && h->ck.size() != (size_t)(nck + 1) * 3) {
     for (auto& p : h->ck) dfree(p);
-    h->ck.assign((size_t)nck * 3, nullptr);
+    h->ck.assign((size_t)(nck + 1) * 3, nullptr);
     for (auto& p : h->ck)
       if ((rc = dalloc(&p, h->box_floats()))) { h->free_adjoint(); return rc; }
   }
@@ -1160,8 +1181,10 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   h->ck_every = every;
   h->estore_seg = -1;
   // the adjoint starts with the last segment: store its E during this pass so
-  // that segment needs no recomputation
-  if (!h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * h->box_floats()))) return rc;
+  // that segment needs no recomputation (the reversible adjoint reads no store)
+  const bool rev = h->use_rev();
+  if (!rev && !h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * h->box_floats())))
+    return rc;
   if ((rc = zero_state(h))) return rc;
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), h->stream));
   if (h->fwd64) return run_checkpointed64(h, nck, traces_out);
@@ -1172,7 +1195,7 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
       CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ck[(size_t)k * 3 + f], h->box_floats(), h->stream));
     const int s0 = k * every, n = std::min(every, h->T - s0);
     float* es = nullptr;
-    if (k == nck - 1) {
+    if (k == nck - 1 && !rev) {
       CEEP_HIP(ceep::launch_copy(h->f[cur][0], h->estore, h->box_floats(), h->stream));
       es = h->estore;
     }
@@ -1180,11 +1203,14 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
     const int launches = h->use_tb() ? (n + h->tb_k - 1) / h->tb_k : n;
     cur ^= launches & 1;
   }
+  for (int f = 0; f < 3; ++f)   // the final state: where the reversible adjoint starts
+    CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ck[(size_t)nck * 3 + f], h->box_floats(), h->stream));
   h->parity = cur;
   CEEP_HIP(hipStreamSynchronize(h->stream));
   h->has_run = true;
   h->ck_valid = true;
-  h->estore_seg = nck - 1;
+  h->ck_rev = rev;
+  h->estore_seg = rev ? -1 : nck - 1;
   if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
   return CEEP_OK;
 }
@@ -1208,7 +1234,8 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   const double scale = std::ldexp(1.0, -std::ilogb(mx));
   std::vector<float> dtr(ntr);
   for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
-  if (!h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * nbox))) return rc;
+  const bool rev = h->ck_rev && h->use_rev();
+  if (!rev && !h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * nbox))) return rc;
   for (int i = 0; i < 2; ++i)
     for (int f = 0; f < 3; ++f)
       if (!h->adj[i][f] && (rc = dalloc(&h->adj[i][f], nbox))) return rc;
@@ -1233,7 +1260,38 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   a.probe_head = h->probe_head; a.probe_next = h->probe_next;
   a.g_ca = h->g_ca; a.g_cbi = h->g_cbi;
   int acur = 0;
-  for (int k = h->ck_count - 1; k >= 0; --k) {
+  for (int k = h->ck_count - 1; k >= 0 && rev; --k) {
+    // reversible: start from the exact state at the segment's end (the next
+    // checkpoint, or the final state) and step forward and adjoint back together
+    const int s0 = k * every, n = std::min(every, T - s0);
+    for (int f = 0; f < 3; ++f)
+      CEEP_HIP(ceep::launch_copy(h->ck[(size_t)(k + 1) * 3 + f], h->f[0][f], (long long)nbox, st));
+    const int K = h->tb_k, R = h->tb_r;
+    ceep::AdjTB2DArgs t{};
+    t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
+    t.ca = h->ca; t.cb = h->cb; t.ch = h->ch; t.idx = h->idx;
+    t.src_cell = h->src2; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
+    t.T = T; t.P = P; t.dtr = h->d_dtr; t.probe_cell = h->probe_cell;
+    t.g_ca = h->g_ca; t.g_cbi = h->g_cbi; t.gstride = h->plane;
+    t.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
+    t.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
+    t.bpw = h->tb_bpw;
+    t.ntasks = t.tiles_x * t.tiles_y * ((B + t.bpw - 1) / t.bpw);
+    int fcur = 0;
+    for (int hi = s0 + n; hi > s0; hi -= K) {
+      const int o = acur ^ 1, fo = fcur ^ 1;
+      t.le_in = h->adj[acur][0]; t.mx_in = h->adj[acur][1]; t.my_in = h->adj[acur][2];
+      t.le_out = h->adj[o][0]; t.mx_out = h->adj[o][1]; t.my_out = h->adj[o][2];
+      t.fe_in = h->f[fcur][0]; t.fhx_in = h->f[fcur][1]; t.fhy_in = h->f[fcur][2];
+      t.fe_out = h->f[fo][0]; t.fhx_out = h->f[fo][1]; t.fhy_out = h->f[fo][2];
+      t.step_hi = hi;
+      t.ksteps = std::min(K, hi - s0);
+      CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, st));
+      acur = o;
+      fcur = fo;
+    }
+  }
+  for (int k = h->ck_count - 1; k >= 0 && !rev; --k) {
     const int s0 = k * every, n = std::min(every, T - s0);
     // recompute the segment from its checkpoint, storing E (unless the
     // store already holds this segment)
@@ -1336,6 +1394,20 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 4) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
     h->tb_bpw = value;
+  } else if (key == 7) {
+    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "adjoint form must be 0 or 1");
+    if (value != h->adj_rev) h->ck_valid = false;
+    h->adj_rev = value;
+  } else if (key == 6) {
+    if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D step form must be 0, 1 or 2");
+    if ((value != 0) != (h->fused3d != 0)) {   // the state's buffer convention changes: start over
+      h->fused3d = value;
+      h->ck_valid = false;
+      int rc = zero_state(h);
+      if (rc) return rc;
+      CEEP_HIP(hipStreamSynchronize(h->stream));
+    }
+    h->fused3d = value;
   } else if (key == 5) {
     if (value != 0 && value != 1) return fail(CEEP_EINVAL, "forward precision must be 0 (FP32) or 1 (FP64)");
     if (value && h->ndim != 2) return fail(CEEP_EUNSUPPORTED, "the FP64 forward covers the 2D gradient path");

@@ -150,7 +150,15 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          (0 = FP32, 1 = FP64; 2D non-dispersive): with 1,
  *          ceep_fdtd_run_checkpointed steps FP64 fields and returns FP64
  *          traces, for misfits whose residual lies below FP32 trace accuracy;
- *          the adjoint stays FP32.  Not a launch-shape knob: results differ. */
+ *          the adjoint stays FP32.  Not a launch-shape knob: results differ.
+ *   key 6: 3D step form: 0 = H kernel then E kernel in place; 1 = one
+ *          fused H+E launch per step on ping-pong fields, workgroups ordered
+ *          so each XCD walks contiguous x-slabs; 2 = fused, dispatch order.
+ *          Switching between 0 and 1/2 clears the fields.
+ *   key 7: 2D FP32 blocked adjoint form: 1 = reversible (the forward state is
+ *          rebuilt backward inside the adjoint launches from the checkpoint at
+ *          each segment's end; no recomputation, no E store; default),
+ *          0 = recompute each segment into an E store. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* Geometry of the stepped box: out[0..2] = box size per axis, out[3] =

@@ -384,3 +384,44 @@ def test_3d_batch_independence_bitwise(amd):
         s.run()
         runs.append(s.probe_data)
     assert np.array_equal(runs[0][0], runs[1][0])
+
+
+@pytest.mark.parametrize("magnetic", [False, True])
+def test_3d_fused_step_matches_two_kernel_step(amd, magnetic):
+    """The fused H+E launch (default) against the H-then-E pair, both against
+    the FP64 oracle; Debye + Lorentz poles, optional mu_r / sigma_m maps."""
+    shape, n, T, dx = (34, 30, 28), 4, 90, 1e-3
+    dt = W.dt_3d_courant(dx, 0.5)
+    w = W.modulated_gaussian(T, dt, 10e9, 20e9, delay_factor=3.0)
+    ph = amd.phantoms.DebyeHeadPhantom3D((102, 90, 84), 2)
+    m = {k: v[34:68, 30:60, 28:56].copy() for k, v in ph.get_debye_maps().items()}
+    rng = np.random.default_rng(9)
+    mu = 1.0 + rng.uniform(0, 1, shape) if magnetic else None
+    sm = rng.uniform(0, 3e3, shape) if magnetic else None
+    src = [(17, 15, 14), (12, 17, 13), (20, 12, 16)]
+    prb = src + [(22, 11, 17)]
+    lor = (m["lorentz_delta_eps"], m["lorentz_omega_0"], m["lorentz_delta"])
+    out = {}
+    for fused in (1, 0):
+        s = amd.BatchedFDTD3D(nx=shape[0], ny=shape[1], nz=shape[2], dx=dx, total_steps=T,
+                              cpml_thickness=n, source_positions=src, probe_positions=prb,
+                              frequency=10e9, courant=0.5, waveform=w)
+        s.set_debye(m["eps_inf"], m["delta_eps"], m["tau"], m["sigma"])
+        s.set_lorentz(*lor)
+        if magnetic:
+            s._mu_r[:] = mu
+            s._sigma_m[:] = sm
+        s.tuning = {"fused_3d": fused}
+        s.run()
+        out[fused] = (s.probe_data.copy(), s.ez, s.hx, s.compute_energy())
+    poles = [ade.debye_pole(m["delta_eps"], m["tau"], dt), ade.lorentz_pole(*lor, dt)]
+    ref = fdtd3d.run_cavity(shape, dx, dt, n, src, prb, w, m["eps_inf"], m["sigma"],
+                            mu_r=mu, sigma_m=sm, poles=poles)
+    e1, e0 = rel_l2(out[1][0], ref), rel_l2(out[0][0], ref)
+    d = rel_l2(out[1][0], out[0][0])
+    print(f"3d fused (magnetic={magnetic}): trace rel-L2 fused {e1:.2e}, two-kernel {e0:.2e}, "
+          f"between {d:.2e}")
+    assert e1 < TRACE_TOL and e0 < TRACE_TOL and d < 1e-5
+    assert rel_l2(out[1][1], out[0][1]) < 1e-5 and rel_l2(out[1][2], out[0][2]) < 1e-5
+    assert out[1][3] == pytest.approx(out[0][3], rel=1e-5)
+

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
   const bool col_ok = c >= 0 && c < p.My;
   const bool out_lane = lane >= K && lane < 64 - K && col_ok;
 
-  float mca[R], mica[R], mcbi[R];
+  float mca[R], mcbi[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int a = a0 + r;
@@ -215,7 +215,6 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
     const long long cell = (long long)a * p.pitch + c;
     mca[r] = in ? p.ca[cell] : 0.f;
     mcbi[r] = in ? p.cb[cell] * p.idx : 0.f;
-    mica[r] = mca[r] != 0.f ? 1.f / mca[r] : 0.f;
   }
   float gca[NO], gcb[NO];
 #pragma unroll
@@ -397,7 +396,7 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
   const bool col_ok = c >= 0 && c < p.My;
   const bool out_lane = lane >= K && lane < 64 - K && col_ok;
 
-  float mca[R], mcbi[R];
+  float mca[R], mica[R], mcbi[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int a = a0 + r;
@@ -405,6 +404,7 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
     const long long cell = (long long)a * p.pitch + c;
     mca[r] = in ? p.ca[cell] : 0.f;
     mcbi[r] = in ? p.cb[cell] * p.idx : 0.f;
+    mica[r] = mca[r] != 0.f ? 1.f / mca[r] : 0.f;
   }
   float gca[NO], gcb[NO];
 #pragma unroll

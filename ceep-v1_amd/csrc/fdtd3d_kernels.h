@@ -95,9 +95,14 @@ struct Step3DFArgs {
   float* e_out[3];
   float* h_out[3];
   int xcd_order;                  // 1: renumber workgroups so each XCD walks contiguous x-slabs
+  int seg_len;                    // marching form: x-slabs per workgroup
 };
 
 hipError_t launch_step3d_fused(const Step3DFArgs& a, int poles, int bb, hipStream_t s);
+// The same step as a march along x: each thread owns a (y, z) column of
+// seg_len slabs and carries E of the current slab and the new hy, hz of the
+// previous slab in registers, so no slab is read twice from HBM.
+hipError_t launch_step3d_march(const Step3DFArgs& a, int poles, int bb, hipStream_t s);
 
 hipError_t launch_h3d(const Step3DArgs& a, int bb, hipStream_t s);
 hipError_t launch_e3d(const Step3DArgs& a, int poles, int bb, hipStream_t s);

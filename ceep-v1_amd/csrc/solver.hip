@@ -132,7 +132,8 @@ struct ceep_fdtd {
   int tb_k = 0;                // 2D temporal depth (0: one step per launch)
   int tb_r = 16;               // 2D rows per wavefront window
   int tb_bpw = 1;              // 2D batch planes per wavefront
-  int fused3d = 0;             // 3D: 1/2 = one fused H+E launch per step (ping-pong fields)
+  int fused3d = 0;             // 3D: 1/2 = one fused H+E launch per step, 3 = marching form (ping-pong fields)
+  int march_len = 24;          // 3D marching form: x-slabs per workgroup
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
@@ -404,11 +405,15 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
         ceep::Step3DFArgs fa{};
         fa.a = a;
         fa.xcd_order = h->fused3d == 1;
+        fa.seg_len = h->march_len;
         for (int k = 0; k < 3; ++k) {
           fa.e_in[k] = h->f[i][k]; fa.h_in[k] = h->f[i][3 + k];
           fa.e_out[k] = h->f[o][k]; fa.h_out[k] = h->f[o][3 + k];
         }
-        CEEP_HIP(ceep::launch_step3d_fused(fa, poles, bb, s));
+        if (h->fused3d == 3)
+          CEEP_HIP(ceep::launch_step3d_march(fa, poles, bb, s));
+        else
+          CEEP_HIP(ceep::launch_step3d_fused(fa, poles, bb, s));
         continue;
       }
       CEEP_HIP(ceep::launch_h3d(a, bb, s));
@@ -1394,12 +1399,15 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 4) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
     h->tb_bpw = value;
+  } else if (key == 8) {
+    if (value < 1 || value > 65536) return fail(CEEP_EINVAL, "x-slabs per marching workgroup must be >= 1");
+    h->march_len = value;
   } else if (key == 7) {
     if (value != 0 && value != 1) return fail(CEEP_EINVAL, "adjoint form must be 0 or 1");
     if (value != h->adj_rev) h->ck_valid = false;
     h->adj_rev = value;
   } else if (key == 6) {
-    if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D step form must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(CEEP_EINVAL, "3D step form must be 0, 1, 2 or 3");
     if ((value != 0) != (h->fused3d != 0)) {   // the state's buffer convention changes: start over
       h->fused3d = value;
       h->ck_valid = false;

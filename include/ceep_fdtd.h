@@ -153,12 +153,14 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          the adjoint stays FP32.  Not a launch-shape knob: results differ.
  *   key 6: 3D step form: 0 = H kernel then E kernel in place; 1 = one
  *          fused H+E launch per step on ping-pong fields, workgroups ordered
- *          so each XCD walks contiguous x-slabs; 2 = fused, dispatch order.
- *          Switching between 0 and 1/2 clears the fields.
+ *          so each XCD walks contiguous x-slabs; 2 = fused, dispatch order;
+ *          3 = fused, each thread marching along x over key-8 slabs.
+ *          Switching between 0 and 1/2/3 clears the fields.
  *   key 7: 2D FP32 blocked adjoint form: 1 = reversible (the forward state is
  *          rebuilt backward inside the adjoint launches from the checkpoint at
  *          each segment's end; no recomputation, no E store; default),
- *          0 = recompute each segment into an E store. */
+ *          0 = recompute each segment into an E store.
+ *   key 8: x-slabs per workgroup of the marching 3D form (default 24). */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* Geometry of the stepped box: out[0..2] = box size per axis, out[3] =

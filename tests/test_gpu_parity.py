@@ -388,8 +388,9 @@ def test_3d_batch_independence_bitwise(amd):
 
 @pytest.mark.parametrize("magnetic", [False, True])
 def test_3d_fused_step_matches_two_kernel_step(amd, magnetic):
-    """The fused H+E launch (default) against the H-then-E pair, both against
-    the FP64 oracle; Debye + Lorentz poles, optional mu_r / sigma_m maps."""
+    """The fused H+E launch and its marching form against the H-then-E pair
+    (the default), all against the FP64 oracle; Debye + Lorentz poles,
+    optional mu_r / sigma_m maps."""
     shape, n, T, dx = (34, 30, 28), 4, 90, 1e-3
     dt = W.dt_3d_courant(dx, 0.5)
     w = W.modulated_gaussian(T, dt, 10e9, 20e9, delay_factor=3.0)
@@ -402,7 +403,7 @@ def test_3d_fused_step_matches_two_kernel_step(amd, magnetic):
     prb = src + [(22, 11, 17)]
     lor = (m["lorentz_delta_eps"], m["lorentz_omega_0"], m["lorentz_delta"])
     out = {}
-    for fused in (1, 0):
+    for fused in (3, 1, 0):
         s = amd.BatchedFDTD3D(nx=shape[0], ny=shape[1], nz=shape[2], dx=dx, total_steps=T,
                               cpml_thickness=n, source_positions=src, probe_positions=prb,
                               frequency=10e9, courant=0.5, waveform=w)
@@ -411,7 +412,7 @@ def test_3d_fused_step_matches_two_kernel_step(amd, magnetic):
         if magnetic:
             s._mu_r[:] = mu
             s._sigma_m[:] = sm
-        s.tuning = {"fused_3d": fused}
+        s.tuning = {"fused_3d": fused, "march_slabs": 7}    # several x-runs, a ragged last one
         s.run()
         out[fused] = (s.probe_data.copy(), s.ez, s.hx, s.compute_energy())
     poles = [ade.debye_pole(m["delta_eps"], m["tau"], dt), ade.lorentz_pole(*lor, dt)]
@@ -424,4 +425,6 @@ def test_3d_fused_step_matches_two_kernel_step(amd, magnetic):
     assert e1 < TRACE_TOL and e0 < TRACE_TOL and d < 1e-5
     assert rel_l2(out[1][1], out[0][1]) < 1e-5 and rel_l2(out[1][2], out[0][2]) < 1e-5
     assert out[1][3] == pytest.approx(out[0][3], rel=1e-5)
+    # the marching form computes the same operations as the fused launch
+    assert np.array_equal(out[3][0], out[1][0]) and np.array_equal(out[3][1], out[1][1])
 

@@ -28,6 +28,7 @@ CEEP_EUNSUPPORTED = 5
 
 BOX_REFERENCE = 0
 BOX_FULL = 1
+BOX_CPML = 2
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -63,6 +64,7 @@ SIGNATURES = {
     "ceep_fdtd_multi_info": (_I, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     "ceep_fdtd_multi_destroy": (_I, [_P]),
     "ceep_fdtd_box_info": (_I, [_P, _PI64]),
+    "ceep_fdtd_set_cpml": (_I, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
 }
 
 _lib = None
@@ -265,6 +267,10 @@ class Handle:
     def apply_tuning(self, tuning):
         for k, v in (tuning or {}).items():
             self.set_tuning(k, v)
+
+    def set_cpml(self, order=4, sigma_factor=2.0, alpha_max=0.05, kappa_max=1.0):
+        check(self._lib.ceep_fdtd_set_cpml(self._h, int(order), float(sigma_factor),
+                                           float(alpha_max), float(kappa_max)))
 
     def box_info(self):
         out = np.zeros(5, dtype=np.int64)

@@ -45,7 +45,21 @@ class BatchedFDTD2D:
                  source_positions: List[Tuple[int, int]],
                  probe_positions: List[Tuple[int, int]], frequency: float,
                  delay_factor: float = 5.0, *, courant: Optional[float] = None,
-                 waveform: Optional[np.ndarray] = None, cpml_R: float = 1e-8):
+                 waveform: Optional[np.ndarray] = None, cpml_R: float = 1e-8,
+                 boundary: str = "pec", cpml_order: int = 4, cpml_sigma_factor: float = 2.0,
+                 cpml_alpha_max: float = 0.05, cpml_kappa_max: float = 1.0):
+        """``boundary="pec"`` (default) is the reference's batched solver: its
+        CPML is inert, so the result is the PEC cavity it encloses.
+        ``boundary="cpml"`` is an opt-in physical absorbing boundary: the full
+        grid stepped in the order of the reference's single-run ``FDTD2D``
+        with its ``CPML(thickness=cpml_thickness, order, sigma_factor,
+        alpha_max, kappa_max)`` (boundaries/absorbing.py:174-561); sources and
+        probes may lie anywhere in the grid."""
+        if boundary not in ("pec", "cpml"):
+            raise ValueError("boundary must be 'pec' or 'cpml'")
+        self.boundary = boundary
+        self.cpml_params = dict(order=cpml_order, sigma_factor=cpml_sigma_factor,
+                                alpha_max=cpml_alpha_max, kappa_max=cpml_kappa_max)
         self.nx = nx
         self.ny = ny
         self.dx = dx
@@ -177,7 +191,11 @@ class BatchedFDTD2D:
         self.waveform = self._waveform()
         if self._handle is not None:
             self._handle.close()
-        h = self._make_handle(_native.BOX_REFERENCE)
+        if self.boundary == "cpml":
+            h = self._make_handle(_native.BOX_CPML)
+            h.set_cpml(**self.cpml_params)
+        else:
+            h = self._make_handle(_native.BOX_REFERENCE)
         h.set_sources(src, self.waveform)
         h.set_probes(prb)
         h.reset()

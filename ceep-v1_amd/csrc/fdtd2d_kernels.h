@@ -100,6 +100,34 @@ struct Step2D64Args {
 
 hipError_t launch_step2d_f64(const Step2D64Args& a, hipStream_t s);
 
+// Physical-CPML mode (full grid, the reference single-run solver's step
+// order and convolutional PML); profiles per depth index.
+constexpr int kMaxCpml = 64;
+struct Cpml2DArgs {
+  int Mx, My, pitch, B, t;        // full grid, CPML thickness t
+  long long bstride;
+  float* ez; float* hx; float* hy;
+  float* jd; float* jl_cur; float* jl_prev;
+  float* psi_hyx; float* psi_hxy; float* psi_ezx; float* psi_ezy;   // [B][plane]
+  const float* ca; const float* cb;
+  const float* d_k1; const float* d_k2dt;
+  const float* l_k1; const float* l_k2dt; const float* l_k3;
+  float dbdy;                     // (dt / mu0) / dx
+  float idx;                      // 1 / dx
+  float dtmu;                     // dt / mu0
+  float dteps;                    // dt / eps0
+  float bh[kMaxCpml], chh[kMaxCpml], be[kMaxCpml], ce[kMaxCpml];
+  const int* src_cell;
+  const float* wave;
+  long long wave_stride_b;
+  int step, T, P;
+  const short* probe_head;
+  const short* probe_next;
+  float* traces;
+};
+
+hipError_t launch_cpml2d(const Cpml2DArgs& a, int poles, hipStream_t s);
+
 // poles: bit 0 = Debye pole present, bit 1 = Lorentz pole present.
 // bb: batch planes per thread (1, 2, 4, 8).
 hipError_t launch_step2d(const Step2DArgs& a, int poles, int bb, hipStream_t s);

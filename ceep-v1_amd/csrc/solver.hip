@@ -138,13 +138,20 @@ struct ceep_fdtd {
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
 
+  // physical-CPML mode (CEEP_BOX_CPML): profile parameters of the reference's
+  // CPML class (absorbing.py:226-230 defaults) and the psi state
+  int cpml_order = 4;
+  double cpml_sigma_factor = 2.0, cpml_alpha_max = 0.05, cpml_kappa_max = 1.0;
+  float* psi[4] = {nullptr, nullptr, nullptr, nullptr};   // hyx, hxy, ezx, ezy
+  bool cpml() const { return box_mode == CEEP_BOX_CPML; }
+
   long long box_floats() const { return (long long)B * plane; }
   // field buffer holding the current state (3D fields stay in buffer 0 with
   // the two-kernel in-place update)
-  int fbuf() const { return (ndim == 2 || fused3d) ? parity : 0; }
+  int fbuf() const { return cpml() ? 0 : (ndim == 2 || fused3d) ? parity : 0; }
   int poles() const { return (has_debye ? 1 : 0) | (has_lorentz ? 2 : 0); }
   // the temporally blocked 2D kernel covers the plain and Debye cases
-  bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz; }
+  bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz && !cpml(); }
   // 2D gradient path: reversible reconstruction instead of recompute + E store
   bool use_rev() const { return adj_rev && use_tb() && !fwd64; }
   int launches_per_run() const { return use_tb() ? (T + tb_k - 1) / tb_k : T; }
@@ -237,8 +244,10 @@ int commit_materials(ceep_fdtd* h) {
         const double s = h->sigma_e[g] * dt / (2.0 * eps);
         // E is stepped on the interior only; elsewhere every E-side
         // coefficient is 0 so the kernels' uniform update keeps E at 0
-        const bool interior = a >= 1 && a <= h->M[0] - 2 && b >= 1 && b <= h->M[1] - 2 &&
-                              (h->ndim == 2 || (c >= 1 && c <= mz - 2));
+        const bool interior =
+            h->cpml() ? (a >= 1 && b >= 1)        // the E update domain [1:, 1:] (fdtd_2d.py:168)
+                      : (a >= 1 && a <= h->M[0] - 2 && b >= 1 && b <= h->M[1] - 2 &&
+                         (h->ndim == 2 || (c >= 1 && c <= mz - 2)));
         ca[o] = interior ? (float)((1.0 - s) / (1.0 + s)) : 0.f;
         cb[o] = interior ? (float)((dt / eps) / (1.0 + s)) : 0.f;
         if (!interior) {
@@ -311,6 +320,8 @@ int zero_state(ceep_fdtd* h) {
     CEEP_HIP(hipMemsetAsync(h->f[0][k], 0, bytes, s));
     CEEP_HIP(hipMemsetAsync(h->f[1][k], 0, bytes, s));
   }
+  for (int k = 0; k < 4; ++k)
+    if (h->psi[k]) CEEP_HIP(hipMemsetAsync(h->psi[k], 0, bytes, s));
   for (int k = 0; k < 3; ++k) {
     if (h->jd[k]) CEEP_HIP(hipMemsetAsync(h->jd[k], 0, bytes, s));
     if (h->jl[0][k]) CEEP_HIP(hipMemsetAsync(h->jl[0][k], 0, bytes, s));
@@ -321,6 +332,27 @@ int zero_state(ceep_fdtd* h) {
 }
 
 int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save);
+
+// CPML profiles of CPML._compute_pml_params (absorbing.py:266-311) in FP64,
+// rounded once: b, c at the E (integer) and H (half-integer) depths of a slab
+// of thickness n cells (dx == dy, so one table serves both axes).
+void cpml_profiles(const ceep_fdtd* h, ceep::Cpml2DArgs& a) {
+  const int n = h->n;
+  const double dt = h->dt, d = h->dx, ord = (double)h->cpml_order;
+  const double sigma_max = h->cpml_sigma_factor * (h->cpml_order + 1) / (150.0 * M_PI * d);
+  for (int k = 0; k < n; ++k) {
+    for (int half = 0; half < 2; ++half) {
+      const double rho = half ? (k + 0.5) / n : (double)k / n;
+      const double sg = sigma_max * std::pow(rho, ord);
+      const double kap = 1.0 + (h->cpml_kappa_max - 1.0) * std::pow(rho, ord);
+      const double al = h->cpml_alpha_max * (1.0 - rho);
+      const double b = std::exp(-(sg / kap + al) * dt / kEps0);
+      const double c = std::fabs(sg) < 1e-20 ? 0.0 : sg / (sg * kap + kap * kap * al) * (b - 1.0);
+      if (half) { a.bh[k] = (float)b; a.chh[k] = (float)c; }
+      else { a.be[k] = (float)b; a.ce[k] = (float)c; }
+    }
+  }
+}
 
 // Enqueue total_steps steps starting from ping-pong buffer `start`.
 int launch_steps(ceep_fdtd* h, int start) {
@@ -359,6 +391,29 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
       CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
       cur = o;
+    }
+  } else if (h->cpml()) {
+    ceep::Cpml2DArgs a{};
+    a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.t = h->n;
+    a.bstride = h->plane;
+    a.ez = h->f[0][0]; a.hx = h->f[0][1]; a.hy = h->f[0][2];
+    a.jd = h->jd[0];
+    for (int k = 0; k < 4; ++k) (&a.psi_hyx)[k] = h->psi[k];
+    a.ca = h->ca; a.cb = h->cb;
+    a.d_k1 = h->dk1; a.d_k2dt = h->dk2; a.l_k1 = h->lk1; a.l_k2dt = h->lk2; a.l_k3 = h->lk3;
+    a.dbdy = (float)((h->dt / kMu0) / h->dx);
+    a.idx = h->idx;
+    a.dtmu = (float)(h->dt / kMu0);
+    a.dteps = (float)(h->dt / kEps0);
+    cpml_profiles(h, a);
+    a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+    a.T = h->T; a.P = h->P;
+    a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
+    for (int st = step0; st < step_end; ++st) {
+      const int i = (start + st - step0) & 1, o = i ^ 1;
+      a.jl_cur = h->jl[i][0]; a.jl_prev = h->jl[o][0];
+      a.step = st;
+      CEEP_HIP(ceep::launch_cpml2d(a, poles, s));
     }
   } else if (h->ndim == 2) {
     ceep::Step2DArgs a{};
@@ -528,7 +583,7 @@ int run_device(ceep_fdtd* h) {
   }
   if (rc) return rc;
   h->parity = start ^ (h->launches_per_run() & 1);
-  if (h->ndim == 2 && h->has_debye && !h->use_tb() && h->parity != start) {
+  if (h->ndim == 2 && h->has_debye && !h->use_tb() && !h->cpml() && h->parity != start) {
     // the per-step kernel updates J in place in jd[start]; keep J with the fields
     CEEP_HIP(ceep::launch_copy(h->jd[start], h->jd[h->parity], h->box_floats(), h->stream));
   }
@@ -557,10 +612,17 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
   *out = nullptr;
   if (ndim != 2 && ndim != 3) return fail(CEEP_EINVAL, "ndim must be 2 or 3");
   if (!shape) return fail(CEEP_EINVAL, "shape is null");
-  if (box_mode != CEEP_BOX_REFERENCE && box_mode != CEEP_BOX_FULL)
+  if (box_mode != CEEP_BOX_REFERENCE && box_mode != CEEP_BOX_FULL && box_mode != CEEP_BOX_CPML)
     return fail(CEEP_EINVAL, "unknown box mode");
   if (box_mode == CEEP_BOX_FULL && ndim != 2)
     return fail(CEEP_EINVAL, "the full-grid PEC box (run_cpu) exists for 2D only");
+  if (box_mode == CEEP_BOX_CPML) {
+    if (ndim != 2) return fail(CEEP_EUNSUPPORTED, "the physical-CPML mode covers 2D TMz");
+    if (cpml_n < 1 || cpml_n > ceep::kMaxCpml)
+      return fail(CEEP_EINVAL, "CPML thickness must be in [1, 64]");
+    if (2 * (int64_t)cpml_n > shape[0] || 2 * (int64_t)cpml_n > shape[1])
+      return fail(CEEP_EINVAL, "the two CPML slabs of an axis must not overlap (2*thickness <= N)");
+  }
   if (batch < 1 || batch > ceep::kMaxBatch) return fail(CEEP_EINVAL, "batch must be in [1, 1024]");
   if (n_probes < 0 || n_probes > 32767) return fail(CEEP_EINVAL, "n_probes must be in [0, 32767]");
   if (total_steps < 1) return fail(CEEP_EINVAL, "total_steps must be >= 1");
@@ -609,6 +671,7 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
     rc = dalloc(&h->f[0][k], h->box_floats());
     if (!rc) rc = dalloc(&h->f[1][k], h->box_floats());
   }
+  for (int k = 0; k < 4 && !rc && h->cpml(); ++k) rc = dalloc(&h->psi[k], h->box_floats());
   if (!rc) rc = dalloc(&h->ca, h->plane);
   if (!rc) rc = dalloc(&h->cb, h->plane);
   if (!rc) rc = dalloc(&h->traces, (size_t)batch * (n_probes > 0 ? n_probes : 1) * total_steps);
@@ -653,6 +716,7 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   for (int i = 0; i < 2; ++i)
     for (int k = 0; k < 3; ++k) dfree(h->f64[i][k]);
   dfree(h->ca64); dfree(h->cb64); dfree(h->wave64); dfree(h->traces64);
+  for (int k = 0; k < 4; ++k) dfree(h->psi[k]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CEEP_OK;
@@ -1167,6 +1231,7 @@ extern "C" {
 int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   int rc = ready_check(h);
   if (rc) return rc;
+  if (h->cpml()) return fail(CEEP_EUNSUPPORTED, "the adjoint covers the reference's cavity modes, not the CPML mode");
   if (every < 1) return fail(CEEP_EINVAL, "checkpoint interval must be >= 1");
   if (h->ndim == 3) return run_checkpointed3d(h, every, traces_out);
   if (h->has_debye || h->has_lorentz)
@@ -1428,6 +1493,21 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   return CEEP_OK;
 }
 
+int ceep_fdtd_set_cpml(ceep_fdtd* h, int order, double sigma_factor, double alpha_max,
+                       double kappa_max) {
+  if (!h) return fail(CEEP_EINVAL, "null handle");
+  if (!h->cpml()) return fail(CEEP_ESTATE, "the handle was not created in CEEP_BOX_CPML mode");
+  if (order < 0 || order > 16) return fail(CEEP_EINVAL, "CPML grading order must be in [0, 16]");
+  if (!(sigma_factor >= 0) || !(alpha_max >= 0) || !(kappa_max >= 1))
+    return fail(CEEP_EINVAL, "CPML needs sigma_factor >= 0, alpha_max >= 0, kappa_max >= 1");
+  h->cpml_order = order;
+  h->cpml_sigma_factor = sigma_factor;
+  h->cpml_alpha_max = alpha_max;
+  h->cpml_kappa_max = kappa_max;
+  h->drop_graph();   // the profiles travel in the launch arguments
+  return CEEP_OK;
+}
+
 int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out) {
   if (!h || !out) return fail(CEEP_EINVAL, "null argument");
   out[0] = h->M[0]; out[1] = h->M[1]; out[2] = h->ndim == 3 ? h->M[2] : 1;
@@ -1521,8 +1601,8 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   for (int i = 0; i < n; ++i) {
     ceep_fdtd* h = handles[i];
     if ((rc = ready_check(h))) return rc;
-    if (h->ndim != 2 || h->has_lorentz)
-      return fail(CEEP_EUNSUPPORTED, "uneven batches take 2D problems without Lorentz poles");
+    if (h->ndim != 2 || h->has_lorentz || h->cpml())
+      return fail(CEEP_EUNSUPPORTED, "uneven batches take 2D cavity problems without Lorentz poles");
     if (poles >= 0 && poles != h->poles())
       return fail(CEEP_EUNSUPPORTED, "every problem of an uneven batch needs the same pole set");
     poles = h->poles();

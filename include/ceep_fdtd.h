@@ -36,7 +36,11 @@ enum {
 /* Box modes for ceep_fdtd_create. */
 enum {
   CEEP_BOX_REFERENCE = 0, /* BatchedFDTD.run(): E interior [n+1, N-n-1], box [n, N-n] */
-  CEEP_BOX_FULL = 1       /* BatchedFDTD2D.run_cpu(): PEC box over the full grid */
+  CEEP_BOX_FULL = 1,      /* BatchedFDTD2D.run_cpu(): PEC box over the full grid */
+  CEEP_BOX_CPML = 2       /* 2D opt-in physical absorbing boundary: the full grid stepped
+                             in the order of FDTDBase.step (fdtd_base.py:161-185) with the
+                             CPML class's psi recursions (absorbing.py:266-561) on slabs of
+                             cpml_n cells; sources and probes anywhere */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -162,6 +166,13 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          0 = recompute each segment into an E store.
  *   key 8: x-slabs per workgroup of the marching 3D form (default 24). */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
+
+/* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,
+ * sigma_factor 2.0, alpha_max 0.05, kappa_max 1.0, as CPML's fields,
+ * absorbing.py:226-230).  Replaces: the CPML(...) boundary object handed to
+ * FDTD2D (solvers/fdtd_2d.py, boundaries/absorbing.py:174-311). */
+int ceep_fdtd_set_cpml(ceep_fdtd* h, int order, double sigma_factor, double alpha_max,
+                       double kappa_max);
 
 /* Geometry of the stepped box: out[0..2] = box size per axis, out[3] =
  * row pitch in floats, out[4] = box origin (grid index of box cell 0). */

@@ -68,3 +68,20 @@ def modulated_gaussian(total_steps: int, dt: float, frequency: float,
     envelope = np.exp(-((t - t0) / tau) ** 2)
     carrier = np.sin(2.0 * math.pi * frequency * t)
     return amplitude * envelope * carrier
+
+
+def modulated_gaussian_value_at(total_steps: int, dt: float, frequency: float,
+                                bandwidth: float, amplitude: float = 1.0,
+                                delay_factor: float = 6.0) -> np.ndarray:
+    """ModulatedGaussianSource.value_at (waveforms.py:114-118) for each step:
+    the scalar ``math`` form ``FDTD2D._inject_sources`` uses, which may differ
+    from the vectorised ``waveform`` in the last bit."""
+    tau = 1.0 / (math.pi * bandwidth)
+    t0 = delay_factor * tau
+    out = np.empty(total_steps)
+    for k in range(total_steps):
+        t = k * dt
+        envelope = math.exp(-((t - t0) / tau) ** 2)
+        carrier = math.sin((2.0 * math.pi) * frequency * t)
+        out[k] = amplitude * envelope * carrier
+    return out

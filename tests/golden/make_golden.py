@@ -16,6 +16,9 @@ Vectors written (all float64, ``numpy.savez_compressed``, no pickles):
 * ``ade2d_*.npz`` -- ``FDTD2D`` with one Debye / one Lorentz pole
   (fdtd_2d.py:142-178, dispersive.py:240-426) on a grid the pulse never
   crosses, pinning the ADE operation order.
+* ``cpml2d_*.npz`` -- ``FDTD2D`` with a ``CPML`` boundary
+  (boundaries/absorbing.py:174-561, fdtd_base.py:161-185): the physical
+  absorbing-boundary mode, traces and final fields.
 """
 from __future__ import annotations
 
@@ -174,9 +177,56 @@ def golden_ade2d():
                  nx=nx, ny=ny, dx=dx)
 
 
+def golden_cpml2d():
+    """``FDTD2D`` (TMz) with a ``CPML`` boundary (boundaries/absorbing.py:174-561):
+    one soft source, probes in the interior, the PML slabs and a corner, a
+    lossy disk; default CPML parameters and a second set (thickness 8, order 3,
+    sigma_factor 1.5, alpha_max 0.02, kappa_max 3), plus one run with a Debye
+    pole in the disk."""
+    from ceep.boundaries.absorbing import CPML
+    from ceep.core.backend import set_backend
+    from ceep.core.config import GridConfig, SimulationConfig
+    from ceep.materials.dispersive import DebyePole
+    from ceep.solvers.fdtd_2d import FDTD2D
+    from ceep.sources.waveforms import ModulatedGaussianSource
+    set_backend("numpy")
+
+    nx, ny, T, dx = 60, 52, 240, 1e-3
+    probes = [(25, 22), (40, 30), (3, 20), (57, 45), (2, 2), (30, 50)]
+    for name, ckw, debye in (
+        ("default", dict(thickness=10), None),
+        ("custom", dict(thickness=8, order=3, sigma_factor=1.5, alpha_max=0.02, kappa_max=3.0), None),
+        ("debye", dict(thickness=10), DebyePole(delta_eps=30.0, tau=8e-12)),
+    ):
+        cfg = SimulationConfig(grid=GridConfig(nx=nx, ny=ny, dx=dx, dy=dx), courant=0.5,
+                               total_steps=T)
+        src = ModulatedGaussianSource(x=25, y=22, frequency=20e9, bandwidth=30e9,
+                                      amplitude=1.0, delay_factor=3.0)
+        cpml = CPML(**ckw)
+        sim = FDTD2D(config=cfg, sources=[src], boundaries=[cpml], probe_points=probes)
+        kw = {"debye_poles": [debye]} if debye is not None else {}
+        sim.grid.set_material_circle(33, 27, 7, eps_r=6.0, sigma_e=0.4, **kw)
+        sim.run(T)
+        tr = np.array([sim.probe_data[p] for p in probes], dtype=np.float64)
+        extra = {}
+        if debye is not None:
+            extra = dict(delta_eps=np.asarray(sim.grid.dispersive.delta_eps[0]),
+                         tau=np.asarray(sim.grid.dispersive.tau[0]))
+        np.savez_compressed(os.path.join(HERE, f"cpml2d_{name}.npz"), traces=tr, dt=cfg.dt,
+                            waveform=src.waveform(T, cfg.dt),
+                            eps_r=np.asarray(sim.grid.eps_inf), sigma_e=np.asarray(sim.grid.sigma_e),
+                            ez=np.asarray(sim.grid.ez), hx=np.asarray(sim.grid.hx),
+                            hy=np.asarray(sim.grid.hy), probes=np.array(probes),
+                            source=np.array([25, 22]), nx=nx, ny=ny, dx=dx,
+                            thickness=cpml.thickness, order=cpml.order,
+                            sigma_factor=cpml.sigma_factor, alpha_max=cpml.alpha_max,
+                            kappa_max=cpml.kappa_max, **extra)
+
+
 if __name__ == "__main__":
     golden_ceep_results()
     golden_bf2d()
     golden_bf3d()
     golden_ade2d()
+    golden_cpml2d()
     print("golden vectors written to", HERE)

@@ -6,7 +6,7 @@ The solvers run in hand-written HIP kernels for gfx950 behind the C ABI of
 from .batched_2d import BatchedFDTD2D
 from .batched_3d import BatchedFDTD3D
 from .sparams import MultistaticSParameters
-from . import _native, phantoms
+from . import _native, imaging, phantoms
 
 __all__ = ["BatchedFDTD2D", "BatchedFDTD3D", "MultistaticSParameters"]
 __version__ = "0.1.0"

@@ -65,6 +65,8 @@ SIGNATURES = {
     "ceep_fdtd_multi_destroy": (_I, [_P]),
     "ceep_fdtd_box_info": (_I, [_P, _PI64]),
     "ceep_fdtd_set_cpml": (_I, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "ceep_beamform": (_I, [_PD, ctypes.c_int, ctypes.c_int, _PD, _PI64, _PI64, _PI64,
+                           ctypes.c_double, ctypes.c_double, _PD, ctypes.c_int, ctypes.c_int, _PD]),
 }
 
 _lib = None

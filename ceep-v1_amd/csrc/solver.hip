@@ -19,11 +19,13 @@
 #include "fdtd2d_kernels.h"
 #include "fdtd3d_kernels.h"
 #include "reduce_kernels.h"
+#include "imaging.h"
 
 namespace {
 
 constexpr double kEps0 = 8.854187817e-12;            // core/constants.py:24
 constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1e-7;  // core/constants.py:27
+constexpr double kC0 = 299792458.0;                    // core/constants.py:21
 
 thread_local std::string g_err;
 
@@ -1505,6 +1507,68 @@ int ceep_fdtd_set_cpml(ceep_fdtd* h, int order, double sigma_factor, double alph
   h->cpml_alpha_max = alpha_max;
   h->cpml_kappa_max = kappa_max;
   h->drop_graph();   // the profiles travel in the launch arguments
+  return CEEP_OK;
+}
+
+int ceep_beamform(const double* s_matrix, int n_freq, int n_ant, const double* freqs,
+                  const int64_t* antennas, const int64_t* x_range, const int64_t* y_range,
+                  double dx, double background_permittivity, const double* frequency_range,
+                  int method, int normalize, double* image_out) {
+  if (!s_matrix || !freqs || !antennas || !x_range || !y_range || !image_out)
+    return fail(CEEP_EINVAL, "null argument");
+  if (n_freq < 1 || n_ant < 1) return fail(CEEP_EINVAL, "need frequencies and antennas");
+  if (method != 0 && method != 1) return fail(CEEP_EINVAL, "method must be 0 (das) or 1 (dmas)");
+  if (!(dx > 0) || !(background_permittivity > 0)) return fail(CEEP_EINVAL, "dx and permittivity must be > 0");
+  const int64_t nx = x_range[1] - x_range[0], ny = y_range[1] - y_range[0];
+  if (nx < 1 || ny < 1 || nx * ny > (1LL << 31)) return fail(CEEP_EINVAL, "bad imaging region");
+  // frequency selection and the S columns it keeps (beamforming.py:131-139)
+  std::vector<double> fsel;
+  std::vector<double> ssel;
+  for (int f = 0; f < n_freq; ++f) {
+    if (frequency_range && !(freqs[f] >= frequency_range[0] && freqs[f] <= frequency_range[1])) continue;
+    fsel.push_back(freqs[f]);
+    ssel.insert(ssel.end(), s_matrix + (size_t)f * n_ant * n_ant * 2,
+                s_matrix + (size_t)(f + 1) * n_ant * n_ant * 2);
+  }
+  std::vector<double> ant(2 * (size_t)n_ant);
+  for (int k = 0; k < 2 * n_ant; ++k) ant[k] = (double)antennas[k] * dx;
+  const size_t npix = (size_t)(nx * ny);
+  double *d_s = nullptr, *d_f = nullptr, *d_a = nullptr, *d_o = nullptr;
+  int rc = CEEP_OK;
+  const size_t nf = fsel.size();
+  if (!rc) rc = dalloc(&d_s, std::max<size_t>(ssel.size(), 2));
+  if (!rc) rc = dalloc(&d_f, std::max<size_t>(nf, 1));
+  if (!rc) rc = dalloc(&d_a, ant.size());
+  if (!rc) rc = dalloc(&d_o, npix);
+  hipError_t e = hipSuccess;
+  if (!rc && nf) e = hipMemcpy(d_s, ssel.data(), ssel.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (!rc && nf && e == hipSuccess) e = hipMemcpy(d_f, fsel.data(), nf * sizeof(double), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) e = hipMemcpy(d_a, ant.data(), ant.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) {
+    ceep::BeamformArgs a{};
+    a.s = reinterpret_cast<const double2*>(d_s); a.freqs = d_f; a.ant = d_a;
+    a.n_freq = (int)nf; a.n_ant = n_ant;
+    a.nx = (int)nx; a.ny = (int)ny; a.x0 = (int)x_range[0]; a.y0 = (int)y_range[0];
+    a.dx = dx;
+    a.velocity = kC0 / std::sqrt(background_permittivity);
+    a.two_pi = 2.0 * M_PI;
+    a.dmas = method;
+    a.out = d_o;
+    e = ceep::launch_beamform(a, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(image_out, d_o, npix * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  dfree(d_s); dfree(d_f); dfree(d_a); dfree(d_o);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(CEEP_EHIP, std::string("beamform: ") + hipGetErrorString(e));
+  if (normalize) {   // beamforming.py:249-255
+    double mn = image_out[0];
+    for (size_t k = 1; k < npix; ++k) mn = std::min(mn, image_out[k]);
+    double mx = 0.0;
+    for (size_t k = 0; k < npix; ++k) { image_out[k] -= mn; mx = std::max(mx, image_out[k]); }
+    if (mx > 0)
+      for (size_t k = 0; k < npix; ++k) image_out[k] /= mx;
+  }
   return CEEP_OK;
 }
 

@@ -174,6 +174,23 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 int ceep_fdtd_set_cpml(ceep_fdtd* h, int order, double sigma_factor, double alpha_max,
                        double kappa_max);
 
+/* Delay-and-sum / delay-multiply-and-sum image from a multistatic S-matrix.
+ * Replaces: DelayAndSumBeamformer(s_parameters, antenna_positions,
+ * ImagingRegion(x_range, y_range, dx, background_permittivity),
+ * frequency_range).reconstruct(method, normalize)
+ * (src/ceep/imaging/beamforming.py:86-313).
+ * s_matrix: [n_freq][n_ant (RX)][n_ant (TX)] complex128 as interleaved
+ * (re, im) doubles (the MultistaticSParameters layout); antennas: [n_ant][2]
+ * grid indices; x_range, y_range: [2] pixel index ranges (end excluded);
+ * frequency_range: [2] inclusive (f_min, f_max) or NULL for all; method:
+ * 0 = DAS, 1 = DMAS; image_out: [nx][ny] magnitude, normalised to [0, 1]
+ * (minus the minimum, over the maximum) when normalize != 0.  Synchronous;
+ * allocates and frees its own device buffers. */
+int ceep_beamform(const double* s_matrix, int n_freq, int n_ant, const double* freqs,
+                  const int64_t* antennas, const int64_t* x_range, const int64_t* y_range,
+                  double dx, double background_permittivity, const double* frequency_range,
+                  int method, int normalize, double* image_out);
+
 /* Geometry of the stepped box: out[0..2] = box size per axis, out[3] =
  * row pitch in floats, out[4] = box origin (grid index of box cell 0). */
 int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out);

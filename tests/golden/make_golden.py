@@ -16,6 +16,8 @@ Vectors written (all float64, ``numpy.savez_compressed``, no pickles):
 * ``ade2d_*.npz`` -- ``FDTD2D`` with one Debye / one Lorentz pole
   (fdtd_2d.py:142-178, dispersive.py:240-426) on a grid the pulse never
   crosses, pinning the ADE operation order.
+* ``das_small.npz`` -- ``DelayAndSumBeamformer.reconstruct`` DAS / DMAS images
+  (imaging/beamforming.py:86-313) of a seeded S-matrix.
 * ``cpml2d_*.npz`` -- ``FDTD2D`` with a ``CPML`` boundary
   (boundaries/absorbing.py:174-561, fdtd_base.py:161-185): the physical
   absorbing-boundary mode, traces and final fields.
@@ -223,10 +225,37 @@ def golden_cpml2d():
                             kappa_max=cpml.kappa_max, **extra)
 
 
+def golden_das():
+    """``DelayAndSumBeamformer.reconstruct`` (imaging/beamforming.py:86-313),
+    DAS and DMAS, raw and normalised, on a seeded S-matrix."""
+    from ceep.imaging.beamforming import DelayAndSumBeamformer, ImagingRegion
+    from ceep.solvers.s_params import MultistaticSParameters
+
+    rng = np.random.default_rng(12)
+    n_ant, F = 6, 13
+    freqs = np.linspace(0.5e9, 3.5e9, F)
+    S = rng.standard_normal((F, n_ant, n_ant)) + 1j * rng.standard_normal((F, n_ant, n_ant))
+    ants = [(20 + int(14 * np.cos(2 * np.pi * k / n_ant)), 18 + int(13 * np.sin(2 * np.pi * k / n_ant)))
+            for k in range(n_ant)]
+    sp = MultistaticSParameters(frequencies=freqs, s_matrix=S, antenna_positions=ants)
+    region = ImagingRegion(x_range=(9, 31), y_range=(7, 27), dx=1e-3, background_permittivity=4.0)
+    out = {}
+    for method in ("das", "dmas"):
+        bf = DelayAndSumBeamformer(sp, ants, region, frequency_range=(1e9, 3e9))
+        out[f"{method}_raw"] = bf.reconstruct(method=method, normalize=False)
+        out[f"{method}_norm"] = bf.reconstruct(method=method, normalize=True)
+    np.savez_compressed(os.path.join(HERE, "das_small.npz"), freqs=freqs, s_matrix=S,
+                        antennas=np.array(ants), x_range=np.array(region.x_range),
+                        y_range=np.array(region.y_range), dx=region.dx,
+                        background_permittivity=region.background_permittivity,
+                        frequency_range=np.array([1e9, 3e9]), **out)
+
+
 if __name__ == "__main__":
     golden_ceep_results()
     golden_bf2d()
     golden_bf3d()
     golden_ade2d()
     golden_cpml2d()
+    golden_das()
     print("golden vectors written to", HERE)

@@ -132,11 +132,15 @@ TUNE_FP64_FORWARD = 5
 TUNE_FUSED_3D = 6
 TUNE_ADJOINT_REVERSIBLE = 7
 TUNE_MARCH_SLABS = 8
+TUNE_XCD_ORDER = 9
+TUNE_ADJ_ROWS = 10
+TUNE_ADJ_BPW = 11
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
                "fused_3d": TUNE_FUSED_3D, "adjoint_reversible": TUNE_ADJOINT_REVERSIBLE,
-               "march_slabs": TUNE_MARCH_SLABS}
+               "march_slabs": TUNE_MARCH_SLABS, "xcd_order": TUNE_XCD_ORDER,
+               "adjoint_rows_per_wave": TUNE_ADJ_ROWS, "adjoint_batch_per_wave": TUNE_ADJ_BPW}
 
 
 class Handle:

@@ -21,8 +21,8 @@ _LOADED = False
 
 # measured best on MI355X for the 600^2 x 16 Debye workload; used when nothing
 # was tuned for a shape
-DEFAULT_2D = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4,
-              "batch_per_thread": 4, "graph": 1}
+DEFAULT_2D = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 2,
+              "batch_per_thread": 4, "graph": 1, "xcd_order": 1}
 
 
 def _key(ndim, shape, n, batch, debye, lorentz):

@@ -167,6 +167,7 @@ struct TB2DArgs {
   int ntasks;                     // tiles_x * tiles_y * ceil(B / bpw)
   float* e_save;                  // if set: E after local step s -> e_save + s * slot ([B][plane])
   long long slot;                 // floats per e_save slot
+  int xcd_order;                  // contiguous tile ranges per XCD (xcd_block)
 };
 
 // poles: 0 or 1 (Debye); R in {16, 24, 32}; K in {1, 2, 3, 4}.
@@ -208,6 +209,7 @@ struct AdjTB2DArgs {
   float* fe_out;
   float* fhx_out;
   float* fhy_out;
+  int xcd_order;                  // contiguous tile ranges per XCD (xcd_block)
 };
 
 hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);

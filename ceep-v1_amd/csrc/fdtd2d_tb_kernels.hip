@@ -33,7 +33,11 @@ __device__ __forceinline__ float from_prev_lane(float v) {
 }  // namespace
 
 // One task = one (output tile, batch chunk) window advanced p.ksteps steps by
-// the calling wavefront.
+// the calling wavefront.  Only the rows that can still reach the output
+// window are updated: at local step k (0-based) H rows [k, R-k-1) and E rows
+// [k+1, R-k-1) (the rest of the window is stale and never read again), which
+// skips about 30 % of the row updates at R = 16, K = 4.  Loads and stores go
+// through buffer resources with one byte offset per window row.
 template <int R, int K, int POLES>
 __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
   const int tiles = p.tiles_x * p.tiles_y;
@@ -45,18 +49,33 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
   const int cb0 = ty * (64 - 2 * K) - K;              // column of lane 0
   const int c = cb0 + lane;
   const bool col_ok = c >= 0 && c < p.My;
+  // byte offset of this lane's cell in window row r; kOffNone off the box
+  // (a0 may be negative: the unsigned sum wraps back on rows inside the box)
+  const unsigned pitch4 = (unsigned)p.pitch * 4u;
+  const unsigned lane_off = col_ok ? (unsigned)(a0 * p.pitch + c) * 4u : kOffNone;
+  const unsigned plane_bytes = (unsigned)p.Mx * pitch4;
+  auto row_off = [&](int r) -> unsigned {
+    const int a = a0 + r;
+    return (a >= 0 && a < p.Mx) ? lane_off + (unsigned)r * pitch4 : kOffNone;
+  };
 
   float mca[R], mcb[R], mk1[R], mk2[R];
+  {
+    const __amdgpu_buffer_rsrc_t rca = plane_rsrc(p.ca, plane_bytes);
+    const __amdgpu_buffer_rsrc_t rcb = plane_rsrc(p.cb, plane_bytes);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int a = a0 + r;
-    const bool in = col_ok && a >= 0 && a < p.Mx;
-    const long long cell = (long long)a * p.pitch + c;
-    mca[r] = in ? p.ca[cell] : 0.f;
-    mcb[r] = in ? p.cb[cell] : 0.f;
+    for (int r = 0; r < R; ++r) {
+      mca[r] = bload(rca, row_off(r));
+      mcb[r] = bload(rcb, row_off(r));
+    }
     if (POLES & 1) {
-      mk1[r] = in ? p.d_k1[cell] : 0.f;
-      mk2[r] = in ? p.d_k2dt[cell] : 0.f;
+      const __amdgpu_buffer_rsrc_t rk1 = plane_rsrc(p.d_k1, plane_bytes);
+      const __amdgpu_buffer_rsrc_t rk2 = plane_rsrc(p.d_k2dt, plane_bytes);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        mk1[r] = bload(rk1, row_off(r));
+        mk2[r] = bload(rk2, row_off(r));
+      }
     }
   }
   const float ch = p.ch, idx = p.idx;
@@ -80,15 +99,22 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
   for (int bt = b_begin; bt < b_end; ++bt) {
     const long long base = (long long)bt * p.bstride;
     float ez[R], hx[R], hy[R], jd[R];
+    {
+      const __amdgpu_buffer_rsrc_t r_ez = plane_rsrc(p.ez_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hx = plane_rsrc(p.hx_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hy = plane_rsrc(p.hy_in + base, plane_bytes);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int a = a0 + r;
-      const bool in = col_ok && a >= 0 && a < p.Mx;
-      const long long o = base + (long long)a * p.pitch + c;
-      ez[r] = in ? p.ez_in[o] : 0.f;
-      hx[r] = in ? p.hx_in[o] : 0.f;
-      hy[r] = in ? p.hy_in[o] : 0.f;
-      if (POLES & 1) jd[r] = in ? p.jd_in[o] : 0.f;
+      for (int r = 0; r < R; ++r) {
+        const unsigned o = row_off(r);
+        ez[r] = bload(r_ez, o);
+        hx[r] = bload(r_hx, o);
+        hy[r] = bload(r_hy, o);
+      }
+      if (POLES & 1) {
+        const __amdgpu_buffer_rsrc_t r_jd = plane_rsrc(p.jd_in + base, plane_bytes);
+#pragma unroll
+        for (int r = 0; r < R; ++r) jd[r] = bload(r_jd, row_off(r));
+      }
     }
     // this TX's source, if it falls inside the window (halo included)
     const int sc = p.src_cell[bt];
@@ -96,12 +122,13 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
     const int s_lane = sc % p.pitch - cb0;
     const bool src_here = s_row >= 0 && s_row < R && s_lane >= 0 && s_lane < 64;
 
+#pragma unroll
     for (int s = 0; s < K; ++s) {
       if (s >= p.ksteps) break;
       const int step = p.step0 + s;
       // H half step (run_cpu form; the stale outer ring reads zeros)
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = s; r < R - s - 1; ++r) {
         const float eR = from_next_lane(ez[r]);
         const float eD = (r + 1 < R) ? ez[r + 1] : 0.f;
         hx[r] = hx[r] - ch * (eR - ez[r]);
@@ -109,7 +136,7 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
       }
       // E step (+ Debye ADE)
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = s + 1; r < R - s - 1; ++r) {
         const float hxl = from_prev_lane(hx[r]);
         const float hyu = (r > 0) ? hy[r - 1] : 0.f;
         const float curl = (hy[r] - hyu) - (hx[r] - hxl);
@@ -130,12 +157,10 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
           if (r == s_row && lane == s_lane) ez[r] += w;
       }
       if (p.e_save) {   // E after this step on the output window (adjoint store)
-        float* dst = p.e_save + (long long)s * p.slot + base;
+        const __amdgpu_buffer_rsrc_t r_es =
+            plane_rsrc(p.e_save + (long long)s * p.slot + base, plane_bytes);
 #pragma unroll
-        for (int r = K; r < R - K; ++r) {
-          const int a = a0 + r;
-          if (out_lane && a < p.Mx) dst[(long long)a * p.pitch + c] = ez[r];
-        }
+        for (int r = K; r < R - K; ++r) bstore(ez[r], r_es, out_lane ? row_off(r) : kOffNone);
       }
       // probes inside this wavefront's output window
       for (unsigned long long m = pmask; m;) {
@@ -171,15 +196,21 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
       }
     }
     // write back the exact inner window
+    {
+      const __amdgpu_buffer_rsrc_t r_ez = plane_rsrc(p.ez_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hx = plane_rsrc(p.hx_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hy = plane_rsrc(p.hy_out + base, plane_bytes);
 #pragma unroll
-    for (int r = K; r < R - K; ++r) {
-      const int a = a0 + r;
-      if (out_lane && a < p.Mx) {
-        const long long o = base + (long long)a * p.pitch + c;
-        p.ez_out[o] = ez[r];
-        p.hx_out[o] = hx[r];
-        p.hy_out[o] = hy[r];
-        if (POLES & 1) p.jd_out[o] = jd[r];
+      for (int r = K; r < R - K; ++r) {
+        const unsigned o = out_lane ? row_off(r) : kOffNone;
+        bstore(ez[r], r_ez, o);
+        bstore(hx[r], r_hx, o);
+        bstore(hy[r], r_hy, o);
+      }
+      if (POLES & 1) {
+        const __amdgpu_buffer_rsrc_t r_jd = plane_rsrc(p.jd_out + base, plane_bytes);
+#pragma unroll
+        for (int r = K; r < R - K; ++r) bstore(jd[r], r_jd, out_lane ? row_off(r) : kOffNone);
       }
     }
   }
@@ -194,7 +225,7 @@ template <int R, int K>
 __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
   constexpr int NO = R - 2 * K;                        // output rows
   const int lane = threadIdx.x & 63;
-  const int task = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int task = xcd_block(p.xcd_order) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (task >= p.ntasks) return;
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = task % tiles;
@@ -379,11 +410,18 @@ __global__ __launch_bounds__(256) void adj2d_tb_kernel(AdjTB2DArgs p) {
 // for E, right/bottom for H), so the exact inner window is the same.  Every
 // segment restarts from an exact checkpoint, which bounds the rounding drift
 // of the reconstruction to one segment (the (1/ca)^n growth in lossy cells).
+//
+// Memory access is through buffer resources: one 32-bit byte offset per
+// window row (kOffNone off the box), so the 6 x R loads and the output
+// stores carry no per-cell branches and no 64-bit address arithmetic.  As in
+// the forward task only rows that reach the output window are updated: at
+// reversed step k (0-based) E is rebuilt on rows [k+1, R-k), H on
+// [k+1, R-k-1), q is formed on [k, R-k), mu on [k, R-k-1), lam on [k+1, R-k-1).
 template <int R, int K>
 __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
   constexpr int NO = R - 2 * K;
   const int lane = threadIdx.x & 63;
-  const int task = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int task = xcd_block(p.xcd_order) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (task >= p.ntasks) return;
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = task % tiles;
@@ -395,21 +433,31 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
   const int c = cb0 + lane;
   const bool col_ok = c >= 0 && c < p.My;
   const bool out_lane = lane >= K && lane < 64 - K && col_ok;
+  // byte offset of this lane's cell in window row r (a0 may be negative: the
+  // unsigned sum wraps back to the true offset on the rows inside the box)
+  const unsigned pitch4 = (unsigned)p.pitch * 4u;
+  const unsigned lane_off = col_ok ? (unsigned)(a0 * p.pitch + c) * 4u : kOffNone;
+  const unsigned plane_bytes = (unsigned)p.Mx * pitch4;
+  auto row_off = [&](int r) -> unsigned {
+    const int a = a0 + r;
+    return (a >= 0 && a < p.Mx) ? lane_off + (unsigned)r * pitch4 : kOffNone;
+  };
 
   float mca[R], mica[R], mcbi[R];
+  {
+    const __amdgpu_buffer_rsrc_t rca = plane_rsrc(p.ca, plane_bytes);
+    const __amdgpu_buffer_rsrc_t rcb = plane_rsrc(p.cb, plane_bytes);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int a = a0 + r;
-    const bool in = col_ok && a >= 0 && a < p.Mx;
-    const long long cell = (long long)a * p.pitch + c;
-    mca[r] = in ? p.ca[cell] : 0.f;
-    mcbi[r] = in ? p.cb[cell] * p.idx : 0.f;
-    mica[r] = mca[r] != 0.f ? 1.f / mca[r] : 0.f;
+    for (int r = 0; r < R; ++r) {
+      mca[r] = bload(rca, row_off(r));
+      mcbi[r] = bload(rcb, row_off(r)) * p.idx;
+      mica[r] = mca[r] != 0.f ? 1.f / mca[r] : 0.f;
+    }
   }
   float gca[NO], gcb[NO];
 #pragma unroll
   for (int r = 0; r < NO; ++r) { gca[r] = 0.f; gcb[r] = 0.f; }
-  const float ch = p.ch, idx = p.idx;
+  const float ch = p.ch;
   const bool few_probes = p.P <= 64;
   int pr_h = -1, pl_h = -1;
   unsigned long long pmask = 0;
@@ -427,21 +475,29 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
   for (int bt = b_begin; bt < b_end; ++bt) {
     const long long base = (long long)bt * p.bstride;
     float le[R], mx[R], my[R], fe[R], fx[R], fy[R];
+    {
+      const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_my = plane_rsrc(p.my_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fe = plane_rsrc(p.fe_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fx = plane_rsrc(p.fhx_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fy = plane_rsrc(p.fhy_in + base, plane_bytes);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int a = a0 + r;
-      const bool in = col_ok && a >= 0 && a < p.Mx;
-      const long long o = base + (long long)a * p.pitch + c;
-      le[r] = in ? p.le_in[o] : 0.f;
-      mx[r] = in ? p.mx_in[o] : 0.f;
-      my[r] = in ? p.my_in[o] : 0.f;
-      fe[r] = in ? p.fe_in[o] : 0.f;
-      fx[r] = in ? p.fhx_in[o] : 0.f;
-      fy[r] = in ? p.fhy_in[o] : 0.f;
+      for (int r = 0; r < R; ++r) {
+        const unsigned o = row_off(r);
+        le[r] = bload(r_le, o);
+        mx[r] = bload(r_mx, o);
+        my[r] = bload(r_my, o);
+        fe[r] = bload(r_fe, o);
+        fx[r] = bload(r_fx, o);
+        fy[r] = bload(r_fy, o);
+      }
     }
     const int sc = p.src_cell[bt];
     const int s_row = sc / p.pitch - a0;
     const int s_lane = sc % p.pitch - cb0;
+    const bool src_rows = s_row >= 0 && s_row < R;   // uniform
+    const bool src_lane = lane == s_lane;
     float wsrc[K], gq[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -456,16 +512,21 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
     for (int k = 0; k < K; ++k) {
       if (k >= p.ksteps) break;
       const int step = p.step_hi - 1 - k;
+      // E^{s+1} minus the source added after the forward E step
+      if (src_rows) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r == s_row) fe[r] -= src_lane ? wsrc[k] : 0.f;
+      }
       // forward E one step back (in place: row r's E is only read by row r),
       // and the sensitivities (lam here is lam^{step+1})
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = k + 1; r < R - k; ++r) {
         const float hxl = from_prev_lane(fx[r]);
         const float hyu = (r > 0) ? fy[r - 1] : 0.f;
         const float curl = (fy[r] - hyu) - (fx[r] - hxl);
         const float dterm = mcbi[r] * curl;
-        const float inj = (r == s_row && lane == s_lane) ? wsrc[k] : 0.f;
-        const float es = ((fe[r] - inj) - dterm) * mica[r];
+        const float es = (fe[r] - dterm) * mica[r];
         if (r >= K && r < R - K) {
           gca[r - K] += le[r] * es;
           gcb[r - K] += le[r] * dterm;
@@ -474,7 +535,7 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
       }
       // forward H one step back
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = k + 1; r < R - k - 1; ++r) {
         const float eR = from_next_lane(fe[r]);
         const float eD = (r + 1 < R) ? fe[r + 1] : 0.f;
         fx[r] = fx[r] + ch * (eR - fe[r]);
@@ -483,16 +544,16 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
       // adjoint H half step and E step (as adj2d_tb_kernel)
       float q[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) q[r] = mcbi[r] * le[r];
+      for (int r = k; r < R - k; ++r) q[r] = mcbi[r] * le[r];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = k; r < R - k - 1; ++r) {
         const float qR = from_next_lane(q[r]);
         const float qD = (r + 1 < R) ? q[r + 1] : 0.f;
         mx[r] = mx[r] + (qR - q[r]);
         my[r] = my[r] + (q[r] - qD);
       }
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = k + 1; r < R - k - 1; ++r) {
         const float mxL = from_prev_lane(mx[r]);
         const float myU = (r > 0) ? my[r - 1] : 0.f;
         le[r] = mca[r] * le[r] + ch * ((myU - my[r]) - (mxL - mx[r]));
@@ -531,30 +592,34 @@ __global__ __launch_bounds__(256) void adj2d_rev_tb_kernel(AdjTB2DArgs p) {
         }
       }
     }
+    {
+      const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_my = plane_rsrc(p.my_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fe = plane_rsrc(p.fe_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fx = plane_rsrc(p.fhx_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fy = plane_rsrc(p.fhy_out + base, plane_bytes);
 #pragma unroll
-    for (int r = K; r < R - K; ++r) {
-      const int a = a0 + r;
-      if (out_lane && a < p.Mx) {
-        const long long o = base + (long long)a * p.pitch + c;
-        p.le_out[o] = le[r];
-        p.mx_out[o] = mx[r];
-        p.my_out[o] = my[r];
-        p.fe_out[o] = fe[r];
-        p.fhx_out[o] = fx[r];
-        p.fhy_out[o] = fy[r];
+      for (int r = K; r < R - K; ++r) {
+        const unsigned o = out_lane ? row_off(r) : kOffNone;
+        bstore(le[r], r_le, o);
+        bstore(mx[r], r_mx, o);
+        bstore(my[r], r_my, o);
+        bstore(fe[r], r_fe, o);
+        bstore(fx[r], r_fx, o);
+        bstore(fy[r], r_fy, o);
       }
     }
   }
-  float* gca_p = p.g_ca + (long long)chunk * p.gstride;
-  float* gcb_p = p.g_cbi + (long long)chunk * p.gstride;
+  // this wavefront is the only writer of its output cells in its chunk's
+  // partial-sum plane during this launch: plain read-modify-write
+  const __amdgpu_buffer_rsrc_t rgca = plane_rsrc(p.g_ca + (long long)chunk * p.gstride, plane_bytes);
+  const __amdgpu_buffer_rsrc_t rgcb = plane_rsrc(p.g_cbi + (long long)chunk * p.gstride, plane_bytes);
 #pragma unroll
   for (int r = 0; r < NO; ++r) {
-    const int a = a0 + K + r;
-    if (out_lane && a < p.Mx && mcbi[K + r] != 0.f) {
-      const long long cell = (long long)a * p.pitch + c;
-      gca_p[cell] += gca[r];
-      gcb_p[cell] += gcb[r];
-    }
+    const unsigned o = (out_lane && mcbi[K + r] != 0.f) ? row_off(K + r) : kOffNone;
+    bstore(bload(rgca, o) + gca[r], rgca, o);
+    bstore(bload(rgcb, o) + gcb[r], rgcb, o);
   }
 }
 
@@ -563,7 +628,7 @@ __global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
   const int lane = threadIdx.x & 63;
   // readfirstlane: the wavefront index is uniform, and saying so keeps the
   // task decomposition and every derived address in scalar registers
-  const int task = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int task = xcd_block(p.xcd_order) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (task >= p.ntasks) return;                       // uniform per wavefront
   tb_task<R, K, POLES>(p, task, lane);
 }

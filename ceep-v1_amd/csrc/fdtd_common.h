@@ -25,4 +25,38 @@ struct PoleMaps {
   const float* k3;     // Lorentz only (J_prev coefficient); null for Debye
 };
 
+// Raw buffer addressing (CDNA buffer instructions).  A plane is described by
+// a 128-bit resource (base, size in bytes; word 3 = 0x00020000, the gfx9
+// family's raw 32-bit format); a lane addresses it with a 32-bit byte offset.
+// The hardware range check turns any offset >= the size into a load of 0
+// and a dropped store, so out-of-window cells need neither a branch nor a
+// select: their offset is pushed to kOffNone or beyond (planes stay below
+// 2 GiB, so kOffNone plus a row offset does not wrap).
+constexpr unsigned kOffNone = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* base,
+                                                             unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+}
+
+// Workgroups of a 1D grid are dealt round-robin over MI355X's 8 XCDs (each
+// with its own L2).  With `on`, physical block b is renumbered so that XCD x
+// runs one contiguous range of logical blocks: neighbouring tiles, which
+// re-read each other's halo rows, then share an L2.  A bijection for any
+// block count.
+__device__ __forceinline__ int xcd_block(int on) {
+  const int phys = blockIdx.x;
+  if (!on) return phys;
+  const int n = gridDim.x;
+  const int q = n / 8, r = n % 8, xcd = phys % 8;
+  return xcd * q + min(xcd, r) + phys / 8;
+}
+
 }  // namespace ceep

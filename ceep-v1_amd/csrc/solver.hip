@@ -136,6 +136,9 @@ struct ceep_fdtd {
   int tb_bpw = 1;              // 2D batch planes per wavefront
   int fused3d = 0;             // 3D: 1/2 = one fused H+E launch per step, 3 = marching form (ping-pong fields)
   int march_len = 24;          // 3D marching form: x-slabs per workgroup
+  int xcd_order = 1;           // 2D blocked kernels: contiguous tile ranges per XCD
+  int adj_r = 16;              // 2D blocked adjoint: rows per wavefront window
+  int adj_bpw = 8;             // 2D blocked adjoint: batch planes per wavefront
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
@@ -371,6 +374,7 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
   const int poles = h->poles();
   if (h->use_tb()) {
     ceep::TB2DArgs a{};
+    a.xcd_order = h->xcd_order;
     a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
     a.ca = h->ca; a.cb = h->cb; a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
     a.ch = h->ch; a.idx = h->idx;
@@ -1312,7 +1316,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     for (int f = 0; f < 3; ++f)
       if (!h->adj[i][f] && (rc = dalloc(&h->adj[i][f], nbox))) return rc;
   // one partial-sum plane per TX chunk of the blocked kernel (at most B)
-  const int nchunks = h->use_tb() ? (B + h->tb_bpw - 1) / h->tb_bpw : 1;
+  const int nchunks = h->use_tb() ? (B + h->adj_bpw - 1) / h->adj_bpw : 1;
   if (!h->g_ca && (rc = dalloc(&h->g_ca, (size_t)B * h->plane))) return rc;
   if (!h->g_cbi && (rc = dalloc(&h->g_cbi, (size_t)B * h->plane))) return rc;
   if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
@@ -1338,8 +1342,9 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     const int s0 = k * every, n = std::min(every, T - s0);
     for (int f = 0; f < 3; ++f)
       CEEP_HIP(ceep::launch_copy(h->ck[(size_t)(k + 1) * 3 + f], h->f[0][f], (long long)nbox, st));
-    const int K = h->tb_k, R = h->tb_r;
+    const int K = h->tb_k, R = h->adj_r;
     ceep::AdjTB2DArgs t{};
+    t.xcd_order = h->xcd_order;
     t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
     t.ca = h->ca; t.cb = h->cb; t.ch = h->ch; t.idx = h->idx;
     t.src_cell = h->src2; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
@@ -1347,7 +1352,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     t.g_ca = h->g_ca; t.g_cbi = h->g_cbi; t.gstride = h->plane;
     t.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
     t.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
-    t.bpw = h->tb_bpw;
+    t.bpw = h->adj_bpw;
     t.ntasks = t.tiles_x * t.tiles_y * ((B + t.bpw - 1) / t.bpw);
     int fcur = 0;
     for (int hi = s0 + n; hi > s0; hi -= K) {
@@ -1381,8 +1386,9 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       h->estore_seg = k;
     }
     if (h->use_tb()) {
-      const int K = h->tb_k, R = h->tb_r;
+      const int K = h->tb_k, R = h->adj_r;
       ceep::AdjTB2DArgs t{};
+      t.xcd_order = h->xcd_order;
       t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
       t.ca = h->ca; t.cb = h->cb; t.ch = h->ch; t.idx = h->idx;
       t.e_store = h->estore; t.slot = (long long)nbox; t.seg0 = s0;
@@ -1391,7 +1397,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       t.g_ca = h->g_ca; t.g_cbi = h->g_cbi; t.gstride = h->plane;
       t.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
       t.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
-      t.bpw = h->tb_bpw;
+      t.bpw = h->adj_bpw;
       t.ntasks = t.tiles_x * t.tiles_y * ((B + t.bpw - 1) / t.bpw);
       for (int hi = s0 + n; hi > s0; hi -= K) {
         const int o = acur ^ 1;
@@ -1466,6 +1472,16 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 4) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
     h->tb_bpw = value;
+  } else if (key == 10) {
+    if (value != 16 && value != 24 && value != 32)
+      return fail(CEEP_EINVAL, "adjoint rows per wavefront must be 16, 24 or 32");
+    h->adj_r = value;
+  } else if (key == 11) {
+    if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "adjoint batch planes per wavefront must be >= 1");
+    h->adj_bpw = value;
+  } else if (key == 9) {
+    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "XCD tile order must be 0 or 1");
+    h->xcd_order = value;
   } else if (key == 8) {
     if (value < 1 || value > 65536) return fail(CEEP_EINVAL, "x-slabs per marching workgroup must be >= 1");
     h->march_len = value;

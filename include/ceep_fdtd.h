@@ -164,7 +164,14 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          rebuilt backward inside the adjoint launches from the checkpoint at
  *          each segment's end; no recomputation, no E store; default),
  *          0 = recompute each segment into an E store.
- *   key 8: x-slabs per workgroup of the marching 3D form (default 24). */
+ *   key 8: x-slabs per workgroup of the marching 3D form (default 24).
+ *   key 9: 2D blocked kernels: 1 = renumber workgroups so each XCD runs a
+ *          contiguous range of tiles (neighbouring tiles share an L2; default),
+ *          0 = dispatch order.  Results are identical.
+ *   key 10, 11: rows per wavefront window (16, 24, 32; default 16) and batch
+ *          planes per wavefront (default 8) of the 2D blocked adjoint, which
+ *          holds twice the forward's state per cell and prefers other shapes
+ *          than the forward (keys 3, 4). */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,

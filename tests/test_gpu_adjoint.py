@@ -99,11 +99,16 @@ def test_adjoint_fp64_forward_matches_autograd(amd, reference, every):
     {"temporal_depth": 1, "rows_per_wave": 16, "batch_per_wave": 2},
     {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4, "adjoint_reversible": 0},
     {"temporal_depth": 3, "rows_per_wave": 24, "batch_per_wave": 3, "adjoint_reversible": 0},
+    {"temporal_depth": 4, "rows_per_wave": 24, "batch_per_wave": 2,
+     "adjoint_rows_per_wave": 24, "adjoint_batch_per_wave": 3, "xcd_order": 0},
+    {"temporal_depth": 3, "rows_per_wave": 16, "batch_per_wave": 5,
+     "adjoint_rows_per_wave": 32, "adjoint_batch_per_wave": 1, "xcd_order": 1},
 ])
 def test_adjoint_kernel_variants(amd, reference, tuning):
     """One-step and blocked kernels, reversible (default) and recompute + E store
     adjoint forms, segment lengths that are not multiples of K, batch not a
-    multiple of the planes per wavefront."""
+    multiple of the planes per wavefront, adjoint launch shapes other than the
+    forward's, both workgroup orders."""
     from ceep_amd import adjoint
     r = reference
     s = _solver(amd, r["dt"], r["w"], r["pos"])

@@ -141,12 +141,13 @@ def test_2d_tuning_variants_bitwise(amd):
             if bb == 1 and graph == 0:
                 one_step = s.probe_data.copy()
             assert np.array_equal(s.probe_data, one_step), (bb, graph)
-    for graph in (0, 1):          # blocked kernel: graph on/off
-        s.tuning = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4,
-                    "graph": graph}
-        s._build()
-        s.run()
-        assert np.array_equal(s.probe_data, base), graph
+    for graph in (0, 1):          # blocked kernel: graph on/off, both workgroup orders
+        for xcd in (0, 1):
+            s.tuning = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4,
+                        "graph": graph, "xcd_order": xcd}
+            s._build()
+            s.run()
+            assert np.array_equal(s.probe_data, base), (graph, xcd)
     assert rel_l2(base, one_step) < 1e-6
     assert ref[0][0] is not None
 

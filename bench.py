@@ -378,12 +378,15 @@ def _bench_3d_gradient(T3=1000, every=100):
                          "dL/deps_r and dL/dsigma over the full volume")}
 
 
-def _traffic():
+def _traffic(kernel):
+    """Measured HBM bytes per launch from the committed rocprofv3 counter
+    summary, if it was taken for the kernel this run uses (else None)."""
     path = os.path.join(ROOT, "profiles", "step2d_debye_600x600x16_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        d = json.load(f)
+    return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
 
 
 def main():
@@ -433,11 +436,19 @@ def main():
         from ceep_amd.autotune import autotune_2d
         tuning = autotune_2d(s, reps=2)
         _progress(f"autotuned 2D launch: {tuning}")
+    else:
+        from ceep_amd.autotune import tuning_for
+        tuning = dict(s.tuning) if s.tuning is not None else tuning_for(
+            2, (s.nx, s.ny), s.cpml_n, s.batch, True, False)
     h = s._handle
     box = h.box_info()
     cells = int(box[0] * box[1])
     K = (tuning or {}).get("temporal_depth", 0)
-    if K:
+    if K and (tuning or {}).get("forward_waves", 1) == 4:
+        kernel = f"step2d_mw_kernel<{tuning['rows_per_wave'] // 4}, {K}, 1>"
+        launches = -(-T // K)
+        steps_per_launch = K
+    elif K:
         kernel = f"step2d_tb_kernel<{tuning['rows_per_wave']}, {K}, 1>"
         launches = -(-T // K)
         steps_per_launch = K
@@ -521,7 +532,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _traffic(),
+            "traffic": _traffic(kernel),
             "algorithmic_bytes_per_launch": bytes_launch,
             "launch_ms": round(ms_launch, 5),
         },

@@ -135,12 +135,19 @@ TUNE_MARCH_SLABS = 8
 TUNE_XCD_ORDER = 9
 TUNE_ADJ_ROWS = 10
 TUNE_ADJ_BPW = 11
+TUNE_XCD_3D = 12
+TUNE_ADJ_WAVES = 14
+TUNE_ADJ_DEPTH = 15
+TUNE_FWD_WAVES = 16
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
                "fused_3d": TUNE_FUSED_3D, "adjoint_reversible": TUNE_ADJOINT_REVERSIBLE,
                "march_slabs": TUNE_MARCH_SLABS, "xcd_order": TUNE_XCD_ORDER,
-               "adjoint_rows_per_wave": TUNE_ADJ_ROWS, "adjoint_batch_per_wave": TUNE_ADJ_BPW}
+               "adjoint_rows_per_wave": TUNE_ADJ_ROWS, "adjoint_batch_per_wave": TUNE_ADJ_BPW,
+               "xcd_order_3d": TUNE_XCD_3D,
+               "adjoint_waves": TUNE_ADJ_WAVES,
+               "adjoint_temporal_depth": TUNE_ADJ_DEPTH, "forward_waves": TUNE_FWD_WAVES}
 
 
 class Handle:

@@ -21,8 +21,8 @@ _LOADED = False
 
 # measured best on MI355X for the 600^2 x 16 Debye workload; used when nothing
 # was tuned for a shape
-DEFAULT_2D = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 2,
-              "batch_per_thread": 4, "graph": 1, "xcd_order": 1}
+DEFAULT_2D = {"temporal_depth": 4, "rows_per_wave": 32, "batch_per_wave": 2,
+              "batch_per_thread": 4, "graph": 1, "xcd_order": 1, "forward_waves": 4}
 
 
 def _key(ndim, shape, n, batch, debye, lorentz):
@@ -57,7 +57,13 @@ def candidates_2d(lorentz=False):
     if not lorentz:
         for k, r, bpw in itertools.product((1, 2, 3, 4), (16, 24, 32), (1, 2, 4)):
             if r - 2 * k >= 4:
-                out.append({"temporal_depth": k, "rows_per_wave": r, "batch_per_wave": bpw})
+                out.append({"forward_waves": 1, "temporal_depth": k, "rows_per_wave": r,
+                            "batch_per_wave": bpw})
+        # four wavefronts per window (taller windows, deeper blocking)
+        for k, r, bpw in ((4, 32, 1), (4, 32, 2), (4, 32, 4), (6, 32, 2), (4, 48, 2), (6, 48, 2),
+                          (8, 64, 2)):
+            out.append({"forward_waves": 4, "temporal_depth": k, "rows_per_wave": r,
+                        "batch_per_wave": bpw})
     return out
 
 

@@ -172,6 +172,10 @@ struct TB2DArgs {
 
 // poles: 0 or 1 (Debye); R in {16, 24, 32}; K in {1, 2, 3, 4}.
 hipError_t launch_step2d_tb(const TB2DArgs& a, int poles, int R, int K, hipStream_t s);
+// The same update with an R x 64 window split over the four wavefronts of a
+// workgroup (R / 4 rows each, halo rows through LDS); one workgroup per task.
+// R in {32, 48, 64}, K in {4, 6, 8}, poles 0 or 1.
+hipError_t launch_step2d_mw(const TB2DArgs& a, int poles, int R, int K, hipStream_t s);
 
 // Temporally blocked adjoint: K adjoint steps (step_hi-1 down to step_hi-ksteps)
 // per launch on R x 64 register windows, the transpose of step2d_tb_kernel's
@@ -218,6 +222,10 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
 // H^{s-1/2} = H^{s+1/2} -+ ch dE^s) inside the same windows as the adjoint,
 // so no segment is recomputed and no E store is read.
 hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
+// The same reversed steps with an R x 64 window split over the four
+// wavefronts of a workgroup (R / 4 rows each; halo rows through LDS); one
+// workgroup per task, R in {16, 24, 32}.
+hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
 
 // One problem of an uneven batch: its blocked-kernel arguments (buffers set
 // per round from the ping-pong pairs) and the number of K-step rounds.

@@ -861,11 +861,497 @@ hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s
   return hipErrorInvalidValue;
 }
 
+// Four-wavefront form of adj2d_rev_tb_kernel: one workgroup owns an
+// R x 64 window (R = 4 RW) and each of its wavefronts RW consecutive rows,
+// so a wavefront holds a quarter of the state of the one-wavefront form.
+// Each reversed step needs one row across every wavefront boundary four
+// times -- the old hy above (forward E), the new E below (forward H), q
+// below (adjoint H) and the new mu_y above (lam) -- and passes them through
+// LDS between three workgroup barriers.  The arithmetic per cell is that of
+// adj2d_rev_tb_kernel; with the same R and K the results are bit-identical.
+// The smaller per-wavefront footprint (about 100 VGPRs at RW = 8) lets four
+// or more wavefronts share a SIMD and makes R = 32 windows affordable, whose
+// (R - 2K) / R output fraction is 75 % at K = 4.
+template <int RW, int K>
+__global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
+  constexpr int R = 4 * RW;
+  constexpr int NO = R - 2 * K;
+  __shared__ float x_fy[4][64], x_es[4][64], x_q[4][64], x_my[4][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int task = xcd_block(p.xcd_order);
+  if (task >= p.ntasks) return;                       // uniform per workgroup
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = task % tiles;
+  const int chunk = task / tiles;
+  const int ty = tile % p.tiles_y;
+  const int tx = tile / p.tiles_y;
+  const int a0 = tx * NO - K;                          // row of window row 0
+  const int cb0 = ty * (64 - 2 * K) - K;
+  const int c = cb0 + lane;
+  const bool col_ok = c >= 0 && c < p.My;
+  const bool out_lane = lane >= K && lane < 64 - K && col_ok;
+  const int r0 = w * RW;                               // this wavefront's first window row
+  const unsigned pitch4 = (unsigned)p.pitch * 4u;
+  const unsigned lane_off = col_ok ? (unsigned)((a0 + r0) * p.pitch + c) * 4u : kOffNone;
+  const unsigned plane_bytes = (unsigned)p.Mx * pitch4;
+  auto row_off = [&](int i) -> unsigned {               // local row i of this wavefront
+    const int a = a0 + r0 + i;
+    return (a >= 0 && a < p.Mx) ? lane_off + (unsigned)i * pitch4 : kOffNone;
+  };
+  auto is_out = [&](int i) -> bool {                    // window row inside [K, R - K)
+    return r0 + i >= K && r0 + i < R - K;
+  };
+  const bool has_up = w > 0, has_down = w < 3;
+
+  float mca[RW], mica[RW], mcbi[RW];
+  {
+    const __amdgpu_buffer_rsrc_t rca = plane_rsrc(p.ca, plane_bytes);
+    const __amdgpu_buffer_rsrc_t rcb = plane_rsrc(p.cb, plane_bytes);
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+      mca[i] = bload(rca, row_off(i));
+      mcbi[i] = bload(rcb, row_off(i)) * p.idx;
+      mica[i] = mca[i] != 0.f ? 1.f / mca[i] : 0.f;
+    }
+  }
+  float gca[RW], gcb[RW];
+#pragma unroll
+  for (int i = 0; i < RW; ++i) { gca[i] = 0.f; gcb[i] = 0.f; }
+  const float ch = p.ch;
+  const bool few_probes = p.P <= 64;
+  int pr_h = -1, pl_h = -1;
+  unsigned long long pmask = 0;
+  if (few_probes) {
+    const int pc = lane < p.P ? p.probe_cell[lane] : -1;
+    if (pc >= 0) {
+      pr_h = pc / p.pitch - (a0 + r0);                 // local row in this wavefront
+      pl_h = pc % p.pitch - cb0;
+    }
+    pmask = __ballot(pc >= 0 && pr_h >= 0 && pr_h < RW && pl_h >= 0 && pl_h < 64);
+  }
+
+  const int b_begin = chunk * p.bpw;
+  const int b_end = min(b_begin + p.bpw, p.B);
+  for (int bt = b_begin; bt < b_end; ++bt) {
+    const long long base = (long long)bt * p.bstride;
+    float le[RW], mx[RW], my[RW], fe[RW], fx[RW], fy[RW];
+    {
+      const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_my = plane_rsrc(p.my_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fe = plane_rsrc(p.fe_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fx = plane_rsrc(p.fhx_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fy = plane_rsrc(p.fhy_in + base, plane_bytes);
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const unsigned o = row_off(i);
+        le[i] = bload(r_le, o);
+        mx[i] = bload(r_mx, o);
+        my[i] = bload(r_my, o);
+        fe[i] = bload(r_fe, o);
+        fx[i] = bload(r_fx, o);
+        fy[i] = bload(r_fy, o);
+      }
+    }
+    const int sc = p.src_cell[bt];
+    const int s_row = sc / p.pitch - (a0 + r0);          // local row
+    const int s_lane = sc % p.pitch - cb0;
+    const bool src_rows = s_row >= 0 && s_row < RW;      // uniform
+    const bool src_lane = lane == s_lane;
+    float wsrc[K], gq[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int step = p.step_hi - 1 - k;
+      const bool live = k < p.ksteps;
+      wsrc[k] = live ? p.wave[(long long)bt * p.wave_stride_b + step] : 0.f;
+      gq[k] = (live && step >= 1 && few_probes && ((pmask >> lane) & 1ull))
+                  ? p.dtr[((long long)bt * p.P + lane) * p.T + (step - 1)] : 0.f;
+    }
+
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (k >= p.ksteps) break;                        // uniform per workgroup
+      const int step = p.step_hi - 1 - k;
+      if (src_rows) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+          if (i == s_row) fe[i] -= src_lane ? wsrc[k] : 0.f;
+      }
+      // forward E one step back: the row above this wavefront's first row
+      // comes from the wavefront above (its current hy)
+      x_fy[w][lane] = fy[RW - 1];
+      __syncthreads();
+      const float fy_up = has_up ? x_fy[w - 1][lane] : 0.f;
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const float hxl = from_prev_lane(fx[i]);
+        const float hyu = (i > 0) ? fy[i - 1] : fy_up;
+        const float curl = (fy[i] - hyu) - (fx[i] - hxl);
+        const float dterm = mcbi[i] * curl;
+        const float es = (fe[i] - dterm) * mica[i];
+        gca[i] += le[i] * es;
+        gcb[i] += le[i] * dterm;
+        fe[i] = es;
+      }
+      // forward H one step back and the adjoint H step: the rows below come
+      // from the wavefront below (its new E and its q)
+      x_es[w][lane] = fe[0];
+      x_q[w][lane] = mcbi[0] * le[0];
+      __syncthreads();
+      const float es_dn = has_down ? x_es[w + 1][lane] : 0.f;
+      const float q_dn = has_down ? x_q[w + 1][lane] : 0.f;
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const float eR = from_next_lane(fe[i]);
+        const float eD = (i + 1 < RW) ? fe[i + 1] : es_dn;
+        fx[i] = fx[i] + ch * (eR - fe[i]);
+        fy[i] = fy[i] - ch * (eD - fe[i]);
+      }
+      float q[RW];
+#pragma unroll
+      for (int i = 0; i < RW; ++i) q[i] = mcbi[i] * le[i];
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const float qR = from_next_lane(q[i]);
+        const float qD = (i + 1 < RW) ? q[i + 1] : q_dn;
+        mx[i] = mx[i] + (qR - q[i]);
+        my[i] = my[i] + (q[i] - qD);
+      }
+      // adjoint E step: the new mu_y above comes from the wavefront above
+      x_my[w][lane] = my[RW - 1];
+      __syncthreads();
+      const float my_up = has_up ? x_my[w - 1][lane] : 0.f;
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const float mxL = from_prev_lane(mx[i]);
+        const float myU = (i > 0) ? my[i - 1] : my_up;
+        le[i] = mca[i] * le[i] + ch * ((myU - my[i]) - (mxL - mx[i]));
+      }
+      for (unsigned long long m = pmask; m;) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int qr = __shfl(pr_h, l);
+        const int ql = __shfl(pl_h, l);
+        const float g = __shfl(gq[k], l);
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+          if (i == qr && lane == ql) le[i] += g;
+      }
+      if (!few_probes && step >= 1) {
+        for (int q0 = 0; q0 < p.P; q0 += 64) {
+          const int qi = q0 + lane;
+          const int pc = qi < p.P ? p.probe_cell[qi] : -1;
+          int pr = -1, pl = -1;
+          if (pc >= 0) {
+            pr = pc / p.pitch - (a0 + r0);
+            pl = pc % p.pitch - cb0;
+          }
+          const bool here = pc >= 0 && pr >= 0 && pr < RW && pl >= 0 && pl < 64;
+          unsigned long long m = __ballot(here);
+          while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const int qr = __shfl(pr, l);
+            const int ql = __shfl(pl, l);
+            const float g = p.dtr[((long long)bt * p.P + q0 + l) * p.T + (step - 1)];
+#pragma unroll
+            for (int i = 0; i < RW; ++i)
+              if (i == qr && lane == ql) le[i] += g;
+          }
+        }
+      }
+    }
+    {
+      const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_my = plane_rsrc(p.my_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fe = plane_rsrc(p.fe_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fx = plane_rsrc(p.fhx_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_fy = plane_rsrc(p.fhy_out + base, plane_bytes);
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const unsigned o = (out_lane && is_out(i)) ? row_off(i) : kOffNone;
+        bstore(le[i], r_le, o);
+        bstore(mx[i], r_mx, o);
+        bstore(my[i], r_my, o);
+        bstore(fe[i], r_fe, o);
+        bstore(fx[i], r_fx, o);
+        bstore(fy[i], r_fy, o);
+      }
+    }
+  }
+  // the workgroup is the only writer of its output cells in its chunk's
+  // partial-sum plane during this launch: plain read-modify-write
+  const __amdgpu_buffer_rsrc_t rgca = plane_rsrc(p.g_ca + (long long)chunk * p.gstride, plane_bytes);
+  const __amdgpu_buffer_rsrc_t rgcb = plane_rsrc(p.g_cbi + (long long)chunk * p.gstride, plane_bytes);
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    const unsigned o = (out_lane && is_out(i) && mcbi[i] != 0.f) ? row_off(i) : kOffNone;
+    bstore(bload(rgca, o) + gca[i], rgca, o);
+    bstore(bload(rgcb, o) + gcb[i], rgcb, o);
+  }
+}
+
+template <int RW>
+static hipError_t launch_rev_mw_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((unsigned)a.ntasks, 1, 1);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 1>), grd, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 2>), grd, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 3>), grd, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 4>), grd, blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
+  if (a.ntasks < 1) return hipSuccess;
+  switch (R) {
+    case 16: return launch_rev_mw_r<4>(a, K, s);
+    case 24: return launch_rev_mw_r<6>(a, K, s);
+    case 32: return launch_rev_mw_r<8>(a, K, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
   switch (R) {
     case 16: return launch_adj_r<16>(a, K, s);
     case 24: return launch_adj_r<24>(a, K, s);
     case 32: return launch_adj_r<32>(a, K, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Four-wavefront form of step2d_tb_kernel.  One workgroup owns an R x 64
+// window (R = 4 RW) and each of its wavefronts RW consecutive rows.  A step
+// needs one row across each wavefront boundary twice -- the old E below (H
+// update) and the new hy above (E update) -- passed through LDS between two
+// workgroup barriers.  With a quarter of the rows per wavefront the window
+// can be tall (R = 48 or 64) and the blocking deep (K up to 8) within the
+// register file: the loads per output cell-step drop from
+// (R/(R-2K) (64/(64-2K)) + 1)/K = 0.82 at R = 16, K = 4 to 0.38 at R = 48,
+// K = 8.  Per-cell arithmetic is that of step2d_tb_kernel.
+template <int RW, int K, int POLES>
+__global__ __launch_bounds__(256) void step2d_mw_kernel(TB2DArgs p) {
+  constexpr int R = 4 * RW;
+  __shared__ float x_e[4][64], x_h[4][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int task = xcd_block(p.xcd_order);
+  if (task >= p.ntasks) return;                       // uniform per workgroup
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = task % tiles;
+  const int chunk = task / tiles;
+  const int ty = tile % p.tiles_y;
+  const int tx = tile / p.tiles_y;
+  const int a0 = tx * (R - 2 * K) - K;                // row of window row 0
+  const int cb0 = ty * (64 - 2 * K) - K;              // column of lane 0
+  const int c = cb0 + lane;
+  const bool col_ok = c >= 0 && c < p.My;
+  const int r0 = w * RW;                              // this wavefront's first window row
+  const unsigned pitch4 = (unsigned)p.pitch * 4u;
+  const unsigned lane_off = col_ok ? (unsigned)((a0 + r0) * p.pitch + c) * 4u : kOffNone;
+  const unsigned plane_bytes = (unsigned)p.Mx * pitch4;
+  auto row_off = [&](int i) -> unsigned {
+    const int a = a0 + r0 + i;
+    return (a >= 0 && a < p.Mx) ? lane_off + (unsigned)i * pitch4 : kOffNone;
+  };
+  auto is_out = [&](int i) -> bool { return r0 + i >= K && r0 + i < R - K; };
+  const bool has_up = w > 0, has_down = w < 3;
+
+  float mca[RW], mcb[RW], mk1[RW], mk2[RW];
+  {
+    const __amdgpu_buffer_rsrc_t rca = plane_rsrc(p.ca, plane_bytes);
+    const __amdgpu_buffer_rsrc_t rcb = plane_rsrc(p.cb, plane_bytes);
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+      mca[i] = bload(rca, row_off(i));
+      mcb[i] = bload(rcb, row_off(i));
+    }
+    if (POLES & 1) {
+      const __amdgpu_buffer_rsrc_t rk1 = plane_rsrc(p.d_k1, plane_bytes);
+      const __amdgpu_buffer_rsrc_t rk2 = plane_rsrc(p.d_k2dt, plane_bytes);
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        mk1[i] = bload(rk1, row_off(i));
+        mk2[i] = bload(rk2, row_off(i));
+      }
+    }
+  }
+  const float ch = p.ch, idx = p.idx;
+  const bool out_lane = lane >= K && lane < 64 - K && col_ok;
+  const bool few_probes = p.P <= 64;
+  int pr_h = -1, pl_h = -1;
+  unsigned long long pmask = 0;
+  if (few_probes) {
+    const int pc = lane < p.P ? p.probe_cell[lane] : -1;
+    int pr_w = -1;
+    if (pc >= 0) {
+      pr_w = pc / p.pitch - a0;                       // window row
+      pr_h = pr_w - r0;                               // this wavefront's local row
+      pl_h = pc % p.pitch - cb0;
+    }
+    pmask = __ballot(pc >= 0 && pr_w >= K && pr_w < R - K && pr_h >= 0 && pr_h < RW &&
+                     pl_h >= K && pl_h < 64 - K);
+  }
+
+  const int b_begin = chunk * p.bpw;
+  const int b_end = min(b_begin + p.bpw, p.B);
+  for (int bt = b_begin; bt < b_end; ++bt) {
+    const long long base = (long long)bt * p.bstride;
+    float ez[RW], hx[RW], hy[RW], jd[RW];
+    {
+      const __amdgpu_buffer_rsrc_t r_ez = plane_rsrc(p.ez_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hx = plane_rsrc(p.hx_in + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hy = plane_rsrc(p.hy_in + base, plane_bytes);
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const unsigned o = row_off(i);
+        ez[i] = bload(r_ez, o);
+        hx[i] = bload(r_hx, o);
+        hy[i] = bload(r_hy, o);
+      }
+      if (POLES & 1) {
+        const __amdgpu_buffer_rsrc_t r_jd = plane_rsrc(p.jd_in + base, plane_bytes);
+#pragma unroll
+        for (int i = 0; i < RW; ++i) jd[i] = bload(r_jd, row_off(i));
+      }
+    }
+    const int sc = p.src_cell[bt];
+    const int s_row = sc / p.pitch - (a0 + r0);        // local row
+    const int s_lane = sc % p.pitch - cb0;
+    const bool src_here = s_row >= 0 && s_row < RW && s_lane >= 0 && s_lane < 64;
+
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      if (s >= p.ksteps) break;                        // uniform per workgroup
+      const int step = p.step0 + s;
+      // H half step: the E row below the last row comes from the wavefront below
+      x_e[w][lane] = ez[0];
+      __syncthreads();
+      const float e_dn = has_down ? x_e[w + 1][lane] : 0.f;
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const float eR = from_next_lane(ez[i]);
+        const float eD = (i + 1 < RW) ? ez[i + 1] : e_dn;
+        hx[i] = hx[i] - ch * (eR - ez[i]);
+        hy[i] = hy[i] + ch * (eD - ez[i]);
+      }
+      // E step (+ Debye ADE): the new hy above the first row comes from the wavefront above
+      x_h[w][lane] = hy[RW - 1];
+      __syncthreads();
+      const float h_up = has_up ? x_h[w - 1][lane] : 0.f;
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const float hxl = from_prev_lane(hx[i]);
+        const float hyu = (i > 0) ? hy[i - 1] : h_up;
+        const float curl = (hy[i] - hyu) - (hx[i] - hxl);
+        const float e0 = ez[i];
+        if (POLES == 0) {
+          ez[i] = mca[i] * e0 + mcb[i] * (curl * idx);
+        } else {
+          const float sgj = 0.5f * (1.f + mk1[i]) * jd[i];
+          const float en = mca[i] * e0 + mcb[i] * (curl * idx) - mcb[i] * sgj;
+          jd[i] = mk1[i] * jd[i] + mk2[i] * (en - e0);
+          ez[i] = en;
+        }
+      }
+      if (src_here) {
+        const float wv = p.wave[(long long)bt * p.wave_stride_b + step];
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+          if (i == s_row && lane == s_lane) ez[i] += wv;
+      }
+      if (p.e_save) {   // E after this step on the output window (adjoint store)
+        const __amdgpu_buffer_rsrc_t r_es =
+            plane_rsrc(p.e_save + (long long)s * p.slot + base, plane_bytes);
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+          bstore(ez[i], r_es, (out_lane && is_out(i)) ? row_off(i) : kOffNone);
+      }
+      for (unsigned long long m = pmask; m;) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int qr = __shfl(pr_h, l);
+        const int ql = __shfl(pl_h, l);
+        float v = 0.f;
+#pragma unroll
+        for (int i = 0; i < RW; ++i) v = (i == qr) ? ez[i] : v;
+        if (lane == ql) p.traces[((long long)bt * p.P + l) * p.T + step] = v;
+      }
+      for (int q0 = 0; !few_probes && q0 < p.P; q0 += 64) {
+        const int q = q0 + lane;
+        const int pc = q < p.P ? p.probe_cell[q] : -1;
+        int pw = -1, pr = -1, pl = -1;
+        if (pc >= 0) {
+          pw = pc / p.pitch - a0;
+          pr = pw - r0;
+          pl = pc % p.pitch - cb0;
+        }
+        const bool mine = pc >= 0 && pw >= K && pw < R - K && pr >= 0 && pr < RW &&
+                          pl >= K && pl < 64 - K;
+        unsigned long long m = __ballot(mine);
+        while (m) {
+          const int l = __ffsll((long long)m) - 1;
+          m &= m - 1;
+          const int qr = __shfl(pr, l);
+          const int ql = __shfl(pl, l);
+          float v = 0.f;
+#pragma unroll
+          for (int i = 0; i < RW; ++i) v = (i == qr) ? ez[i] : v;
+          if (lane == ql) p.traces[((long long)bt * p.P + q0 + l) * p.T + step] = v;
+        }
+      }
+    }
+    {
+      const __amdgpu_buffer_rsrc_t r_ez = plane_rsrc(p.ez_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hx = plane_rsrc(p.hx_out + base, plane_bytes);
+      const __amdgpu_buffer_rsrc_t r_hy = plane_rsrc(p.hy_out + base, plane_bytes);
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const unsigned o = (out_lane && is_out(i)) ? row_off(i) : kOffNone;
+        bstore(ez[i], r_ez, o);
+        bstore(hx[i], r_hx, o);
+        bstore(hy[i], r_hy, o);
+      }
+      if (POLES & 1) {
+        const __amdgpu_buffer_rsrc_t r_jd = plane_rsrc(p.jd_out + base, plane_bytes);
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+          bstore(jd[i], r_jd, (out_lane && is_out(i)) ? row_off(i) : kOffNone);
+      }
+    }
+  }
+}
+
+template <int RW, int K>
+static hipError_t launch_mw_rk(const TB2DArgs& a, int poles, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((unsigned)a.ntasks, 1, 1);
+  if (poles == 0) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0>), grd, blk, 0, s, a);
+  else if (poles == 1) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1>), grd, blk, 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int RW>
+static hipError_t launch_mw_r(const TB2DArgs& a, int poles, int K, hipStream_t s) {
+  switch (K) {
+    case 4: return launch_mw_rk<RW, 4>(a, poles, s);
+    case 6: return launch_mw_rk<RW, 6>(a, poles, s);
+    case 8: return launch_mw_rk<RW, 8>(a, poles, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_step2d_mw(const TB2DArgs& a, int poles, int R, int K, hipStream_t s) {
+  if (a.ntasks < 1) return hipSuccess;
+  switch (R) {
+    case 32: return launch_mw_r<8>(a, poles, K, s);
+    case 48: return launch_mw_r<12>(a, poles, K, s);
+    case 64: return launch_mw_r<16>(a, poles, K, s);
   }
   return hipErrorInvalidValue;
 }

@@ -59,17 +59,14 @@ __global__ __launch_bounds__(256) void adj3d_a_kernel(Adj3DArgs p) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float lam = p.lam[c][o];
-      float G = 0.f, aD = 0.f, aL = 0.f, bL = 0.f;
+      float G = 0.f, aD = 0.f, aL = 0.f, aLp = 0.f;
       if (POLES & 1) { aD = p.ad[c][o]; G += dk2 * aD; }
-      if (POLES & 2) { aL = p.al[c][o]; bL = p.bl[c][o]; G += lk2 * aL; }
+      if (POLES & 2) { aL = p.al[c][o]; aLp = p.al_prev[c][o]; G += lk2 * aL; }
       const float et = lam + G;                 // adjoint of e~ (it feeds e' and e~ - e)
       p.lam[c][o] = ca * et - G;
       p.q[c][o] = cbi * et;
       if (POLES & 1) p.ad[c][o] = dk1 * aD - cb * gam * et;
-      if (POLES & 2) {
-        p.al[c][o] = lk1 * aL + bL - cb * et;
-        p.bl[c][o] = lk3 * aL;
-      }
+      if (POLES & 2) p.al_prev[c][o] = lk1 * aL + lk3 * aLp - cb * et;
       const float es = p.e_s[c * p.e_cstride + o];
       const float es1 = p.e_s1[c * p.e_cstride + o];
       gca += et * es;

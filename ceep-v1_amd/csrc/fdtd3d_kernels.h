@@ -33,14 +33,18 @@ struct Step3DArgs {
   float* traces;                  // [B][P][T]
   float* e_save;                  // if set: E after this step, [3][B][vol] (adjoint store)
   long long e_cstride;            // floats per component block of e_save (B * bstride)
+  int xcd_order;                  // H/E kernels: block order (0 dispatch, 1/2 per XCD)
 };
 
 // One adjoint (time-reversed) step of the 3D update with the Debye/Lorentz
 // ADE, as three kernels over the adjoint state (lam = adjoint of E, hbar =
-// adjoint of H, ad/al/bl = adjoints of J_Debye, J_Lorentz, J_Lorentz_prev):
+// adjoint of H, ad/al = adjoints of J_Debye, J_Lorentz; the adjoint of
+// J_Lorentz_prev is k3L times al one reversed step back, so al is kept in two
+// buffers that swap roles every step instead of storing that product):
 //   A (pointwise): G = k2D ad + k2L al;  Et = lam + G;  lam <- ca Et - G;
-//       q = cb/dx Et;  ad <- k1D ad - cb gammaD Et;  al <- k1L al + bl - cb Et;
-//       bl <- k3L al;  g_ca += Et e^s;  g_cbi += Et (e^{s+1} - source - ca e^s)
+//       q = cb/dx Et;  ad <- k1D ad - cb gammaD Et;
+//       al_prev <- k1L al + k3L al_prev - cb Et  (the new al);
+//       g_ca += Et e^s;  g_cbi += Et (e^{s+1} - source - ca e^s)
 //   B: hbar <- da hbar + curl^T q        (q taken at the +1 neighbours)
 //   C: lam  <- lam + grad^T (dbi hbar)   (hbar taken at the -1 neighbours),
 //       then lam_z += dL/dtrace(step - 1) at the probes
@@ -52,8 +56,8 @@ struct Adj3DArgs {
   float* hbar[3];
   float* q[3];
   float* ad[3];
-  float* al[3];
-  float* bl[3];
+  float* al[3];                   // adjoint of J_Lorentz at this reversed step
+  float* al_prev[3];              // ... one reversed step back; receives the new al
   const float* ca;
   const float* cb;
   const float* da;                // null -> da = 1, dbi = ch

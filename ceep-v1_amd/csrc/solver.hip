@@ -137,8 +137,13 @@ struct ceep_fdtd {
   int fused3d = 0;             // 3D: 1/2 = one fused H+E launch per step, 3 = marching form (ping-pong fields)
   int march_len = 24;          // 3D marching form: x-slabs per workgroup
   int xcd_order = 1;           // 2D blocked kernels: contiguous tile ranges per XCD
-  int adj_r = 16;              // 2D blocked adjoint: rows per wavefront window
+  int adj_r = 0;               // 2D blocked adjoint: window rows (0: 32 for the four-wavefront
+                               // reversible kernel, 16 for the one-wavefront kernels)
   int adj_bpw = 8;             // 2D blocked adjoint: batch planes per wavefront
+  int adj_waves = 4;           // 2D reversible adjoint: wavefronts per window (1 or 4)
+  int adj_k = 4;               // 2D blocked adjoint: reversed steps per launch (1-4)
+  int fwd_waves = 1;           // 2D blocked forward: wavefronts per window (1 or 4)
+  int xcd3d = 0;               // 3D H/E kernels: block order (0 dispatch, 1/2 per XCD)
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
@@ -381,6 +386,12 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
     a.T = h->T; a.P = h->P; a.probe_cell = h->probe_cell; a.traces = h->traces;
     const int K = h->tb_k, R = h->tb_r;
+    const bool shape_ok = h->fwd_waves == 4
+        ? (R == 32 || R == 48 || R == 64) && (K == 4 || K == 6 || K == 8)
+        : (R == 16 || R == 24 || R == 32) && K >= 1 && K <= 4;
+    if (!shape_ok)
+      return fail(CEEP_EINVAL, "2D blocked launch shape: one wavefront per window takes 16, 24 or 32 "
+                               "rows and depth 1-4; four take 32, 48 or 64 rows and depth 4, 6 or 8");
     a.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
     a.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
     a.bpw = h->tb_bpw;
@@ -395,7 +406,10 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       a.ksteps = std::min(K, step_end - st);
       a.slot = h->box_floats();
       a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
-      CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
+      if (h->fwd_waves == 4)
+        CEEP_HIP(ceep::launch_step2d_mw(a, h->poles(), R, K, s));
+      else
+        CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
       cur = o;
     }
   } else if (h->cpml()) {
@@ -457,6 +471,7 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
     const int bb = h->bb > 4 ? 4 : h->bb;
     a.e_cstride = h->box_floats();
+    a.xcd_order = h->xcd3d;
     for (int st = step0; st < step_end; ++st) {
       const int i = (start + st - step0) & 1, o = i ^ 1;
       for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
@@ -1180,7 +1195,7 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
   a.sx = h->row_block; a.bstride = h->plane;
   for (int c = 0; c < 3; ++c) {
     a.lam[c] = h->adj3[0][c]; a.hbar[c] = h->adj3[1][c]; a.q[c] = h->adj3[2][c];
-    a.ad[c] = h->adj3[3][c]; a.al[c] = h->adj3[4][c]; a.bl[c] = h->adj3[5][c];
+    a.ad[c] = h->adj3[3][c]; a.al[c] = h->adj3[4][c]; a.al_prev[c] = h->adj3[5][c];
   }
   a.ca = h->ca; a.cb = h->cb; a.da = h->da; a.dbi = h->dbi;
   a.d_k1 = h->dk1; a.d_k2dt = h->dk2; a.l_k1 = h->lk1; a.l_k2dt = h->lk2; a.l_k3 = h->lk3;
@@ -1208,6 +1223,7 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
       a.e_s1 = h->estore + (size_t)(s + 1 - s0) * slot;
       a.step = s;
       CEEP_HIP(ceep::launch_adj3d(a, h->poles(), bb, st));
+      for (int c = 0; c < 3; ++c) std::swap(a.al[c], a.al_prev[c]);   // the new al is in al_prev
     }
   }
   if (nchunks > 1) {   // fold the chunk partials into plane 0 (in place: plane 0 is read first)
@@ -1267,8 +1283,11 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   h->traces64_valid = false;
   int cur = 0;
   for (int k = 0; k < nck; ++k) {
-    for (int f = 0; f < 3; ++f)
-      CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ck[(size_t)k * 3 + f], h->box_floats(), h->stream));
+    // the reversible adjoint starts each segment from the checkpoint at its
+    // end, so it never reads checkpoint 0 (the zero state)
+    if (k > 0 || !rev)
+      for (int f = 0; f < 3; ++f)
+        CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ck[(size_t)k * 3 + f], h->box_floats(), h->stream));
     const int s0 = k * every, n = std::min(every, h->T - s0);
     float* es = nullptr;
     if (k == nck - 1 && !rev) {
@@ -1340,9 +1359,8 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     // reversible: start from the exact state at the segment's end (the next
     // checkpoint, or the final state) and step forward and adjoint back together
     const int s0 = k * every, n = std::min(every, T - s0);
-    for (int f = 0; f < 3; ++f)
-      CEEP_HIP(ceep::launch_copy(h->ck[(size_t)(k + 1) * 3 + f], h->f[0][f], (long long)nbox, st));
-    const int K = h->tb_k, R = h->adj_r;
+    const int K = h->adj_k;
+    const int R = h->adj_r ? h->adj_r : (h->adj_waves == 4 ? 32 : 16);
     ceep::AdjTB2DArgs t{};
     t.xcd_order = h->xcd_order;
     t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
@@ -1354,16 +1372,24 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     t.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
     t.bpw = h->adj_bpw;
     t.ntasks = t.tiles_x * t.tiles_y * ((B + t.bpw - 1) / t.bpw);
+    // the first launch reads the segment-end checkpoint in place; later ones
+    // ping-pong between the two field buffers
     int fcur = 0;
+    bool first = true;
     for (int hi = s0 + n; hi > s0; hi -= K) {
       const int o = acur ^ 1, fo = fcur ^ 1;
       t.le_in = h->adj[acur][0]; t.mx_in = h->adj[acur][1]; t.my_in = h->adj[acur][2];
       t.le_out = h->adj[o][0]; t.mx_out = h->adj[o][1]; t.my_out = h->adj[o][2];
-      t.fe_in = h->f[fcur][0]; t.fhx_in = h->f[fcur][1]; t.fhy_in = h->f[fcur][2];
+      float* const* fin = first ? &h->ck[(size_t)(k + 1) * 3] : h->f[fcur];
+      t.fe_in = fin[0]; t.fhx_in = fin[1]; t.fhy_in = fin[2];
       t.fe_out = h->f[fo][0]; t.fhx_out = h->f[fo][1]; t.fhy_out = h->f[fo][2];
+      first = false;
       t.step_hi = hi;
       t.ksteps = std::min(K, hi - s0);
-      CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, st));
+      if (h->adj_waves == 4)
+        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, st));
+      else
+        CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, st));
       acur = o;
       fcur = fo;
     }
@@ -1386,7 +1412,8 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       h->estore_seg = k;
     }
     if (h->use_tb()) {
-      const int K = h->tb_k, R = h->adj_r;
+      const int K = h->adj_k;
+      const int R = h->adj_r ? h->adj_r : 16;
       ceep::AdjTB2DArgs t{};
       t.xcd_order = h->xcd_order;
       t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
@@ -1463,11 +1490,11 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 1) {
     h->use_graph = value ? 1 : 0;
   } else if (key == 2) {
-    if (value < 0 || value > 4) return fail(CEEP_EINVAL, "temporal depth must be in [0, 4]");
+    if (value < 0 || value > 8) return fail(CEEP_EINVAL, "temporal depth must be in [0, 8]");
     h->tb_k = value;
   } else if (key == 3) {
-    if (value != 16 && value != 24 && value != 32)
-      return fail(CEEP_EINVAL, "rows per wavefront must be 16, 24 or 32");
+    if (value != 16 && value != 24 && value != 32 && value != 48 && value != 64)
+      return fail(CEEP_EINVAL, "window rows must be 16, 24, 32, 48 or 64");
     h->tb_r = value;
   } else if (key == 4) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
@@ -1479,6 +1506,18 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 11) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "adjoint batch planes per wavefront must be >= 1");
     h->adj_bpw = value;
+  } else if (key == 14) {
+    if (value != 1 && value != 4) return fail(CEEP_EINVAL, "adjoint wavefronts per window must be 1 or 4");
+    h->adj_waves = value;
+  } else if (key == 15) {
+    if (value < 1 || value > 4) return fail(CEEP_EINVAL, "adjoint temporal depth must be in [1, 4]");
+    h->adj_k = value;
+  } else if (key == 16) {
+    if (value != 1 && value != 4) return fail(CEEP_EINVAL, "forward wavefronts per window must be 1 or 4");
+    h->fwd_waves = value;
+  } else if (key == 12) {
+    if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D block order must be 0, 1 or 2");
+    h->xcd3d = value;
   } else if (key == 9) {
     if (value != 0 && value != 1) return fail(CEEP_EINVAL, "XCD tile order must be 0 or 1");
     h->xcd_order = value;

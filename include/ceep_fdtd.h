@@ -147,9 +147,11 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
 /* Tuning knobs (keys 0-4 give the same results to FP32 rounding for every setting):
  *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8)
  *   key 1: replay the time loop as a captured hipGraph (0/1)
- *   key 2: 2D temporal depth K = steps per launch (0 = one-step kernel, 1..4)
- *   key 3: 2D rows per wavefront window of the blocked kernel (16, 24, 32)
- *   key 4: 2D batch planes per wavefront of the blocked kernel (>= 1)
+ *   key 2: 2D temporal depth K = steps per launch (0 = one-step kernel; 1..4
+ *          with one wavefront per window, 4, 6 or 8 with four, key 16)
+ *   key 3: 2D window rows of the blocked kernel (16, 24, 32 with one
+ *          wavefront per window; 32, 48, 64 with four)
+ *   key 4: 2D batch planes per window of the blocked kernel (>= 1)
  *   key 5: precision of the gradient path's forward and recomputation passes
  *          (0 = FP32, 1 = FP64; 2D non-dispersive): with 1,
  *          ceep_fdtd_run_checkpointed steps FP64 fields and returns FP64
@@ -168,10 +170,21 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *   key 9: 2D blocked kernels: 1 = renumber workgroups so each XCD runs a
  *          contiguous range of tiles (neighbouring tiles share an L2; default),
  *          0 = dispatch order.  Results are identical.
- *   key 10, 11: rows per wavefront window (16, 24, 32; default 16) and batch
- *          planes per wavefront (default 8) of the 2D blocked adjoint, which
- *          holds twice the forward's state per cell and prefers other shapes
- *          than the forward (keys 3, 4). */
+ *   key 10, 11: window rows (16, 24, 32; default 32 for the four-wavefront
+ *          reversible kernel, 16 otherwise) and batch planes per window
+ *          (default 8) of the 2D blocked adjoint, which holds twice the
+ *          forward's state per cell and prefers other shapes than the forward
+ *          (keys 3, 4).
+ *   key 14: 2D reversible adjoint: wavefronts per window, 1 or 4 (default 4:
+ *          each wavefront holds a quarter of the window's rows, halo rows
+ *          pass through LDS).
+ *   key 15: 2D blocked adjoint: reversed steps per launch, 1..4 (default 4).
+ *   key 16: 2D blocked forward: wavefronts per window, 1 (default) or 4.
+ *          Results are bit-identical between the two forms.
+ *   key 12: 3D H/E kernel block order: 0 = dispatch order (tile, x-slab,
+ *          batch chunk; default), 1 = the same renumbered so each XCD walks
+ *          contiguous slabs, 2 = per XCD with the batch chunk fastest.
+ *          Results are identical; 0 measured fastest on MI355X. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,

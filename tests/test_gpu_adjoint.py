@@ -94,21 +94,30 @@ def test_adjoint_fp64_forward_matches_autograd(amd, reference, every):
 @pytest.mark.parametrize("tuning", [
     {"temporal_depth": 0},
     {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4},
-    {"temporal_depth": 3, "rows_per_wave": 24, "batch_per_wave": 1},
-    {"temporal_depth": 2, "rows_per_wave": 32, "batch_per_wave": 3},
-    {"temporal_depth": 1, "rows_per_wave": 16, "batch_per_wave": 2},
+    {"temporal_depth": 3, "rows_per_wave": 24, "batch_per_wave": 1, "adjoint_temporal_depth": 3},
+    {"temporal_depth": 2, "rows_per_wave": 32, "batch_per_wave": 3, "adjoint_temporal_depth": 2},
+    {"temporal_depth": 1, "rows_per_wave": 16, "batch_per_wave": 2, "adjoint_temporal_depth": 1},
     {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4, "adjoint_reversible": 0},
-    {"temporal_depth": 3, "rows_per_wave": 24, "batch_per_wave": 3, "adjoint_reversible": 0},
+    {"temporal_depth": 3, "rows_per_wave": 24, "batch_per_wave": 3, "adjoint_reversible": 0,
+     "adjoint_temporal_depth": 3},
+    {"forward_waves": 4, "temporal_depth": 8, "rows_per_wave": 48, "batch_per_wave": 2},
+    {"forward_waves": 4, "temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 3,
+     "adjoint_reversible": 0, "adjoint_waves": 4, "adjoint_rows_per_wave": 32},
     {"temporal_depth": 4, "rows_per_wave": 24, "batch_per_wave": 2,
      "adjoint_rows_per_wave": 24, "adjoint_batch_per_wave": 3, "xcd_order": 0},
     {"temporal_depth": 3, "rows_per_wave": 16, "batch_per_wave": 5,
      "adjoint_rows_per_wave": 32, "adjoint_batch_per_wave": 1, "xcd_order": 1},
+    {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_rows_per_wave": 32, "adjoint_batch_per_wave": 3},
+    {"temporal_depth": 3, "adjoint_waves": 4, "adjoint_rows_per_wave": 24, "adjoint_batch_per_wave": 16,
+     "adjoint_temporal_depth": 3},
+    {"temporal_depth": 2, "adjoint_waves": 4, "adjoint_rows_per_wave": 16, "adjoint_batch_per_wave": 1,
+     "xcd_order": 0},
 ])
 def test_adjoint_kernel_variants(amd, reference, tuning):
     """One-step and blocked kernels, reversible (default) and recompute + E store
     adjoint forms, segment lengths that are not multiples of K, batch not a
     multiple of the planes per wavefront, adjoint launch shapes other than the
-    forward's, both workgroup orders."""
+    forward's, both workgroup orders, one- and four-wavefront windows."""
     from ceep_amd import adjoint
     r = reference
     s = _solver(amd, r["dt"], r["w"], r["pos"])

@@ -149,6 +149,16 @@ def test_2d_tuning_variants_bitwise(amd):
             s.run()
             assert np.array_equal(s.probe_data, base), (graph, xcd)
     assert rel_l2(base, one_step) < 1e-6
+    # four-wavefront windows: tall windows, deeper blocking, ragged batch chunks
+    for R, K, bpw in ((32, 4, 1), (48, 8, 2), (64, 6, 3), (32, 8, 4)):
+        s.tuning = {"forward_waves": 4, "temporal_depth": K, "rows_per_wave": R,
+                    "batch_per_wave": bpw, "graph": 1}
+        s._build()
+        s.run()
+        d = rel_l2(s.probe_data, base)
+        print(f"four-wavefront R={R} K={K} bpw={bpw}: rel-L2 vs one-wavefront {d:.1e}, "
+              f"bitwise {np.array_equal(s.probe_data, base)}")
+        assert d < 1e-6
     assert ref[0][0] is not None
 
 

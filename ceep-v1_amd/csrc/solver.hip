@@ -34,6 +34,19 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// Pinned host staging buffer of at least n floats (grown, never shrunk):
+// host<->device copies of it run at full PCIe rate and truly asynchronously.
+int pinned(float** p, size_t* cap, size_t n) {
+  if (*cap >= n) return CEEP_OK;
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipHostMalloc(reinterpret_cast<void**>(p), n * sizeof(float), hipHostMallocDefault) != hipSuccess)
+    return fail(CEEP_ENOMEM, "pinned host staging allocation failed");
+  *cap = n;
+  return CEEP_OK;
+}
+
 #define CEEP_HIP(call)                                                               \
   do {                                                                               \
     hipError_t e_ = (call);                                                          \
@@ -141,7 +154,11 @@ struct ceep_fdtd {
                                // reversible kernel, 16 for the one-wavefront kernels)
   int adj_bpw = 8;             // 2D blocked adjoint: batch planes per wavefront
   int adj_waves = 4;           // 2D reversible adjoint: wavefronts per window (1 or 4)
-  int adj_k = 4;               // 2D blocked adjoint: reversed steps per launch (1-4)
+  float* pin_dtr = nullptr;    // pinned host staging: dL/dtrace (FP32) ...
+  float* pin_g = nullptr;      // ... and the two gradient planes
+  size_t pin_dtr_n = 0, pin_g_n = 0;
+  int adj_k = 0;               // 2D blocked adjoint: reversed steps per launch (0: 5 for the
+                               // four-wavefront reversible kernel, 4 otherwise)
   int fwd_waves = 1;           // 2D blocked forward: wavefronts per window (1 or 4)
   int xcd3d = 0;               // 3D H/E kernels: block order (0 dispatch, 1/2 per XCD)
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
@@ -190,6 +207,10 @@ struct ceep_fdtd {
     if (g_cbi) (void)hipFree(g_cbi);
     if (d_dtr) (void)hipFree(d_dtr);
     g_ca = g_cbi = d_dtr = nullptr;
+    if (pin_dtr) (void)hipHostFree(pin_dtr);
+    if (pin_g) (void)hipHostFree(pin_g);
+    pin_dtr = pin_g = nullptr;
+    pin_dtr_n = pin_g_n = 0;
   }
   void drop_graph() {
     for (auto& g : gexec) {
@@ -1327,9 +1348,14 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   std::memset(grad_sigma, 0, (size_t)h->full_cells * sizeof(double));
   if (mx == 0.0 || P == 0) return CEEP_OK;
   const double scale = std::ldexp(1.0, -std::ilogb(mx));
-  std::vector<float> dtr(ntr);
+  if ((rc = pinned(&h->pin_dtr, &h->pin_dtr_n, ntr))) return rc;
+  float* dtr = h->pin_dtr;
   for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
   const bool rev = h->ck_rev && h->use_rev();
+  const bool mw_rev = rev && h->adj_waves == 4;
+  const int adj_k = h->adj_k ? h->adj_k : (mw_rev ? 5 : 4);
+  if (h->use_tb() && adj_k > 4 && !mw_rev)
+    return fail(CEEP_EINVAL, "adjoint temporal depth 5 needs the four-wavefront reversible adjoint");
   if (!rev && !h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * nbox))) return rc;
   for (int i = 0; i < 2; ++i)
     for (int f = 0; f < 3; ++f)
@@ -1340,7 +1366,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   if (!h->g_cbi && (rc = dalloc(&h->g_cbi, (size_t)B * h->plane))) return rc;
   if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
   hipStream_t st = h->stream;
-  CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr.data(), ntr * sizeof(float), hipMemcpyHostToDevice, st));
+  CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr, ntr * sizeof(float), hipMemcpyHostToDevice, st));
   for (int f = 0; f < 3; ++f) CEEP_HIP(hipMemsetAsync(h->adj[0][f], 0, bytes, st));
   CEEP_HIP(hipMemsetAsync(h->g_ca, 0, (size_t)nchunks * h->plane * sizeof(float), st));
   CEEP_HIP(hipMemsetAsync(h->g_cbi, 0, (size_t)nchunks * h->plane * sizeof(float), st));
@@ -1359,7 +1385,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     // reversible: start from the exact state at the segment's end (the next
     // checkpoint, or the final state) and step forward and adjoint back together
     const int s0 = k * every, n = std::min(every, T - s0);
-    const int K = h->adj_k;
+    const int K = adj_k;
     const int R = h->adj_r ? h->adj_r : (h->adj_waves == 4 ? 32 : 16);
     ceep::AdjTB2DArgs t{};
     t.xcd_order = h->xcd_order;
@@ -1412,7 +1438,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       h->estore_seg = k;
     }
     if (h->use_tb()) {
-      const int K = h->adj_k;
+      const int K = adj_k;
       const int R = h->adj_r ? h->adj_r : 16;
       ceep::AdjTB2DArgs t{};
       t.xcd_order = h->xcd_order;
@@ -1453,30 +1479,38 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     CEEP_HIP(ceep::launch_sum_parts(h->g_ca, nchunks, h->plane, h->plane, h->g_ca, st));
     CEEP_HIP(ceep::launch_sum_parts(h->g_cbi, nchunks, h->plane, h->plane, h->g_cbi, st));
   }
-  std::vector<float> gca(h->plane), gcb(h->plane);
-  CEEP_HIP(hipMemcpyAsync(gca.data(), h->g_ca, gca.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-  CEEP_HIP(hipMemcpyAsync(gcb.data(), h->g_cbi, gcb.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  if ((rc = pinned(&h->pin_g, &h->pin_g_n, (size_t)2 * h->plane))) return rc;
+  const float* gca = h->pin_g;
+  const float* gcb = h->pin_g + h->plane;
+  CEEP_HIP(hipMemcpyAsync(h->pin_g, h->g_ca, (size_t)h->plane * sizeof(float), hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipMemcpyAsync(h->pin_g + h->plane, h->g_cbi, (size_t)h->plane * sizeof(float),
+                          hipMemcpyDeviceToHost, st));
   // the recomputation overwrote the fields: leave a defined (zero) state
   if ((rc = zero_state(h))) return rc;
   CEEP_HIP(hipStreamSynchronize(st));
-  // chain rule to eps_r and sigma in FP64 (ca, cb of fdtd_2d_batched.py:153-157)
-  const double dt = h->dt;
+  // chain rule to eps_r and sigma in FP64 (ca, cb of fdtd_2d_batched.py:153-157):
+  //   s = sigma dt / (2 eps),  ca = (1 - s) / (1 + s),  cb = (dt / eps) / (1 + s)
+  //   dca/ds = -2 / (1 + s)^2,  ds/deps = -s / eps,  ds/dsigma = dt / (2 eps)
+  //   dcb/deps = -cb / eps - cb / (1 + s) ds/deps,  dcb/dsigma = -cb / (1 + s) ds/dsigma
+  // g_cbi sums lam * (cb / dx) * curl = cb * dL/dcb, hence the division by cb.
+  const double dt = h->dt, inv_scale = 1.0 / scale;   // scale is a power of two: exact
   for (int x = 1; x < h->M[0] - 1; ++x)
     for (int y = 1; y < h->M[1] - 1; ++y) {
       const long long o = h->box_index(x, y, 0);
       const long long g = h->full_index(x + h->org, y + h->org, 0);
       const double eps = h->eps_r[g] * kEps0, sg = h->sigma_e[g];
-      const double sv = sg * dt / (2.0 * eps);
-      const double cb = (dt / eps) / (1.0 + sv);
-      const double dL_dca = gca[o] / scale;
-      const double dL_dcb = gcb[o] / scale / cb;   // g_cbi sums lam * cbi * curl
-      const double dca_ds = -2.0 / ((1.0 + sv) * (1.0 + sv));
-      const double ds_deps = -sv / eps, ds_dsig = dt / (2.0 * eps);
-      const double dcb_deps = -(dt / (eps * eps)) / (1.0 + sv) +
-                              (dt / eps) * (-1.0 / ((1.0 + sv) * (1.0 + sv))) * ds_deps;
-      const double dcb_dsig = (dt / eps) * (-1.0 / ((1.0 + sv) * (1.0 + sv))) * ds_dsig;
-      grad_eps_r[g] = kEps0 * (dL_dca * dca_ds * ds_deps + dL_dcb * dcb_deps);
-      grad_sigma[g] = dL_dca * dca_ds * ds_dsig + dL_dcb * dcb_dsig;
+      const double inv_eps = 1.0 / eps;
+      const double sv = 0.5 * sg * dt * inv_eps;
+      const double inv_1s = 1.0 / (1.0 + sv);
+      const double dL_dca = gca[o] * inv_scale;
+      const double dL_dcb_cb = gcb[o] * inv_scale;        // cb * dL/dcb
+      const double dca_ds = -2.0 * inv_1s * inv_1s;
+      const double ds_deps = -sv * inv_eps, ds_dsig = 0.5 * dt * inv_eps;
+      // dL/dcb * dcb/dq = (cb dL/dcb) * (dcb/dq / cb)
+      const double rdcb_deps = -inv_eps - inv_1s * ds_deps;
+      const double rdcb_dsig = -inv_1s * ds_dsig;
+      grad_eps_r[g] = kEps0 * (dL_dca * dca_ds * ds_deps + dL_dcb_cb * rdcb_deps);
+      grad_sigma[g] = dL_dca * dca_ds * ds_dsig + dL_dcb_cb * rdcb_dsig;
     }
   return CEEP_OK;
 }
@@ -1510,7 +1544,8 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value != 1 && value != 4) return fail(CEEP_EINVAL, "adjoint wavefronts per window must be 1 or 4");
     h->adj_waves = value;
   } else if (key == 15) {
-    if (value < 1 || value > 4) return fail(CEEP_EINVAL, "adjoint temporal depth must be in [1, 4]");
+    if (value < 1 || value > 5)
+      return fail(CEEP_EINVAL, "adjoint temporal depth must be in [1, 5] (5: four-wavefront reversible form only)");
     h->adj_k = value;
   } else if (key == 16) {
     if (value != 1 && value != 4) return fail(CEEP_EINVAL, "forward wavefronts per window must be 1 or 4");

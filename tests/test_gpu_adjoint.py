@@ -112,6 +112,8 @@ def test_adjoint_fp64_forward_matches_autograd(amd, reference, every):
      "adjoint_temporal_depth": 3},
     {"temporal_depth": 2, "adjoint_waves": 4, "adjoint_rows_per_wave": 16, "adjoint_batch_per_wave": 1,
      "xcd_order": 0},
+    {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_temporal_depth": 5, "adjoint_batch_per_wave": 5},
+    {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_temporal_depth": 5, "adjoint_rows_per_wave": 24},
 ])
 def test_adjoint_kernel_variants(amd, reference, tuning):
     """One-step and blocked kernels, reversible (default) and recompute + E store
@@ -126,6 +128,15 @@ def test_adjoint_kernel_variants(amd, reference, tuning):
     ee, es = rel_l2(ge, r["ge"]), rel_l2(gs, r["gs"])
     print(f"{tuning}: dL/deps rel-L2 {ee:.2e}, dL/dsigma rel-L2 {es:.2e}")
     assert ee < 1e-3 and es < 1e-3
+
+
+def test_adjoint_depth5_needs_four_wavefront_form(amd, reference):
+    from ceep_amd import adjoint
+    r = reference
+    s = _solver(amd, r["dt"], r["w"], r["pos"])
+    s.tuning = {"temporal_depth": 4, "adjoint_waves": 1, "adjoint_temporal_depth": 5}
+    with pytest.raises(ValueError, match="four-wavefront"):
+        adjoint.gradient(s, r["eps"], r["sig"], FREQS, r["St"], every=45)
 
 
 def test_adjoint_matches_finite_differences(amd, reference):

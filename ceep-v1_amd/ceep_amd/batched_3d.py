@@ -28,7 +28,22 @@ class BatchedFDTD3D:
                  cpml_thickness: int, source_positions: List[Tuple[int, int, int]],
                  probe_positions: List[Tuple[int, int, int]], frequency: float,
                  delay_factor: float = 5.0, *, courant: Optional[float] = None,
-                 waveform: Optional[np.ndarray] = None, cpml_R: float = 1e-8):
+                 waveform: Optional[np.ndarray] = None, cpml_R: float = 1e-8,
+                 boundary: str = "pec", cpml_order: int = 4, cpml_sigma_factor: float = 2.0,
+                 cpml_alpha_max: float = 0.05, cpml_kappa_max: float = 1.0):
+        """``boundary="pec"`` (default) is the reference's batched solver: its
+        CPML is inert, so the result is the PEC cavity it encloses.
+        ``boundary="cpml"`` is an opt-in physical absorbing boundary: the full
+        grid stepped in the order of the reference's single-run ``FDTD3D``
+        (solvers/fdtd_3d.py, fdtd_base.py:161-185) with its
+        ``CPML(thickness=cpml_thickness, order, sigma_factor, alpha_max,
+        kappa_max)`` 3D recursions (boundaries/absorbing.py:567-943); sources
+        and probes may lie anywhere in the grid."""
+        if boundary not in ("pec", "cpml"):
+            raise ValueError("boundary must be 'pec' or 'cpml'")
+        self.boundary = boundary
+        self.cpml_params = dict(order=cpml_order, sigma_factor=cpml_sigma_factor,
+                                alpha_max=cpml_alpha_max, kappa_max=cpml_kappa_max)
         self.nx, self.ny, self.nz = nx, ny, nz
         self.dx = dx
         self.dy = dx
@@ -138,9 +153,12 @@ class BatchedFDTD3D:
         self.waveform = self._waveform()
         if self._handle is not None:
             self._handle.close()
-        h = _native.Handle(3, self._shape, self.cpml_thickness, _native.BOX_REFERENCE,
+        mode = _native.BOX_CPML if self.boundary == "cpml" else _native.BOX_REFERENCE
+        h = _native.Handle(3, self._shape, self.cpml_thickness, mode,
                            self.batch, len(self.probe_positions), self.total_steps,
                            self.dx, self.dt)
+        if self.boundary == "cpml":
+            h.set_cpml(**self.cpml_params)
         if self._debye is not None:
             h.set_debye(*self._debye)
         if self._lorentz is not None:

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fdtd_common.h"
+
 namespace ceep {
 
 struct Step2DArgs {
@@ -102,7 +104,6 @@ hipError_t launch_step2d_f64(const Step2D64Args& a, hipStream_t s);
 
 // Physical-CPML mode (full grid, the reference single-run solver's step
 // order and convolutional PML); profiles per depth index.
-constexpr int kMaxCpml = 64;
 struct Cpml2DArgs {
   int Mx, My, pitch, B, t;        // full grid, CPML thickness t
   long long bstride;

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fdtd_common.h"
+
 namespace ceep {
 
 struct Step3DArgs {
@@ -35,6 +37,39 @@ struct Step3DArgs {
   long long e_cstride;            // floats per component block of e_save (B * bstride)
   int xcd_order;                  // H/E kernels: block order (0 dispatch, 1/2 per XCD)
 };
+
+// Physical-CPML mode of the 3D engine: the full grid, fields in place, the
+// reference's single-run 3D update (fdtd_3d.py:97-171) with its convolutional
+// PML (absorbing.py:567-943).  psi arrays use the field layout; only the
+// cells of their slabs are read or written.
+struct Cpml3DArgs {
+  int Mx, My, Mz, pitch, B, t;    // full grid, CPML thickness t
+  long long sx, bstride;
+  float* ex; float* ey; float* ez;
+  float* hx; float* hy; float* hz;
+  float* jd[3]; float* jl_cur[3]; float* jl_prev[3];
+  // psi_<field><axis>: the recursion of that field's correction along that axis
+  float* psi_hxy; float* psi_hxz; float* psi_hyz; float* psi_hyx; float* psi_hzx; float* psi_hzy;
+  float* psi_exy; float* psi_exz; float* psi_eyz; float* psi_eyx; float* psi_ezx; float* psi_ezy;
+  const float* ca; const float* cb;
+  const float* da;                // null -> da = 1, dbi = ch
+  const float* dbi;               // db / dx per cell
+  const float* d_k1; const float* d_k2dt;
+  const float* l_k1; const float* l_k2dt; const float* l_k3;
+  float ch, idx;                  // (dt / mu0) / dx, 1 / dx
+  float dtmu, dteps;              // dt / mu0, dt / eps0 (the psi corrections)
+  float bh[kMaxCpml], chh[kMaxCpml], be[kMaxCpml], ce[kMaxCpml];
+  const long long* src_cell;
+  const float* wave;
+  long long wave_stride_b;
+  int step, T, P;
+  const short* probe_head;
+  const short* probe_next;
+  float* traces;
+};
+
+// One step: H + H psi, then E (+ADE) + Ez source + E psi + Ez probes.
+hipError_t launch_cpml3d(const Cpml3DArgs& a, int poles, hipStream_t s);
 
 // One adjoint (time-reversed) step of the 3D update with the Debye/Lorentz
 // ADE, as three kernels over the adjoint state (lam = adjoint of E, hbar =

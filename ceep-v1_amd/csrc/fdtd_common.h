@@ -16,6 +16,7 @@ namespace ceep {
 
 constexpr int kMaxPoles = 2;          // Debye + Lorentz per cell
 constexpr int kMaxBatch = 1024;
+constexpr int kMaxCpml = 64;          // physical-CPML mode: largest slab thickness
 
 // One ADE pole's per-cell coefficient maps (FP32, cavity box layout without
 // the batch axis).  k2 is pre-divided by dt: J += k2dt * (E_new - E_old).

@@ -169,7 +169,9 @@ struct ceep_fdtd {
   // CPML class (absorbing.py:226-230 defaults) and the psi state
   int cpml_order = 4;
   double cpml_sigma_factor = 2.0, cpml_alpha_max = 0.05, cpml_kappa_max = 1.0;
-  float* psi[4] = {nullptr, nullptr, nullptr, nullptr};   // hyx, hxy, ezx, ezy
+  // physical-CPML psi arrays (field layout): 2D hyx, hxy, ezx, ezy; 3D hxy, hxz,
+  // hyz, hyx, hzx, hzy, exy, exz, eyz, eyx, ezx, ezy
+  float* psi[12] = {nullptr};
   bool cpml() const { return box_mode == CEEP_BOX_CPML; }
 
   long long box_floats() const { return (long long)B * plane; }
@@ -276,7 +278,8 @@ int commit_materials(ceep_fdtd* h) {
         // E is stepped on the interior only; elsewhere every E-side
         // coefficient is 0 so the kernels' uniform update keeps E at 0
         const bool interior =
-            h->cpml() ? (a >= 1 && b >= 1)        // the E update domain [1:, 1:] (fdtd_2d.py:168)
+            h->cpml() ? (a >= 1 && b >= 1 && (h->ndim == 2 || c >= 1))   // [1:, 1:(, 1:)]
+                                                  // (fdtd_2d.py:168, fdtd_3d.py:154)
                       : (a >= 1 && a <= h->M[0] - 2 && b >= 1 && b <= h->M[1] - 2 &&
                          (h->ndim == 2 || (c >= 1 && c <= mz - 2)));
         ca[o] = interior ? (float)((1.0 - s) / (1.0 + s)) : 0.f;
@@ -351,7 +354,7 @@ int zero_state(ceep_fdtd* h) {
     CEEP_HIP(hipMemsetAsync(h->f[0][k], 0, bytes, s));
     CEEP_HIP(hipMemsetAsync(h->f[1][k], 0, bytes, s));
   }
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 12; ++k)
     if (h->psi[k]) CEEP_HIP(hipMemsetAsync(h->psi[k], 0, bytes, s));
   for (int k = 0; k < 3; ++k) {
     if (h->jd[k]) CEEP_HIP(hipMemsetAsync(h->jd[k], 0, bytes, s));
@@ -367,7 +370,7 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save);
 // CPML profiles of CPML._compute_pml_params (absorbing.py:266-311) in FP64,
 // rounded once: b, c at the E (integer) and H (half-integer) depths of a slab
 // of thickness n cells (dx == dy, so one table serves both axes).
-void cpml_profiles(const ceep_fdtd* h, ceep::Cpml2DArgs& a) {
+void cpml_profiles(const ceep_fdtd* h, float* be, float* ce, float* bh, float* ch) {
   const int n = h->n;
   const double dt = h->dt, d = h->dx, ord = (double)h->cpml_order;
   const double sigma_max = h->cpml_sigma_factor * (h->cpml_order + 1) / (150.0 * M_PI * d);
@@ -379,8 +382,8 @@ void cpml_profiles(const ceep_fdtd* h, ceep::Cpml2DArgs& a) {
       const double al = h->cpml_alpha_max * (1.0 - rho);
       const double b = std::exp(-(sg / kap + al) * dt / kEps0);
       const double c = std::fabs(sg) < 1e-20 ? 0.0 : sg / (sg * kap + kap * kap * al) * (b - 1.0);
-      if (half) { a.bh[k] = (float)b; a.chh[k] = (float)c; }
-      else { a.be[k] = (float)b; a.ce[k] = (float)c; }
+      if (half) { bh[k] = (float)b; ch[k] = (float)c; }
+      else { be[k] = (float)b; ce[k] = (float)c; }
     }
   }
 }
@@ -433,7 +436,7 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
         CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
       cur = o;
     }
-  } else if (h->cpml()) {
+  } else if (h->cpml() && h->ndim == 2) {
     ceep::Cpml2DArgs a{};
     a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.t = h->n;
     a.bstride = h->plane;
@@ -446,7 +449,7 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     a.idx = h->idx;
     a.dtmu = (float)(h->dt / kMu0);
     a.dteps = (float)(h->dt / kEps0);
-    cpml_profiles(h, a);
+    cpml_profiles(h, a.be, a.ce, a.bh, a.chh);
     a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
     a.T = h->T; a.P = h->P;
     a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
@@ -475,6 +478,30 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       a.step = st;
       a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
       CEEP_HIP(ceep::launch_step2d(a, poles, h->bb, s));
+    }
+  } else if (h->cpml()) {
+    ceep::Cpml3DArgs a{};
+    a.Mx = h->M[0]; a.My = h->M[1]; a.Mz = h->M[2]; a.pitch = h->pitch; a.B = h->B; a.t = h->n;
+    a.sx = h->row_block; a.bstride = h->plane;
+    a.ex = h->f[0][0]; a.ey = h->f[0][1]; a.ez = h->f[0][2];
+    a.hx = h->f[0][3]; a.hy = h->f[0][4]; a.hz = h->f[0][5];
+    for (int k = 0; k < 3; ++k) a.jd[k] = h->jd[k];
+    float** ps = &a.psi_hxy;
+    for (int k = 0; k < 12; ++k) ps[k] = h->psi[k];
+    a.ca = h->ca; a.cb = h->cb; a.da = h->da; a.dbi = h->dbi;
+    a.d_k1 = h->dk1; a.d_k2dt = h->dk2; a.l_k1 = h->lk1; a.l_k2dt = h->lk2; a.l_k3 = h->lk3;
+    a.ch = h->ch; a.idx = h->idx;
+    a.dtmu = (float)(h->dt / kMu0);
+    a.dteps = (float)(h->dt / kEps0);
+    cpml_profiles(h, a.be, a.ce, a.bh, a.chh);
+    a.src_cell = h->src3; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
+    a.T = h->T; a.P = h->P;
+    a.probe_head = h->probe_head; a.probe_next = h->probe_next; a.traces = h->traces;
+    for (int st = step0; st < step_end; ++st) {
+      const int i = (start + st - step0) & 1, o = i ^ 1;
+      for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
+      a.step = st;
+      CEEP_HIP(ceep::launch_cpml3d(a, poles, s));
     }
   } else {
     ceep::Step3DArgs a{};
@@ -659,11 +686,14 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
   if (box_mode == CEEP_BOX_FULL && ndim != 2)
     return fail(CEEP_EINVAL, "the full-grid PEC box (run_cpu) exists for 2D only");
   if (box_mode == CEEP_BOX_CPML) {
-    if (ndim != 2) return fail(CEEP_EUNSUPPORTED, "the physical-CPML mode covers 2D TMz");
     if (cpml_n < 1 || cpml_n > ceep::kMaxCpml)
       return fail(CEEP_EINVAL, "CPML thickness must be in [1, 64]");
-    if (2 * (int64_t)cpml_n > shape[0] || 2 * (int64_t)cpml_n > shape[1])
-      return fail(CEEP_EINVAL, "the two CPML slabs of an axis must not overlap (2*thickness <= N)");
+    // 2D slabs [0, t) and [N-t, N); 3D H slabs [0, t) and [N-1-t, N-1)
+    const int64_t need = 2 * (int64_t)cpml_n + (ndim == 3 ? 1 : 0);
+    for (int d = 0; d < ndim; ++d)
+      if (need > shape[d])
+        return fail(CEEP_EINVAL, "the two CPML slabs of an axis must not overlap "
+                                 "(2*thickness <= N in 2D, 2*thickness + 1 <= N in 3D)");
   }
   if (batch < 1 || batch > ceep::kMaxBatch) return fail(CEEP_EINVAL, "batch must be in [1, 1024]");
   if (n_probes < 0 || n_probes > 32767) return fail(CEEP_EINVAL, "n_probes must be in [0, 32767]");
@@ -713,7 +743,7 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
     rc = dalloc(&h->f[0][k], h->box_floats());
     if (!rc) rc = dalloc(&h->f[1][k], h->box_floats());
   }
-  for (int k = 0; k < 4 && !rc && h->cpml(); ++k) rc = dalloc(&h->psi[k], h->box_floats());
+  for (int k = 0; k < (ndim == 2 ? 4 : 12) && !rc && h->cpml(); ++k) rc = dalloc(&h->psi[k], h->box_floats());
   if (!rc) rc = dalloc(&h->ca, h->plane);
   if (!rc) rc = dalloc(&h->cb, h->plane);
   if (!rc) rc = dalloc(&h->traces, (size_t)batch * (n_probes > 0 ? n_probes : 1) * total_steps);
@@ -758,7 +788,7 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   for (int i = 0; i < 2; ++i)
     for (int k = 0; k < 3; ++k) dfree(h->f64[i][k]);
   dfree(h->ca64); dfree(h->cb64); dfree(h->wave64); dfree(h->traces64);
-  for (int k = 0; k < 4; ++k) dfree(h->psi[k]);
+  for (int k = 0; k < 12; ++k) dfree(h->psi[k]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CEEP_OK;

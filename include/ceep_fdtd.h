@@ -37,10 +37,11 @@ enum {
 enum {
   CEEP_BOX_REFERENCE = 0, /* BatchedFDTD.run(): E interior [n+1, N-n-1], box [n, N-n] */
   CEEP_BOX_FULL = 1,      /* BatchedFDTD2D.run_cpu(): PEC box over the full grid */
-  CEEP_BOX_CPML = 2       /* 2D opt-in physical absorbing boundary: the full grid stepped
-                             in the order of FDTDBase.step (fdtd_base.py:161-185) with the
-                             CPML class's psi recursions (absorbing.py:266-561) on slabs of
-                             cpml_n cells; sources and probes anywhere */
+  CEEP_BOX_CPML = 2       /* opt-in physical absorbing boundary: the full grid stepped in
+                             the order of FDTDBase.step (fdtd_base.py:161-185) with the CPML
+                             class's psi recursions on slabs of cpml_n cells (2D:
+                             absorbing.py:266-561; 3D: :567-943 around the update of
+                             fdtd_3d.py:97-171); sources and probes anywhere */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -192,7 +193,8 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,
  * sigma_factor 2.0, alpha_max 0.05, kappa_max 1.0, as CPML's fields,
  * absorbing.py:226-230).  Replaces: the CPML(...) boundary object handed to
- * FDTD2D (solvers/fdtd_2d.py, boundaries/absorbing.py:174-311). */
+ * FDTD2D / FDTD3D (solvers/fdtd_2d.py, fdtd_3d.py, boundaries/absorbing.py:
+ * 174-311, 567-626). */
 int ceep_fdtd_set_cpml(ceep_fdtd* h, int order, double sigma_factor, double alpha_max,
                        double kappa_max);
 

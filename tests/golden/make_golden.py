@@ -21,6 +21,9 @@ Vectors written (all float64, ``numpy.savez_compressed``, no pickles):
 * ``cpml2d_*.npz`` -- ``FDTD2D`` with a ``CPML`` boundary
   (boundaries/absorbing.py:174-561, fdtd_base.py:161-185): the physical
   absorbing-boundary mode, traces and final fields.
+* ``cpml3d_*.npz`` -- ``FDTD3D`` with a ``CPML`` boundary
+  (boundaries/absorbing.py:567-943) and mu_r / sigma_m: the 3D physical
+  absorbing-boundary mode, traces and final fields.
 """
 from __future__ import annotations
 
@@ -225,6 +228,69 @@ def golden_cpml2d():
                             kappa_max=cpml.kappa_max, **extra)
 
 
+def golden_cpml3d():
+    """3D convolutional PML (boundaries/absorbing.py:567-943) around the
+    single-run 3D update (solvers/fdtd_3d.py:97-171): one Ez soft source,
+    probes in the interior, in the PML slabs and in a corner, a lossy block
+    with mu_r / sigma_m; default CPML parameters and a second set.
+
+    ``FDTD3D`` itself cannot be instantiated (it leaves ``get_field`` and
+    ``initialize`` of its base class abstract), so the step is driven here in
+    the order of ``FDTDBase.step`` (fdtd_base.py:161-185) by calling the
+    reference's own pieces on its ``Grid3D``: ``FDTD3D._update_h_fields``,
+    ``CPML.apply_h_3d``, ``FDTD3D._update_e_fields``, the source's
+    ``value_at`` added at the source cell (``_inject_sources``, fdtd_3d.py:
+    173-195), ``CPML.apply_e_3d``, then Ez at the probes."""
+    from types import SimpleNamespace
+    from ceep.boundaries.absorbing import CPML
+    from ceep.core.backend import set_backend
+    from ceep.core.config import GridConfig, SimulationConfig
+    from ceep.core.grid_3d import Grid3D
+    from ceep.solvers.fdtd_3d import FDTD3D
+    from ceep.sources.waveforms import ModulatedGaussianSource
+    set_backend("numpy")
+
+    nx, ny, nz, T, dx = 30, 26, 24, 90, 1e-3
+    probes = [(14, 12, 11), (20, 16, 13), (2, 12, 11), (27, 20, 9), (1, 1, 1), (15, 24, 22)]
+    sx, sy, sz = 13, 12, 11
+    for name, ckw in (("default", dict(thickness=6)),
+                      ("custom", dict(thickness=5, order=3, sigma_factor=1.5, alpha_max=0.02,
+                                      kappa_max=3.0))):
+        cfg = SimulationConfig(grid=GridConfig(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=dx),
+                               courant=0.5, total_steps=T)
+        grid = Grid3D(cfg.grid, cfg.dt)
+        grid.set_material_region(16, 22, 9, 17, 8, 15, eps_r=5.0, mu_r=1.5, sigma_e=0.3,
+                                 sigma_m=800.0)
+        src = ModulatedGaussianSource(x=sx, y=sy, z=sz, frequency=20e9, bandwidth=30e9,
+                                      amplitude=1.0, delay_factor=3.0)
+        cpml = CPML(**ckw)
+        cpml.initialize_3d(grid, cfg.dt)
+        solver = SimpleNamespace(grid=grid, config=cfg, _use_fused_kernels=False)
+        tr = np.zeros((len(probes), T))
+        wave = np.zeros(T)
+        for step in range(T):
+            FDTD3D._update_h_fields(solver)
+            cpml.apply_h_3d(grid)
+            FDTD3D._update_e_fields(solver)
+            val = src.value_at(step, cfg.dt)
+            wave[step] = val
+            if val != 0.0:
+                grid.ez[sx, sy, sz] += val
+            cpml.apply_e_3d(grid)
+            for q, p in enumerate(probes):
+                tr[q, step] = grid.ez[p]
+        np.savez_compressed(os.path.join(HERE, f"cpml3d_{name}.npz"), traces=tr, dt=cfg.dt,
+                            waveform=wave, eps_r=np.asarray(grid.eps_r),
+                            sigma_e=np.asarray(grid.sigma_e), mu_r=np.asarray(grid.mu_r),
+                            sigma_m=np.asarray(grid.sigma_m),
+                            ez=np.asarray(grid.ez), hx=np.asarray(grid.hx),
+                            hz=np.asarray(grid.hz), probes=np.array(probes),
+                            source=np.array([sx, sy, sz]), shape=np.array([nx, ny, nz]), dx=dx,
+                            thickness=cpml.thickness, order=cpml.order,
+                            sigma_factor=cpml.sigma_factor, alpha_max=cpml.alpha_max,
+                            kappa_max=cpml.kappa_max)
+
+
 def golden_das():
     """``DelayAndSumBeamformer.reconstruct`` (imaging/beamforming.py:86-313),
     DAS and DMAS, raw and normalised, on a seeded S-matrix."""
@@ -257,5 +323,6 @@ if __name__ == "__main__":
     golden_bf3d()
     golden_ade2d()
     golden_cpml2d()
+    golden_cpml3d()
     golden_das()
     print("golden vectors written to", HERE)

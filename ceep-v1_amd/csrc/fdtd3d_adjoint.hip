@@ -50,27 +50,41 @@ __global__ __launch_bounds__(256) void adj3d_a_kernel(Adj3DArgs p) {
   if (POLES & 1) { dk1 = p.d_k1[cell]; dk2 = p.d_k2dt[cell]; gam = 0.5f * (1.f + dk1); }
   if (POLES & 2) { lk1 = p.l_k1[cell]; lk2 = p.l_k2dt[cell]; lk3 = p.l_k3[cell]; }
   float gca = 0.f, gcb = 0.f;
+  const int nb = min(BB, p.B - b0);
+  // every load of the chunk's BB planes first (this kernel is bound by HBM
+  // and the loads in flight per wave are what feeds it), then the updates
+  float lam[BB][3], aD[BB][3], aL[BB][3], aLp[BB][3], es[BB][3], es1[BB][3];
 #pragma unroll
   for (int k = 0; k < BB; ++k) {
+    if (k >= nb) break;
+    const long long o = (long long)(b0 + k) * p.bstride + cell;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      lam[k][c] = p.lam[c][o];
+      if (POLES & 1) aD[k][c] = p.ad[c][o];
+      if (POLES & 2) { aL[k][c] = p.al[c][o]; aLp[k][c] = p.al_prev[c][o]; }
+      es[k][c] = p.e_s[c * p.e_cstride + o];
+      es1[k][c] = p.e_s1[c * p.e_cstride + o];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < BB; ++k) {
+    if (k >= nb) break;
     const int bt = b0 + k;
-    if (bt >= p.B) break;
     const long long o = (long long)bt * p.bstride + cell;
     const float inj = cell == p.src_cell[bt] ? p.wave[(long long)bt * p.wave_stride_b + p.step] : 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float lam = p.lam[c][o];
-      float G = 0.f, aD = 0.f, aL = 0.f, aLp = 0.f;
-      if (POLES & 1) { aD = p.ad[c][o]; G += dk2 * aD; }
-      if (POLES & 2) { aL = p.al[c][o]; aLp = p.al_prev[c][o]; G += lk2 * aL; }
-      const float et = lam + G;                 // adjoint of e~ (it feeds e' and e~ - e)
+      float G = 0.f;
+      if (POLES & 1) G += dk2 * aD[k][c];
+      if (POLES & 2) G += lk2 * aL[k][c];
+      const float et = lam[k][c] + G;          // adjoint of e~ (it feeds e' and e~ - e)
       p.lam[c][o] = ca * et - G;
       p.q[c][o] = cbi * et;
-      if (POLES & 1) p.ad[c][o] = dk1 * aD - cb * gam * et;
-      if (POLES & 2) p.al_prev[c][o] = lk1 * aL + lk3 * aLp - cb * et;
-      const float es = p.e_s[c * p.e_cstride + o];
-      const float es1 = p.e_s1[c * p.e_cstride + o];
-      gca += et * es;
-      gcb += et * (es1 - (c == 2 ? inj : 0.f) - ca * es);   // = et cb (curl/dx - sum gJ)
+      if (POLES & 1) p.ad[c][o] = dk1 * aD[k][c] - cb * gam * et;
+      if (POLES & 2) p.al_prev[c][o] = lk1 * aL[k][c] + lk3 * aLp[k][c] - cb * et;
+      gca += et * es[k][c];
+      gcb += et * (es1[k][c] - (c == 2 ? inj : 0.f) - ca * es[k][c]);   // = et cb (curl/dx - sum gJ)
     }
   }
   const long long g = (long long)chunk * p.gstride + cell;

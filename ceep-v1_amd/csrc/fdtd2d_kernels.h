@@ -102,6 +102,32 @@ struct Step2D64Args {
 
 hipError_t launch_step2d_f64(const Step2D64Args& a, hipStream_t s);
 
+// Temporally blocked form of step2d_f64_kernel: K steps per launch on R x 64
+// windows split over the four wavefronts of a workgroup (R / 4 rows each,
+// halo rows through LDS), one workgroup per (tile, TX chunk).  Bitwise the
+// one-step kernel's results.  R in {16, 32}, K in {2, 4}.
+struct TB2D64Args {
+  int Mx, My, pitch, B;
+  long long bstride;
+  const double* ez_in; const double* hx_in; const double* hy_in;
+  double* ez_out; double* hx_out; double* hy_out;
+  const double* ca;              // 0 off the E interior
+  const double* cb;
+  double ch, idx;
+  const int* src_cell;
+  const double* wave;
+  long long wave_stride_b;
+  int step0, ksteps, T, P;
+  const int* probe_cell;         // [P] a * pitch + c, or -1 outside the box
+  double* traces;                // [B][P][T]
+  float* traces32;
+  float* e_save;                 // if set: E after local step s (FP32) -> e_save + s * slot
+  long long slot;
+  int tiles_x, tiles_y, bpw, ntasks, xcd_order;
+};
+
+hipError_t launch_step2d_f64_tb(const TB2D64Args& a, int R, int K, hipStream_t s);
+
 // Physical-CPML mode (full grid, the reference single-run solver's step
 // order and convolutional PML); profiles per depth index.
 struct Cpml2DArgs {
@@ -251,6 +277,7 @@ struct MultiArgs {
   unsigned* done;                 // [nprob] finished items (zeroed before the launch)
   unsigned* status;               // set to 1 if a dependency wait ran out
   unsigned long long wait_ticks;  // real-time clock ticks before a wait gives up
+  int xcd_order;                  // per-round launches: contiguous item ranges per XCD
 };
 
 hipError_t launch_multi_tb(const MultiArgs& m, int poles, int R, int K, int blocks, hipStream_t s);

@@ -714,7 +714,10 @@ template <int R, int K, int POLES>
 __global__ __launch_bounds__(256) void round_tb_kernel(MultiArgs m, int seg0, int seg1,
                                                        long long first, long long count) {
   const int lane = threadIdx.x & 63;
-  const long long w = (long long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // neighbouring tiles of a problem are consecutive items: with xcd_order
+  // each XCD runs a contiguous range of them and shares their halo rows in L2
+  const long long w = (long long)xcd_block(m.xcd_order) * 4 +
+                      __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (w >= count) return;
   const long long i = first + w;
   int lo = seg0, hi = seg1 - 1;

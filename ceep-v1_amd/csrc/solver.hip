@@ -588,7 +588,39 @@ int prepare_f64(ceep_fdtd* h) {
 
 // Enqueue FP64 steps [step0, step0 + nsteps) from FP64 buffer `start`; with
 // e_save, E after step s goes (as FP32) to slot s + 1 - step0.
-int launch_range64(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) {
+int launch_range64(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save,
+                   int* end_buf) {
+  if (h->use_tb()) {   // blocked: 4 steps per launch on 32-row four-wavefront windows
+    constexpr int K = 4, R = 32;
+    ceep::TB2D64Args a{};
+    a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
+    a.ca = h->ca64; a.cb = h->cb64;
+    a.ch = (h->dt / kMu0) * (1.0 / h->dx);
+    a.idx = 1.0 / h->dx;
+    a.src_cell = h->src2; a.wave = h->wave64; a.wave_stride_b = h->wave_stride;
+    a.T = h->T; a.P = h->P; a.probe_cell = h->probe_cell;
+    a.traces = h->traces64; a.traces32 = h->traces;
+    a.slot = h->box_floats();
+    a.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
+    a.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
+    a.bpw = 2;
+    a.ntasks = a.tiles_x * a.tiles_y * ((h->B + a.bpw - 1) / a.bpw);
+    a.xcd_order = h->xcd_order;
+    int cur = start;
+    for (int st = step0; st < step0 + nsteps; st += K) {
+      const int o = cur ^ 1;
+      a.ez_in = h->f64[cur][0]; a.hx_in = h->f64[cur][1]; a.hy_in = h->f64[cur][2];
+      a.ez_out = h->f64[o][0]; a.hx_out = h->f64[o][1]; a.hy_out = h->f64[o][2];
+      a.step0 = st;
+      a.ksteps = std::min(K, step0 + nsteps - st);
+      a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
+      CEEP_HIP(ceep::launch_step2d_f64_tb(a, R, K, h->stream));
+      cur = o;
+    }
+    if (end_buf) *end_buf = cur;
+    return CEEP_OK;
+  }
+  if (end_buf) *end_buf = (start + nsteps) & 1;
   ceep::Step2D64Args a{};
   a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
   a.ca = h->ca64; a.cb = h->cb64;
@@ -1098,8 +1130,7 @@ int run_checkpointed64(ceep_fdtd* h, int nck, double* traces_out) {
       CEEP_HIP(ceep::launch_to_f32(h->f64[cur][0], h->estore, nbox, st));
       es = h->estore;
     }
-    if ((rc = launch_range64(h, cur, s0, n, es))) return rc;
-    cur ^= n & 1;
+    if ((rc = launch_range64(h, cur, s0, n, es, &cur))) return rc;
   }
   for (int f = 0; f < 3; ++f) CEEP_HIP(ceep::launch_to_f32(h->f64[cur][f], h->f[0][f], nbox, st));
   h->parity = 0;
@@ -1458,7 +1489,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       for (int f = 0; f < 3; ++f)
         if ((rc = copy64(h->ck64[(size_t)k * 3 + f], h->f64[0][f], (long long)nbox, st))) return rc;
       CEEP_HIP(ceep::launch_to_f32(h->ck64[(size_t)k * 3], h->estore, (long long)nbox, st));
-      if ((rc = launch_range64(h, 0, s0, n, h->estore))) return rc;
+      if ((rc = launch_range64(h, 0, s0, n, h->estore, nullptr))) return rc;
       h->estore_seg = k;
     } else if (k != h->estore_seg) {
       for (int f = 0; f < 3; ++f)
@@ -1755,6 +1786,7 @@ int multi_enqueue(ceep_fdtd_multi* m, int mode) {
   a.nseg = m->nseg; a.total = m->total;
   a.queue = m->d_queue; a.done = m->d_done; a.status = m->d_status;
   a.wait_ticks = 25000000ull;      // 0.25 s of the 100 MHz constant clock
+  a.xcd_order = m->hs[0]->xcd_order;   // key 9 of the first problem's handle
   if (mode == 0) {
     CEEP_HIP(ceep::launch_multi_tb(a, m->poles, m->R, m->K, m->blocks, s));
     return CEEP_OK;

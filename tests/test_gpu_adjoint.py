@@ -91,6 +91,24 @@ def test_adjoint_fp64_forward_matches_autograd(amd, reference, every):
     assert ee < 1e-4 and es < 1e-4
 
 
+@pytest.mark.parametrize("every", [50, 7, 240])
+def test_fp64_forward_blocked_equals_one_step(amd, reference, every):
+    """The blocked FP64 forward (4 steps per launch on four-wavefront
+    windows) reproduces the one-step FP64 kernel bit for bit, checkpoints and
+    segment-length remainders included."""
+    r = reference
+    out = {}
+    for depth in (0, 4):
+        s = _solver(amd, r["dt"], r["w"], r["pos"])
+        s._eps_r[:] = r["eps"]
+        s._sigma_e[:] = r["sig"]
+        s.tuning = {"temporal_depth": depth, "fp64_forward": 1}
+        s._build()
+        out[depth] = s._handle.run_checkpointed(every)
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[4])
+
+
 @pytest.mark.parametrize("tuning", [
     {"temporal_depth": 0},
     {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 4},

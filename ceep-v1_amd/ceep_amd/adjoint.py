@@ -14,8 +14,10 @@ ordinary torch on the host, so autograd supplies dL/dtrace.
 There is no counterpart in the reference (src/ceep/ai/__init__.py is a
 docstring; "Differentiable FDTD" is an unchecked roadmap item).
 
-``fp64_forward=True`` runs the forward and the segment recomputation on FP64
-fields (the adjoint stays FP32).  It is for misfits whose residual
+``fp64_forward=True`` runs the forward on FP64 fields. The adjoint stays FP32.
+It starts each segment from an FP32 rounding of the FP64 segment-end state
+(the reversible form), or recomputes the segment in FP64 from FP64
+checkpoints (the store form).  It is for misfits whose residual
 ``S - S_target`` is smaller than what FP32 traces resolve: with TX = RX probes
 the trace is dominated by the source pulse itself, and a scattered signal of
 1e-6 or less of it (the configuration-1 geometry at 400 steps: 5e-6; the

@@ -182,7 +182,7 @@ struct ceep_fdtd {
   // the temporally blocked 2D kernel covers the plain and Debye cases
   bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz && !cpml(); }
   // 2D gradient path: reversible reconstruction instead of recompute + E store
-  bool use_rev() const { return adj_rev && use_tb() && !fwd64; }
+  bool use_rev() const { return adj_rev && use_tb(); }
   int launches_per_run() const { return use_tb() ? (T + tb_k - 1) / tb_k : T; }
   long long full_index(int64_t x, int64_t y, int64_t z) const {
     return ndim == 2 ? x * N[1] + y : (x * N[1] + y) * N[2] + z;
@@ -1106,13 +1106,13 @@ namespace {
 // The FP64 forward of ceep_fdtd_run_checkpointed: FP64 checkpoints every
 // ck_every steps, the last segment's E kept (as FP32) in the store, FP64
 // traces; the final state is also left, rounded, in the FP32 fields.
-int run_checkpointed64(ceep_fdtd* h, int nck, double* traces_out) {
+int run_checkpointed64(ceep_fdtd* h, int nck, bool rev, double* traces_out) {
   int rc;
   const long long nbox = h->box_floats();
   const int every = h->ck_every;
   hipStream_t st = h->stream;
   if ((rc = prepare_f64(h))) return rc;
-  if ((int)h->ck64.size() != nck * 3) {
+  if (!rev && (int)h->ck64.size() != nck * 3) {
     for (auto& p : h->ck64) dfree(p);
     h->ck64.assign((size_t)nck * 3, nullptr);
     for (auto& p : h->ck64)
@@ -1122,23 +1122,36 @@ int run_checkpointed64(ceep_fdtd* h, int nck, double* traces_out) {
   CEEP_HIP(ceep::launch_zero(h->traces64, (size_t)h->B * h->P * h->T * sizeof(double), st));
   int cur = 0;
   for (int k = 0; k < nck; ++k) {
-    for (int f = 0; f < 3; ++f)
-      if ((rc = copy64(h->f64[cur][f], h->ck64[(size_t)k * 3 + f], nbox, st))) return rc;
+    // reversible adjoint: FP32 copies of the segment-end states (it never
+    // reads checkpoint 0); recompute form: FP64 checkpoints of segment starts
+    if (rev) {
+      if (k > 0)
+        for (int f = 0; f < 3; ++f)
+          CEEP_HIP(ceep::launch_to_f32(h->f64[cur][f], h->ck[(size_t)k * 3 + f], nbox, st));
+    } else {
+      for (int f = 0; f < 3; ++f)
+        if ((rc = copy64(h->f64[cur][f], h->ck64[(size_t)k * 3 + f], nbox, st))) return rc;
+    }
     const int s0 = k * every, n = std::min(every, h->T - s0);
     float* es = nullptr;
-    if (k == nck - 1) {
+    if (k == nck - 1 && !rev) {
       CEEP_HIP(ceep::launch_to_f32(h->f64[cur][0], h->estore, nbox, st));
       es = h->estore;
     }
     if ((rc = launch_range64(h, cur, s0, n, es, &cur))) return rc;
   }
-  for (int f = 0; f < 3; ++f) CEEP_HIP(ceep::launch_to_f32(h->f64[cur][f], h->f[0][f], nbox, st));
+  for (int f = 0; f < 3; ++f) {
+    CEEP_HIP(ceep::launch_to_f32(h->f64[cur][f], h->f[0][f], nbox, st));
+    if (rev)   // the final state: where the reversible adjoint starts
+      CEEP_HIP(ceep::launch_to_f32(h->f64[cur][f], h->ck[(size_t)nck * 3 + f], nbox, st));
+  }
   h->parity = 0;
   CEEP_HIP(hipStreamSynchronize(st));
   h->has_run = true;
   h->traces64_valid = true;
   h->ck_valid = true;
-  h->estore_seg = nck - 1;
+  h->ck_rev = rev;
+  h->estore_seg = rev ? -1 : nck - 1;
   if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
   return CEEP_OK;
 }
@@ -1344,7 +1357,10 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   h->ck_valid = false;
   // checkpoint buffers (FP32: one per segment start plus the final state; or
   // FP64 for the FP64 forward) are kept across calls
-  if (!h->fwd64 && h->ck.size() != (size_t)(nck + 1) * 3) {
+  const bool rev = h->use_rev();
+  // FP32 checkpoints serve the FP32 forward and, rounded from the FP64 state,
+  // the reversible adjoint of the FP64 forward
+  if ((!h->fwd64 || rev) && h->ck.size() != (size_t)(nck + 1) * 3) {
     for (auto& p : h->ck) dfree(p);
     h->ck.assign((size_t)(nck + 1) * 3, nullptr);
     for (auto& p : h->ck)
@@ -1356,12 +1372,11 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   h->estore_seg = -1;
   // the adjoint starts with the last segment: store its E during this pass so
   // that segment needs no recomputation (the reversible adjoint reads no store)
-  const bool rev = h->use_rev();
   if (!rev && !h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * h->box_floats())))
     return rc;
   if ((rc = zero_state(h))) return rc;
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), h->stream));
-  if (h->fwd64) return run_checkpointed64(h, nck, traces_out);
+  if (h->fwd64) return run_checkpointed64(h, nck, rev, traces_out);
   h->traces64_valid = false;
   int cur = 0;
   for (int k = 0; k < nck; ++k) {

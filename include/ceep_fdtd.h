@@ -153,17 +153,19 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *   key 3: 2D window rows of the blocked kernel (16, 24, 32 with one
  *          wavefront per window; 32, 48, 64 with four)
  *   key 4: 2D batch planes per window of the blocked kernel (>= 1)
- *   key 5: precision of the gradient path's forward and recomputation passes
- *          (0 = FP32, 1 = FP64; 2D non-dispersive): with 1,
- *          ceep_fdtd_run_checkpointed steps FP64 fields and returns FP64
- *          traces, for misfits whose residual lies below FP32 trace accuracy;
- *          the adjoint stays FP32.  Not a launch-shape knob: results differ.
+ *   key 5: precision of the gradient path's forward pass (0 = FP32, 1 =
+ *          FP64; 2D non-dispersive): with 1, ceep_fdtd_run_checkpointed steps
+ *          FP64 fields and returns FP64 traces, for misfits whose residual
+ *          lies below FP32 trace accuracy.  The adjoint stays FP32: the
+ *          reversible form (key 7) starts from FP32 roundings of the FP64
+ *          segment-end states, the recompute form re-steps each segment in
+ *          FP64.  Not a launch-shape knob: results differ.
  *   key 6: 3D step form: 0 = H kernel then E kernel in place; 1 = one
  *          fused H+E launch per step on ping-pong fields, workgroups ordered
  *          so each XCD walks contiguous x-slabs; 2 = fused, dispatch order;
  *          3 = fused, each thread marching along x over key-8 slabs.
  *          Switching between 0 and 1/2/3 clears the fields.
- *   key 7: 2D FP32 blocked adjoint form: 1 = reversible (the forward state is
+ *   key 7: 2D blocked adjoint form: 1 = reversible (the forward state is
  *          rebuilt backward inside the adjoint launches from the checkpoint at
  *          each segment's end; no recomputation, no E store; default),
  *          0 = recompute each segment into an E store.

@@ -1103,9 +1103,11 @@ int ceep_fdtd_time_runs(ceep_fdtd* h, int reps, double* ms_per_run) {
 
 namespace {
 
-// The FP64 forward of ceep_fdtd_run_checkpointed: FP64 checkpoints every
-// ck_every steps, the last segment's E kept (as FP32) in the store, FP64
-// traces; the final state is also left, rounded, in the FP32 fields.
+// The FP64 forward of ceep_fdtd_run_checkpointed, with FP64 traces.  For the
+// reversible adjoint it keeps FP32 roundings of the state at each segment end
+// (ck[1..nck]); for the recompute form, FP64 checkpoints every ck_every steps
+// and the last segment's E (as FP32) in the store.  The final state is also
+// left, rounded, in the FP32 fields.
 int run_checkpointed64(ceep_fdtd* h, int nck, bool rev, double* traces_out) {
   int rc;
   const long long nbox = h->box_floats();

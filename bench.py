@@ -45,6 +45,8 @@ HBM_PEAK_GBS = 8000.0
 # TX) and the shared ca, cb, k1, k2 maps read once per launch (16 B per cell)
 BYTES_PER_CELL_PER_TX = 32
 BYTES_PER_CELL_SHARED = 16
+BYTES3D_PER_CELL_PER_TX = 120   # 3D Debye + Lorentz: 15 state floats read + written
+BYTES3D_SHARED = 28             # ca, cb, two Debye and three Lorentz coefficient maps
 
 
 def _workload():
@@ -198,7 +200,7 @@ def _adjoint_ratio(every=50):
                          "L = sum |S - S_target|^2 at 16 freqs in 1-2.5 GHz; dL/deps_r, dL/dsigma")}
 
 
-def _bench_3d(steps_timed=2, T3=1000):
+def _bench_3d(steps_timed=2, T3=1000, tuning=None):
     """3D 200^3, dx = 1 mm, 10-cell CPML, 16 z-directed sources = probes on a
     70-cell ring at z = 100, Debye + Lorentz head (seed 0), 1000 steps."""
     import ceep_amd
@@ -215,6 +217,8 @@ def _bench_3d(steps_timed=2, T3=1000):
                                source_positions=pos, probe_positions=pos, frequency=2e9,
                                waveform=w)
     s.set_phantom(ph)
+    if tuning is not None:
+        s.tuning = tuning
     s._build()
     h = s._handle
     box = h.box_info()
@@ -222,11 +226,26 @@ def _bench_3d(steps_timed=2, T3=1000):
     h.time_runs(1)
     ms = h.time_runs(steps_timed)
     s.close()
+    form = (tuning or {}).get("fused_3d", 4)
+    k3 = (tuning or {}).get("temporal_depth_3d", 2) if form == 4 else 1
+    launches = -(-T3 // k3)
+    # algorithmic bytes per step: the 15-float state (E, H, Debye J, Lorentz J
+    # and J one step older) read and written once per cell and TX, the 7 maps
+    # once per cell; a K-step launch is credited K steps
+    bytes_launch = cells * (16 * BYTES3D_PER_CELL_PER_TX + BYTES3D_SHARED) * k3
+    achieved = bytes_launch / (ms / launches * 1e-3) / 1e9
+    kernel = f"step3d_tb_kernel<{k3}, 3, false>" if form == 4 else "h3d_kernel + e3d_kernel"
     return {"value": round(cells * 16 * T3 / (ms * 1e-3) / 1e9, 3), "unit": "GCell-steps/s",
             "ms_per_run": round(ms, 2), "cells_counted": f"stepped box {box[0]}x{box[1]}x{box[2]}",
             "value_full_grid_convention": round(n3 ** 3 * 16 * T3 / (ms * 1e-3) / 1e9, 3),
+            "roofline": {"bound": "hbm", "kernel": kernel, "steps_per_launch": k3,
+                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": _traffic(kernel, "step3d_debye_lorentz_200x200x200x16_traffic.json"),
+                         "algorithmic_bytes_per_launch": bytes_launch,
+                         "launch_ms": round(ms / launches, 5)},
             "workload": ("3D 200^3 dx=1mm, 10-cell CPML, 16 Ez TX=RX on r=70 ring at z=100, "
-                         "1000 steps, Debye+Lorentz head seed 0 (H and E kernels per step)")}
+                         f"1000 steps, Debye+Lorentz head seed 0 ({kernel}, {k3} steps per launch)")}
 
 
 def _bench_uneven(n_problems=64, seed=4, autotune=True):
@@ -258,7 +277,7 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
         s._build()
         solvers.append(s)
         grids.append(g)
-    shape, log = (4, 16, 4), []       # the tuner's usual pick on MI355X
+    shape, log = (4, 16, 4, 1), []    # K, window rows, TX planes per window, wavefronts
     if autotune:
         shape, log = autotune_uneven(solvers)
     ub = UnevenBatch(solvers, *shape)
@@ -300,7 +319,7 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
             "sequential_one_step_ms": round(t_s * 1e3, 2),
             "speedup_vs_sequential": round(t_s / t_b, 3),
             "launch_shape": {"temporal_depth": shape[0], "rows_per_wave": shape[1],
-                             "batch_per_wave": shape[2]},
+                             "batch_per_wave": shape[2], "wavefronts_per_window": shape[3]},
             "autotune_log_ms": [[round(ms, 2), list(sh)] for ms, sh in sorted(log)],
             "work_items": info["items"], "segments": info["segments"],
             "cell_steps": cell_steps,
@@ -378,10 +397,10 @@ def _bench_3d_gradient(T3=1000, every=100):
                          "dL/deps_r and dL/dsigma over the full volume")}
 
 
-def _traffic(kernel):
+def _traffic(kernel, summary="step2d_debye_600x600x16_traffic.json"):
     """Measured HBM bytes per launch from the committed rocprofv3 counter
     summary, if it was taken for the kernel this run uses (else None)."""
-    path = os.path.join(ROOT, "profiles", "step2d_debye_600x600x16_traffic.json")
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     with open(path) as f:

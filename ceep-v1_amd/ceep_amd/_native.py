@@ -58,7 +58,7 @@ SIGNATURES = {
     "ceep_fdtd_run_checkpointed": (_I, [_P, _I, _PD]),
     "ceep_fdtd_adjoint": (_I, [_P, _PD, _PD, _PD]),
     "ceep_fdtd_set_tuning": (_I, [_P, _I, _I]),
-    "ceep_fdtd_multi_create": (_I, [_I, ctypes.POINTER(_P), _I, _I, _I, ctypes.POINTER(_P)]),
+    "ceep_fdtd_multi_create": (_I, [_I, ctypes.POINTER(_P), _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "ceep_fdtd_multi_run": (_I, [_P, _I]),
     "ceep_fdtd_multi_time": (_I, [_P, _I, _I, _PD]),
     "ceep_fdtd_multi_info": (_I, [_P, ctypes.POINTER(ctypes.c_longlong)]),
@@ -139,6 +139,8 @@ TUNE_XCD_3D = 12
 TUNE_ADJ_WAVES = 14
 TUNE_ADJ_DEPTH = 15
 TUNE_FWD_WAVES = 16
+TUNE_TB3D_DEPTH = 17
+TUNE_TX_GROUP_3D = 18
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
@@ -147,7 +149,8 @@ TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "adjoint_rows_per_wave": TUNE_ADJ_ROWS, "adjoint_batch_per_wave": TUNE_ADJ_BPW,
                "xcd_order_3d": TUNE_XCD_3D,
                "adjoint_waves": TUNE_ADJ_WAVES,
-               "adjoint_temporal_depth": TUNE_ADJ_DEPTH, "forward_waves": TUNE_FWD_WAVES}
+               "adjoint_temporal_depth": TUNE_ADJ_DEPTH, "forward_waves": TUNE_FWD_WAVES,
+               "temporal_depth_3d": TUNE_TB3D_DEPTH, "tx_group_3d": TUNE_TX_GROUP_3D}
 
 
 class Handle:
@@ -294,13 +297,13 @@ class Handle:
 class MultiHandle:
     """Owns one ``ceep_fdtd_multi*`` grouping several 2D ``Handle``s."""
 
-    def __init__(self, handles, K, R, bpw):
+    def __init__(self, handles, K, R, bpw, waves=1):
         self._lib = load()
         self._m = _P()
         self._handles = list(handles)           # keep the members alive
         arr = (_P * len(handles))(*[h._h for h in handles])
         check(self._lib.ceep_fdtd_multi_create(len(handles), arr, int(K), int(R), int(bpw),
-                                               ctypes.byref(self._m)))
+                                               int(waves), ctypes.byref(self._m)))
 
     PERSISTENT = 0
     PER_ROUND = 1

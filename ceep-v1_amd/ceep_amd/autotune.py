@@ -103,12 +103,17 @@ def autotune_2d(solver, reps: int = 2, candidates=None, verbose: bool = False) -
 
 
 def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 32),
-                    batch_per_wave=(1, 2, 4), start=(4, 16, 2), verbose: bool = False):
-    """Launch shape (temporal depth K, rows per wavefront R, TX planes per
-    wavefront) for an ``UnevenBatch`` of ``solvers``, by coordinate search:
-    K over ``depths`` at the starting R and planes, then R at the best K, then
-    the planes at the best (K, R).  Each candidate is a captured graph of the
-    whole batch, timed with HIP events after one warm replay.
+                    batch_per_wave=(1, 2, 4), start=(4, 16, 2),
+                    four_wave=((4, 32, 1), (4, 32, 2), (4, 32, 4), (4, 48, 2), (6, 48, 1),
+                               (6, 48, 2), (6, 48, 4)),
+                    verbose: bool = False):
+    """Launch shape (temporal depth K, window rows R, TX planes per window,
+    wavefronts per window) for an ``UnevenBatch`` of ``solvers``: coordinate
+    search over one-wavefront windows (K over ``depths`` at the starting R and
+    planes, then R at the best K, then the planes at the best (K, R)), then
+    the ``four_wave`` shapes (K, R, planes) with four wavefronts per window.
+    Each candidate is a captured graph of the whole batch, timed with HIP
+    events after one warm replay.
 
     Returns ``(best_shape, log)`` with ``log`` a list of ``(ms, shape)``.
     """
@@ -120,10 +125,10 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
     def measure(shape):
         if shape in timed:
             return timed[shape]
-        k, r, b = shape
+        k, r, b, waves = shape
         if r - 2 * k < 4:
             return float("inf")
-        ub = UnevenBatch(solvers, k, r, b)
+        ub = UnevenBatch(solvers, k, r, b, waves)
         try:
             ms = ub.time(reps)
         finally:
@@ -131,10 +136,10 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
         timed[shape] = ms
         log.append((ms, shape))
         if verbose:
-            print(f"  {ms:9.3f} ms  K={k} R={r} planes/wave={b}", flush=True)
+            print(f"  {ms:9.3f} ms  K={k} R={r} planes/window={b} wavefronts={waves}", flush=True)
         return ms
 
-    best = tuple(start)
+    best = tuple(start) + (1,)
     for axis, values in ((0, depths), (1, rows), (2, batch_per_wave)):
         for v in values:
             cand = list(best)
@@ -142,6 +147,10 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
             cand = tuple(cand)
             if measure(cand) < measure(best):
                 best = cand
+    for k, r, b in four_wave:
+        cand = (k, r, b, 4)
+        if measure(cand) < measure(best):
+            best = cand
     for s in solvers:
         s._handle.reset()
     return best, log

@@ -15,16 +15,20 @@ from . import _native
 
 class UnevenBatch:
     def __init__(self, solvers, temporal_depth: int = 4, rows_per_wave: int = 16,
-                 batch_per_wave: int = 1):
+                 batch_per_wave: int = 1, waves: int = 1):
+        """Launch shape: K = temporal_depth steps per round on windows of
+        rows_per_wave rows x 64 columns, batch_per_wave TX planes per window;
+        waves = 1 (one wavefront per window; K 1..4, rows 16/24/32) or 4 (four
+        wavefronts per window; K 4 or 6, rows 32 or 48)."""
         self.solvers = list(solvers)
         for s in self.solvers:
             if not s._built:
                 s._build()
             else:
                 s._handle.reset()
-        self.shape = (temporal_depth, rows_per_wave, batch_per_wave)
+        self.shape = (temporal_depth, rows_per_wave, batch_per_wave, waves)
         self._multi = _native.MultiHandle([s._handle for s in self.solvers], temporal_depth,
-                                          rows_per_wave, batch_per_wave)
+                                          rows_per_wave, batch_per_wave, waves)
 
     @property
     def cell_steps(self) -> int:

@@ -1138,14 +1138,12 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
 // register file: the loads per output cell-step drop from
 // (R/(R-2K) (64/(64-2K)) + 1)/K = 0.82 at R = 16, K = 4 to 0.38 at R = 48,
 // K = 8.  Per-cell arithmetic is that of step2d_tb_kernel.
+// One task (output tile, TX chunk) advanced p.ksteps steps by the four
+// wavefronts of the calling workgroup; x_e / x_h are its LDS exchange rows.
 template <int RW, int K, int POLES>
-__global__ __launch_bounds__(256) void step2d_mw_kernel(TB2DArgs p) {
+__device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, int w,
+                                        float (*x_e)[64], float (*x_h)[64]) {
   constexpr int R = 4 * RW;
-  __shared__ float x_e[4][64], x_h[4][64];
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int task = xcd_block(p.xcd_order);
-  if (task >= p.ntasks) return;                       // uniform per workgroup
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = task % tiles;
   const int chunk = task / tiles;
@@ -1329,6 +1327,73 @@ __global__ __launch_bounds__(256) void step2d_mw_kernel(TB2DArgs p) {
       }
     }
   }
+}
+
+template <int RW, int K, int POLES>
+__global__ __launch_bounds__(256) void step2d_mw_kernel(TB2DArgs p) {
+  __shared__ float x_e[4][64], x_h[4][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int task = xcd_block(p.xcd_order);
+  if (task >= p.ntasks) return;                       // uniform per workgroup
+  mw_task<RW, K, POLES>(p, task, lane, w, x_e, x_h);
+}
+
+// Uneven batch, one launch per round, four wavefronts per window: each
+// workgroup is one work item (problem, tile, TX chunk) of the round.
+template <int RW, int K, int POLES>
+__global__ __launch_bounds__(256) void round_mw_kernel(MultiArgs m, int seg0, int seg1,
+                                                       long long first, long long count) {
+  __shared__ float x_e[4][64], x_h[4][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long wi = xcd_block(m.xcd_order);
+  if (wi >= count) return;                            // uniform per workgroup
+  const long long i = first + wi;
+  int lo = seg0, hi = seg1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.seg_start[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const int pidx = m.seg_prob[lo];
+  const int round = m.seg_round[lo];
+  const int task = (int)(i - m.seg_start[lo]);
+  const MultiDesc& d = m.probs[pidx];
+  TB2DArgs a = d.a;
+  const int cur = round & 1, nxt = cur ^ 1;
+  a.ez_in = d.f[cur][0]; a.hx_in = d.f[cur][1]; a.hy_in = d.f[cur][2];
+  a.ez_out = d.f[nxt][0]; a.hx_out = d.f[nxt][1]; a.hy_out = d.f[nxt][2];
+  a.jd_in = d.jd[cur]; a.jd_out = d.jd[nxt];
+  a.step0 = round * K;
+  a.ksteps = min(K, a.T - a.step0);
+  a.e_save = nullptr;
+  mw_task<RW, K, POLES>(a, task, lane, w, x_e, x_h);
+}
+
+template <int RW, int K>
+static hipError_t launch_round_mw_rk(const MultiArgs& m, int poles, int seg0, int seg1,
+                                     long long first, long long count, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((unsigned)count, 1, 1);
+  if (poles == 0) hipLaunchKernelGGL((round_mw_kernel<RW, K, 0>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  else if (poles == 1) hipLaunchKernelGGL((round_mw_kernel<RW, K, 1>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
+                           long long first, long long count, hipStream_t s) {
+  if (count < 1) return hipSuccess;
+  if (count > 0x7fffffffLL) return hipErrorInvalidValue;
+#define CEEP_RMW(RW)                                                                         \
+  switch (K) {                                                                               \
+    case 4: return launch_round_mw_rk<RW, 4>(m, poles, seg0, seg1, first, count, s);          \
+    case 6: return launch_round_mw_rk<RW, 6>(m, poles, seg0, seg1, first, count, s);          \
+  }                                                                                          \
+  return hipErrorInvalidValue;
+  if (R == 32) { CEEP_RMW(8) }
+  if (R == 48) { CEEP_RMW(12) }
+#undef CEEP_RMW
+  return hipErrorInvalidValue;
 }
 
 template <int RW, int K>

@@ -143,6 +143,39 @@ hipError_t launch_step3d_fused(const Step3DFArgs& a, int poles, int bb, hipStrea
 // previous slab in registers, so no slab is read twice from HBM.
 hipError_t launch_step3d_march(const Step3DFArgs& a, int poles, int bb, hipStream_t s);
 
+// Temporally blocked 3D step: K = 1 or 2 full steps per launch (H, E + ADE,
+// Ez injection, Ez probes, optional E store per step) on 16 x 64 (y, z)
+// windows that march along x.  The state lives in two sets: each field
+// component is one allocation of two nbox-float sets, the ADE currents one
+// packed array [set][B][cell][nj] (Debye J x, y, z; Lorentz J x, y, z;
+// Lorentz J one step older x, y, z; absent poles omitted).  A launch reads
+// set in_set and writes the other.
+struct TB3DArgs {
+  int Mx, My, Mz, pitch, B;
+  long long sx, bstride;          // floats per x-slab, per TX volume
+  long long nbox;                 // floats per field set (B * bstride)
+  int in_set;
+  float* e[3]; float* h[3];       // set-0 bases
+  float* jp;                      // packed ADE currents (null without poles)
+  const float* mp;                // packed maps [cell][9]: ca, cb, dk1, dk2dt, lk1, lk2dt, lk3, da, dbi
+  int magnetic;                   // da / dbi vary per cell (else da = 1, dbi = ch)
+  float ch, idx;
+  const long long* src_cell;
+  const float* wave;
+  long long wave_stride_b;
+  int step, T, P;                 // step: the launch's first step
+  const short* probe_head;
+  const short* probe_next;
+  float* traces;
+  float* e_save;                  // E after the first step ([3][nbox]); the next step's 3 * nbox later
+  int tiles_y, tiles_z;           // ceil((My-1)/(16-2K)), ceil((Mz-1)/(64-2K))
+  int order;                      // workgroup order: 0 dispatch (y tiles fastest), 1 per XCD, 2 per XCD TX
+                                  // fastest, 3 per XCD in groups of tx_group TX, then y tiles
+  int tx_group;
+};
+
+hipError_t launch_step3d_tb(const TB3DArgs& a, int K, int poles, hipStream_t s);
+
 hipError_t launch_h3d(const Step3DArgs& a, int bb, hipStream_t s);
 hipError_t launch_e3d(const Step3DArgs& a, int poles, int bb, hipStream_t s);
 

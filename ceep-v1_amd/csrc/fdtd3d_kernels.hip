@@ -13,7 +13,9 @@
 // renumbered so each XCD walks a contiguous run of slabs and the +-1-slab
 // neighbour reads hit its L2.  Every thread issues all loads of its BB
 // planes before it computes or stores anything: these kernels are bound by
-// HBM, and the loads in flight per wave are what feeds it.
+// HBM, and the loads in flight per wave are what feeds it.  Floating-point
+// contraction is off, so the other 3D step forms reproduce these results bit
+// for bit from the same expressions.
 #include "fdtd_common.h"
 #include "fdtd3d_kernels.h"
 
@@ -56,6 +58,7 @@ __device__ __forceinline__ Block3D block3d(const Step3DArgs& p, int bb) {
 
 template <bool MAGNETIC, int BB>
 __global__ __launch_bounds__(256) void h3d_kernel(Step3DArgs p) {
+#pragma clang fp contract(off)
   const Block3D w = block3d(p, BB);
   const int a = w.a, b = w.b, d = w.d;
   if (b >= p.My || d >= p.Mz) return;
@@ -102,6 +105,7 @@ __global__ __launch_bounds__(256) void h3d_kernel(Step3DArgs p) {
 
 template <int POLES, int BB>
 __global__ __launch_bounds__(256) void e3d_kernel(Step3DArgs p) {
+#pragma clang fp contract(off)
   const Block3D w = block3d(p, BB);
   const int a = w.a, b = w.b, d = w.d;
   if (b >= p.My || d >= p.Mz) return;

@@ -100,6 +100,12 @@ struct ceep_fdtd {
   float* jl[2][3] = {{nullptr}};
   float *ca = nullptr, *cb = nullptr, *da = nullptr, *dbi = nullptr;
   float *dk1 = nullptr, *dk2 = nullptr, *lk1 = nullptr, *lk2 = nullptr, *lk3 = nullptr;
+  // 3D blocked form: per-cell packed maps [cell][9] (ca, cb, dk1, dk2dt, lk1,
+  // lk2dt, lk3, da, dbi) and ADE currents [set][B][cell][nj] (Debye J x, y, z;
+  // Lorentz J x, y, z; Lorentz J one step older x, y, z; absent poles omitted)
+  float* mpack = nullptr;
+  float* jpack = nullptr;
+  int nj() const { return (has_debye ? 3 : 0) + (has_lorentz ? 6 : 0); }
   int* src2 = nullptr;
   long long* src3 = nullptr;
   float* wave = nullptr;
@@ -147,7 +153,9 @@ struct ceep_fdtd {
   int tb_k = 0;                // 2D temporal depth (0: one step per launch)
   int tb_r = 16;               // 2D rows per wavefront window
   int tb_bpw = 1;              // 2D batch planes per wavefront
-  int fused3d = 0;             // 3D: 1/2 = one fused H+E launch per step, 3 = marching form (ping-pong fields)
+  int fused3d = 4;             // 3D step form: 0 = H then E kernels in place; 1/2 = one fused H+E launch
+                               // per step, 3 = its marching form (ping-pong fields); 4 (default) = the
+                               // temporally blocked kernel, tb3d_k steps per launch (the whole state ping-pongs)
   int march_len = 24;          // 3D marching form: x-slabs per workgroup
   int xcd_order = 1;           // 2D blocked kernels: contiguous tile ranges per XCD
   int adj_r = 0;               // 2D blocked adjoint: window rows (0: 32 for the four-wavefront
@@ -161,6 +169,8 @@ struct ceep_fdtd {
                                // four-wavefront reversible kernel, 4 otherwise)
   int fwd_waves = 1;           // 2D blocked forward: wavefronts per window (1 or 4)
   int xcd3d = 0;               // 3D H/E kernels: block order (0 dispatch, 1/2 per XCD)
+  int tb3d_k = 2;              // 3D blocked form: steps per launch (1 or 2)
+  int tx_group3d = 4;          // 3D blocked form, block order 3: TX per group
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
@@ -179,11 +189,17 @@ struct ceep_fdtd {
   // the two-kernel in-place update)
   int fbuf() const { return cpml() ? 0 : (ndim == 2 || fused3d) ? parity : 0; }
   int poles() const { return (has_debye ? 1 : 0) | (has_lorentz ? 2 : 0); }
+  // 3D temporally blocked form: every state array holds two sets (fields
+  // f[0]/f[1], J at +0 / +box_floats()); set `parity` is live, and each set
+  // keeps its Lorentz J as (jl[0] = newest, jl[1] = one step older)
+  bool tb3d() const { return ndim == 3 && !cpml() && fused3d == 4; }
   // the temporally blocked 2D kernel covers the plain and Debye cases
   bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz && !cpml(); }
   // 2D gradient path: reversible reconstruction instead of recompute + E store
   bool use_rev() const { return adj_rev && use_tb(); }
-  int launches_per_run() const { return use_tb() ? (T + tb_k - 1) / tb_k : T; }
+  int launches_per_run() const {
+    return use_tb() ? (T + tb_k - 1) / tb_k : tb3d() ? (T + tb3d_k - 1) / tb3d_k : T;
+  }
   long long full_index(int64_t x, int64_t y, int64_t z) const {
     return ndim == 2 ? x * N[1] + y : (x * N[1] + y) * N[2] + z;
   }
@@ -316,8 +332,29 @@ int commit_materials(ceep_fdtd* h) {
         (rc = upload_f32(h->lk3, lk3, s)))
       return rc;
   }
-  // ADE current buffers follow the pole set
-  const int ncomp = h->ndim == 2 ? 1 : 3;
+  // 3D blocked form: packed maps, and the ADE currents packed per cell in two sets
+  dfree(h->mpack);
+  dfree(h->jpack);
+  if (h->tb3d()) {
+    std::vector<float> mp((size_t)nbox * 9, 0.f);
+    for (long long o = 0; o < nbox; ++o) {
+      float* q = &mp[(size_t)o * 9];
+      q[0] = ca[o]; q[1] = cb[o];
+      if (h->has_debye) { q[2] = dk1[o]; q[3] = dk2[o]; }
+      if (h->has_lorentz) { q[4] = lk1[o]; q[5] = lk2[o]; q[6] = lk3[o]; }
+      q[7] = da.empty() ? 1.f : da[o];
+      q[8] = da.empty() ? h->ch : dbi[o];
+    }
+    if ((rc = dalloc(&h->mpack, mp.size())) || (rc = upload_f32(h->mpack, mp, s))) return rc;
+    CEEP_HIP(hipStreamSynchronize(s));   // mp goes out of scope
+    const size_t nj = (size_t)h->nj();
+    if (nj) {
+      if ((rc = dalloc(&h->jpack, 2 * nj * h->box_floats()))) return rc;
+      CEEP_HIP(hipMemsetAsync(h->jpack, 0, 2 * nj * h->box_floats() * sizeof(float), s));
+    }
+  }
+  // ADE current buffers follow the pole set (the blocked 3D form uses jpack)
+  const int ncomp = h->tb3d() ? 0 : h->ndim == 2 ? 1 : 3;
   for (int k = 0; k < 3; ++k) {
     dfree(h->jd[k]);
     dfree(h->jl[0][k]);
@@ -350,10 +387,8 @@ int zero_state(ceep_fdtd* h) {
   hipStream_t s = h->stream;
   const size_t bytes = (size_t)h->box_floats() * sizeof(float);
   const int nf = h->ndim == 2 ? 3 : 6;
-  for (int k = 0; k < nf; ++k) {
-    CEEP_HIP(hipMemsetAsync(h->f[0][k], 0, bytes, s));
-    CEEP_HIP(hipMemsetAsync(h->f[1][k], 0, bytes, s));
-  }
+  for (int k = 0; k < nf; ++k)   // f[1][k] = f[0][k] + box_floats(): one clear per pair
+    CEEP_HIP(hipMemsetAsync(h->f[0][k], 0, 2 * bytes, s));
   for (int k = 0; k < 12; ++k)
     if (h->psi[k]) CEEP_HIP(hipMemsetAsync(h->psi[k], 0, bytes, s));
   for (int k = 0; k < 3; ++k) {
@@ -361,11 +396,18 @@ int zero_state(ceep_fdtd* h) {
     if (h->jl[0][k]) CEEP_HIP(hipMemsetAsync(h->jl[0][k], 0, bytes, s));
     if (h->jl[1][k]) CEEP_HIP(hipMemsetAsync(h->jl[1][k], 0, bytes, s));
   }
+  if (h->jpack) CEEP_HIP(hipMemsetAsync(h->jpack, 0, 2 * h->nj() * bytes, s));
   h->parity = 0;
   return CEEP_OK;
 }
 
 int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save);
+
+// The buffer set (ping-pong parity) live after launch_range(h, start, ., nsteps):
+// the 3D blocked form flips it once per launch (tb3d_k steps), the other forms once per step.
+int range_end(const ceep_fdtd* h, int start, int nsteps) {
+  return start ^ ((h->tb3d() ? (nsteps + h->tb3d_k - 1) / h->tb3d_k : nsteps) & 1);
+}
 
 // CPML profiles of CPML._compute_pml_params (absorbing.py:266-311) in FP64,
 // rounded once: b, c at the E (integer) and H (half-integer) depths of a slab
@@ -502,6 +544,37 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
       a.step = st;
       CEEP_HIP(ceep::launch_cpml3d(a, poles, s));
+    }
+  } else if (h->tb3d()) {
+    // two steps per launch (one for an odd remainder), set `cur` -> the other
+    ceep::TB3DArgs t{};
+    t.Mx = h->M[0]; t.My = h->M[1]; t.Mz = h->M[2]; t.pitch = h->pitch; t.B = h->B;
+    t.sx = h->row_block; t.bstride = h->plane; t.nbox = h->box_floats();
+    if ((t.nbox + t.bstride) * 4 >= (1LL << 31))
+      return fail(CEEP_EUNSUPPORTED, "3D blocked form: batch x volume above 2^29 floats; use step form 0");
+    if ((long long)t.bstride * 9 * 4 >= (1LL << 31))
+      return fail(CEEP_EUNSUPPORTED, "3D blocked form: volume per TX above 2^31 / 36 cells; use step form 0");
+    for (int k = 0; k < 3; ++k) { t.e[k] = h->f[0][k]; t.h[k] = h->f[0][3 + k]; }
+    t.jp = h->jpack;
+    t.mp = h->mpack;
+    t.magnetic = h->da != nullptr;
+    t.ch = h->ch; t.idx = h->idx;
+    t.src_cell = h->src3; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
+    t.T = h->T; t.P = h->P;
+    t.probe_head = h->probe_head; t.probe_next = h->probe_next; t.traces = h->traces;
+    t.order = h->xcd3d;
+    t.tx_group = std::min(h->tx_group3d, h->B);
+    int cur = start;
+    for (int st = step0; st < step_end;) {
+      const int K = std::min(h->tb3d_k, step_end - st);
+      t.in_set = cur;
+      t.step = st;
+      t.tiles_y = (t.My - 1 + (16 - 2 * K) - 1) / (16 - 2 * K);
+      t.tiles_z = (t.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K);
+      t.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * 3 * h->box_floats() : nullptr;
+      CEEP_HIP(ceep::launch_step3d_tb(t, K, poles, s));
+      cur ^= 1;
+      st += K;
     }
   } else {
     ceep::Step3DArgs a{};
@@ -771,9 +844,9 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
     return fail(CEEP_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   const int nf = ndim == 2 ? 3 : 6;
-  for (int k = 0; k < nf && !rc; ++k) {
-    rc = dalloc(&h->f[0][k], h->box_floats());
-    if (!rc) rc = dalloc(&h->f[1][k], h->box_floats());
+  for (int k = 0; k < nf && !rc; ++k) {   // the two ping-pong sets of a component are adjacent
+    rc = dalloc(&h->f[0][k], 2 * h->box_floats());
+    if (!rc) h->f[1][k] = h->f[0][k] + h->box_floats();
   }
   for (int k = 0; k < (ndim == 2 ? 4 : 12) && !rc && h->cpml(); ++k) rc = dalloc(&h->psi[k], h->box_floats());
   if (!rc) rc = dalloc(&h->ca, h->plane);
@@ -810,9 +883,12 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   h->drop_graph();
   h->free_adjoint();
-  for (int i = 0; i < 2; ++i)
-    for (int k = 0; k < 6; ++k) dfree(h->f[i][k]);
+  for (int k = 0; k < 6; ++k) {
+    dfree(h->f[0][k]);   // f[1][k] lives in the same allocation
+    h->f[1][k] = nullptr;
+  }
   for (int k = 0; k < 3; ++k) { dfree(h->jd[k]); dfree(h->jl[0][k]); dfree(h->jl[1][k]); }
+  dfree(h->mpack); dfree(h->jpack);
   dfree(h->ca); dfree(h->cb); dfree(h->da); dfree(h->dbi);
   dfree(h->dk1); dfree(h->dk2); dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
   dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next); dfree(h->probe_cell); dfree(h->probe_cell3);
@@ -1164,7 +1240,12 @@ std::vector<float*> state3d(ceep_fdtd* h, int par) {
   std::vector<float*> v;
   const int fb = h->fused3d ? par : 0;
   for (int k = 0; k < 6; ++k) v.push_back(h->f[fb][k]);
-  for (int k = 0; k < 3; ++k) {
+  if (h->tb3d()) {   // set `par` of the packed currents, as nj chunks of box_floats()
+    const long long nbox = h->box_floats();
+    for (int j = 0; j < h->nj(); ++j) v.push_back(h->jpack + ((long long)par * h->nj() + j) * nbox);
+    return v;
+  }
+  for (int k = 0; k < 3; ++k) {   // J in place (Lorentz roles by step parity)
     if (h->has_debye) v.push_back(h->jd[k]);
     if (h->has_lorentz) { v.push_back(h->jl[0][k]); v.push_back(h->jl[1][k]); }
   }
@@ -1197,9 +1278,10 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
   if ((rc = zero_state(h))) return rc;
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), st));
   h->traces64_valid = false;
+  int cur = 0;
   for (int k = 0; k < nck; ++k) {
     const int s0 = k * every, n = std::min(every, h->T - s0);
-    const std::vector<float*> live = state3d(h, s0 & 1);
+    const std::vector<float*> live = state3d(h, cur);
     for (size_t i = 0; i < nstate; ++i)
       CEEP_HIP(ceep::launch_copy(live[i], h->ck[(size_t)k * nstate + i], nbox, st));
     float* es = nullptr;
@@ -1208,9 +1290,10 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
         CEEP_HIP(ceep::launch_copy(live[c], h->estore + (size_t)c * nbox, nbox, st));
       es = h->estore;
     }
-    if ((rc = launch_range(h, s0 & 1, s0, n, es))) return rc;
+    if ((rc = launch_range(h, cur, s0, n, es))) return rc;
+    cur = range_end(h, cur, n);
   }
-  h->parity = h->T & 1;
+  h->parity = cur;
   CEEP_HIP(hipStreamSynchronize(st));
   h->has_run = true;
   h->ck_valid = true;
@@ -1306,13 +1389,16 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
   const size_t slot = (size_t)3 * nbox;
   for (int k = h->ck_count - 1; k >= 0; --k) {
     const int s0 = k * every, n = std::min(every, T - s0);
-    const std::vector<float*> live = state3d(h, s0 & 1);
+    // the one-step forms keep Lorentz J roles by step parity; the blocked
+    // form's sets carry their roles, so it restarts from set 0
+    const int par = h->tb3d() ? 0 : (s0 & 1);
+    const std::vector<float*> live = state3d(h, par);
     if (k != h->estore_seg) {   // recompute the segment from its checkpoint, storing E
       for (size_t i = 0; i < nstate; ++i)
         CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + i], live[i], nbox, st));
       for (int c = 0; c < 3; ++c)
         CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + c], h->estore + (size_t)c * nbox, nbox, st));
-      if ((rc = launch_range(h, s0 & 1, s0, n, h->estore))) return rc;
+      if ((rc = launch_range(h, par, s0, n, h->estore))) return rc;
       h->estore_seg = k;
     }
     for (int s = s0 + n - 1; s >= s0; --s) {
@@ -1625,12 +1711,18 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 1 || value > 5)
       return fail(CEEP_EINVAL, "adjoint temporal depth must be in [1, 5] (5: four-wavefront reversible form only)");
     h->adj_k = value;
+  } else if (key == 17) {
+    if (value != 1 && value != 2) return fail(CEEP_EINVAL, "3D blocked steps per launch must be 1 or 2");
+    h->tb3d_k = value;
   } else if (key == 16) {
     if (value != 1 && value != 4) return fail(CEEP_EINVAL, "forward wavefronts per window must be 1 or 4");
     h->fwd_waves = value;
   } else if (key == 12) {
-    if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D block order must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(CEEP_EINVAL, "3D block order must be 0, 1, 2 or 3");
     h->xcd3d = value;
+  } else if (key == 18) {
+    if (value < 1 || value > 64) return fail(CEEP_EINVAL, "3D TX group must be in [1, 64]");
+    h->tx_group3d = value;
   } else if (key == 9) {
     if (value != 0 && value != 1) return fail(CEEP_EINVAL, "XCD tile order must be 0 or 1");
     h->xcd_order = value;
@@ -1642,10 +1734,13 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value != h->adj_rev) h->ck_valid = false;
     h->adj_rev = value;
   } else if (key == 6) {
-    if (value < 0 || value > 3) return fail(CEEP_EINVAL, "3D step form must be 0, 1, 2 or 3");
-    if ((value != 0) != (h->fused3d != 0)) {   // the state's buffer convention changes: start over
+    if (value < 0 || value > 4) return fail(CEEP_EINVAL, "3D step form must be 0, 1, 2, 3 or 4");
+    // buffer convention: 0 in place, 1-3 ping-pong fields, 4 ping-pong state
+    auto conv = [](int f) { return f == 0 ? 0 : f == 4 ? 2 : 1; };
+    if (conv(value) != conv(h->fused3d)) {   // the state's buffer convention changes: start over
       h->fused3d = value;
       h->ck_valid = false;
+      h->mats_dirty = true;   // the blocked form's packed maps and currents
       int rc = zero_state(h);
       if (rc) return rc;
       CEEP_HIP(hipStreamSynchronize(h->stream));
@@ -1753,7 +1848,7 @@ int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out) {
 
 struct ceep_fdtd_multi {
   std::vector<ceep_fdtd*> hs;
-  int K = 4, R = 16, bpw = 1, poles = 0, blocks = 0;
+  int K = 4, R = 16, bpw = 1, poles = 0, blocks = 0, waves = 1;
   ceep::MultiDesc* d_desc = nullptr;
   long long* d_seg_start = nullptr;
   int* d_seg_prob = nullptr;
@@ -1813,7 +1908,10 @@ int multi_enqueue(ceep_fdtd_multi* m, int mode) {
     const int s0 = m->round_seg[r], s1 = m->round_seg[r + 1];
     const long long first = m->seg_start_h[s0];
     const long long last = s1 < m->nseg ? m->seg_start_h[s1] : (long long)m->total;
-    CEEP_HIP(ceep::launch_round_tb(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
+    if (m->waves == 4)
+      CEEP_HIP(ceep::launch_round_mw(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
+    else
+      CEEP_HIP(ceep::launch_round_tb(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
   }
   return CEEP_OK;
 }
@@ -1822,12 +1920,19 @@ int multi_enqueue(ceep_fdtd_multi* m, int mode) {
 
 extern "C" {
 
-int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw,
+int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw, int waves,
                            ceep_fdtd_multi** out) {
   if (!out || !handles || n < 1) return fail(CEEP_EINVAL, "bad argument");
   *out = nullptr;
-  if (K < 1 || K > 4) return fail(CEEP_EINVAL, "temporal depth must be in [1, 4]");
-  if (R != 16 && R != 24 && R != 32) return fail(CEEP_EINVAL, "rows per wavefront must be 16, 24 or 32");
+  if (waves == 1) {
+    if (K < 1 || K > 4) return fail(CEEP_EINVAL, "temporal depth must be in [1, 4]");
+    if (R != 16 && R != 24 && R != 32) return fail(CEEP_EINVAL, "window rows must be 16, 24 or 32");
+  } else if (waves == 4) {
+    if (K != 4 && K != 6) return fail(CEEP_EINVAL, "four-wavefront rounds: temporal depth 4 or 6");
+    if (R != 32 && R != 48) return fail(CEEP_EINVAL, "four-wavefront rounds: window rows 32 or 48");
+  } else {
+    return fail(CEEP_EINVAL, "wavefronts per window must be 1 or 4");
+  }
   if (bpw < 1) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
   int rc;
   int poles = -1;
@@ -1842,7 +1947,7 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   }
   ceep_fdtd_multi* m = new ceep_fdtd_multi();
   m->hs.assign(handles, handles + n);
-  m->K = K; m->R = R; m->bpw = bpw; m->poles = poles;
+  m->K = K; m->R = R; m->bpw = bpw; m->poles = poles; m->waves = waves;
   std::vector<ceep::MultiDesc> desc(n);
   std::vector<int> ntasks(n);
   int max_rounds = 0;
@@ -1888,7 +1993,8 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) { multi_free(m); return fail(CEEP_EHIP, std::string("device query: ") + hipGetErrorString(e)); }
-  const int per_cu = ceep::multi_tb_blocks_per_cu(poles, R, K);
+  // workgroups of the persistent form (one-wavefront windows only)
+  const int per_cu = waves == 1 ? ceep::multi_tb_blocks_per_cu(poles, R, K) : 1;
   m->blocks = std::max(1, per_cu) * cus;
   rc = CEEP_OK;
   if (!rc) rc = dalloc(&m->d_desc, (size_t)n);

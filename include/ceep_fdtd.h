@@ -160,11 +160,14 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          reversible form (key 7) starts from FP32 roundings of the FP64
  *          segment-end states, the recompute form re-steps each segment in
  *          FP64.  Not a launch-shape knob: results differ.
- *   key 6: 3D step form: 0 = H kernel then E kernel in place; 1 = one
- *          fused H+E launch per step on ping-pong fields, workgroups ordered
- *          so each XCD walks contiguous x-slabs; 2 = fused, dispatch order;
- *          3 = fused, each thread marching along x over key-8 slabs.
- *          Switching between 0 and 1/2/3 clears the fields.
+ *   key 6: 3D step form: 4 (default) = temporally blocked, key-17 steps
+ *          per launch on 16 x 64 (y, z) windows marching along x, the whole
+ *          state in two ping-pong sets; 0 = H kernel then E kernel in place;
+ *          1 = one fused H+E launch per step on ping-pong fields, workgroups
+ *          ordered so each XCD walks contiguous x-slabs; 2 = fused, dispatch
+ *          order; 3 = fused, each thread marching along x over key-8 slabs.
+ *          All forms give bit-identical results.  Switching between 0, 1-3
+ *          and 4 (different state layouts) clears the fields.
  *   key 7: 2D blocked adjoint form: 1 = reversible (the forward state is
  *          rebuilt backward inside the adjoint launches from the checkpoint at
  *          each segment's end; no recomputation, no E store; default),
@@ -186,10 +189,13 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          otherwise).
  *   key 16: 2D blocked forward: wavefronts per window, 1 (default) or 4.
  *          Results are bit-identical between the two forms.
- *   key 12: 3D H/E kernel block order: 0 = dispatch order (tile, x-slab,
- *          batch chunk; default), 1 = the same renumbered so each XCD walks
- *          contiguous slabs, 2 = per XCD with the batch chunk fastest.
- *          Results are identical; 0 measured fastest on MI355X. */
+ *   key 12: 3D workgroup order: 0 = dispatch order (default; blocked form:
+ *          y tiles fastest, then z tiles, then TX), 1 = the same renumbered so
+ *          each XCD runs a contiguous range, 2 = per XCD with the TX (batch
+ *          chunk) fastest, 3 = blocked form only: per XCD in groups of key-18
+ *          TX, then y tiles.  Results are identical; 0 measured fastest.
+ *   key 17: 3D blocked form: steps per launch, 1 or 2 (default 2).
+ *   key 18: 3D blocked form, order 3: TX per group (default 4). */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,
@@ -230,9 +236,11 @@ int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out);
  * ceep_fdtd_dft). */
 typedef struct ceep_fdtd_multi ceep_fdtd_multi;
 
-/* K = steps per round (1..4), R = rows per wavefront window (16, 24, 32),
- * bpw = TX planes per wavefront.  The handles must outlive the group. */
-int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw,
+/* waves = 1: one wavefront per window, K = steps per round (1..4), R =
+ * window rows (16, 24, 32); waves = 4: four wavefronts per window (halo rows
+ * through LDS), K = 4 or 6, R = 32 or 48.  bpw = TX planes per window.  The
+ * handles must outlive the group. */
+int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw, int waves,
                            ceep_fdtd_multi** out);
 
 /* Zero every problem and run all of them to their own total_steps, as a

@@ -48,12 +48,13 @@ def _make(amd, spec, debye, tuning):
     return s, dt, w, m, pos
 
 
-def _check(amd, debye, K, R, bpw, mode):
+def _check(amd, debye, K, R, bpw, mode, waves=1):
     from ceep_amd.uneven import UnevenBatch
-    tun = {"temporal_depth": K, "rows_per_wave": R, "batch_per_wave": bpw}
+    tun = {"temporal_depth": K, "rows_per_wave": R, "batch_per_wave": bpw, "forward_waves": waves}
     group = [_make(amd, sp, debye, tun) for sp in SPECS]
-    ub = UnevenBatch([g[0] for g in group], K, R, bpw)
-    print(f"uneven batch mode={mode} K={K} R={R} bpw={bpw} debye={debye} {ub.info()}", flush=True)
+    ub = UnevenBatch([g[0] for g in group], K, R, bpw, waves)
+    print(f"uneven batch mode={mode} K={K} R={R} bpw={bpw} waves={waves} debye={debye} "
+          f"{ub.info()}", flush=True)
     ub.run(mode)
     multi = [g[0].probe_data.copy() for g in group]
     ub.run(mode)                                  # graph replay
@@ -77,6 +78,23 @@ def _check(amd, debye, K, R, bpw, mode):
 def test_uneven_per_round_launches(amd, debye, K, R, bpw):
     from ceep_amd.uneven import UnevenBatch
     _check(amd, debye, K, R, bpw, UnevenBatch.PER_ROUND)
+
+
+@pytest.mark.parametrize("debye", [True, False])
+@pytest.mark.parametrize("K,R,bpw", [(4, 32, 1), (4, 32, 2), (6, 48, 2)])
+def test_uneven_four_wavefront_rounds(amd, debye, K, R, bpw):
+    """Rounds on four-wavefront windows: every problem bit-identical to its
+    own run with the four-wavefront forward kernel of the same shape."""
+    from ceep_amd.uneven import UnevenBatch
+    _check(amd, debye, K, R, bpw, UnevenBatch.PER_ROUND, waves=4)
+
+
+def test_uneven_shape_checks(amd):
+    from ceep_amd.uneven import UnevenBatch
+    group = [_make(amd, sp, True, None)[0] for sp in SPECS[:2]]
+    for k, r, waves in ((4, 16, 4), (5, 32, 4), (4, 48, 1), (4, 16, 2)):
+        with pytest.raises(ValueError):
+            UnevenBatch(group, k, r, 1, waves)
 
 
 @pytest.mark.skip(reason="persistent single-launch mode switched off: its MI355X runs did not "

@@ -277,7 +277,7 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
         s._build()
         solvers.append(s)
         grids.append(g)
-    shape, log = (4, 16, 4, 1), []    # K, window rows, TX planes per window, wavefronts
+    shape, log = (6, 48, 4, 4), []    # K, window rows, TX planes per window, wavefronts (the tuner's pick)
     if autotune:
         shape, log = autotune_uneven(solvers)
     ub = UnevenBatch(solvers, *shape)
@@ -405,6 +405,8 @@ def _traffic(kernel, summary="step2d_debye_600x600x16_traffic.json"):
         return None
     with open(path) as f:
         d = json.load(f)
+    if "kernels" in d:
+        return (d["kernels"].get(kernel) or {}).get("bytes_per_launch")
     return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
 
 
@@ -421,6 +423,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--single-run", action="store_true",
                     help="one simulation + DFT with the default tuning, no output (for rocprofv3 --pmc)")
+    ap.add_argument("--tuning", default=None,
+                    help="with --single-run: JSON dict of launch settings applied on top of the defaults")
     ap.add_argument("--single-run-3d", action="store_true",
                     help="one 3D simulation (200^3, Debye + Lorentz, 100 steps), no output (for rocprofv3 --pmc)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -438,6 +442,8 @@ def main():
         return
     if args.single_run:
         s, *_ = _workload()
+        if args.tuning:
+            s._handle.apply_tuning(json.loads(args.tuning))
         s._handle.reset()
         s._handle.run(copy=False)
         s._handle.dft(FREQS)

@@ -21,7 +21,7 @@ _LOADED = False
 
 # measured best on MI355X for the 600^2 x 16 Debye workload; used when nothing
 # was tuned for a shape
-DEFAULT_2D = {"temporal_depth": 4, "rows_per_wave": 32, "batch_per_wave": 2,
+DEFAULT_2D = {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2,
               "batch_per_thread": 4, "graph": 1, "xcd_order": 1, "forward_waves": 4}
 
 

@@ -150,7 +150,7 @@ def test_2d_tuning_variants_bitwise(amd):
             assert np.array_equal(s.probe_data, base), (graph, xcd)
     assert rel_l2(base, one_step) < 1e-6
     # four-wavefront windows: tall windows, deeper blocking, ragged batch chunks
-    for R, K, bpw in ((32, 4, 1), (48, 8, 2), (64, 6, 3), (32, 8, 4)):
+    for R, K, bpw in ((32, 4, 1), (32, 6, 2), (48, 8, 2), (64, 6, 3), (32, 8, 4)):
         s.tuning = {"forward_waves": 4, "temporal_depth": K, "rows_per_wave": R,
                     "batch_per_wave": bpw, "graph": 1}
         s._build()

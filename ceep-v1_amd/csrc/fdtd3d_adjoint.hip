@@ -194,6 +194,7 @@ static hipError_t launch_adj3d_p(const Adj3DArgs& a, int bb, hipStream_t s) {
     case 1: return launch_adj3d_pb<POLES, 1>(a, s);
     case 2: return launch_adj3d_pb<POLES, 2>(a, s);
     case 4: return launch_adj3d_pb<POLES, 4>(a, s);
+    case 8: return launch_adj3d_pb<POLES, 8>(a, s);
   }
   return hipErrorInvalidValue;
 }

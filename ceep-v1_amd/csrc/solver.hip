@@ -1351,7 +1351,7 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
   const double scale = std::ldexp(1.0, -std::ilogb(mx));
   std::vector<float> dtr(ntr);
   for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
-  const int bb = h->bb >= 4 ? 4 : (h->bb >= 2 ? 2 : 1);
+  const int bb = h->bb >= 8 ? 8 : h->bb >= 4 ? 4 : (h->bb >= 2 ? 2 : 1);
   const int nchunks = (B + bb - 1) / bb;
   const bool use[6] = {true, true, true, h->has_debye, h->has_lorentz, h->has_lorentz};
   for (int r = 0; r < 6; ++r)

@@ -146,7 +146,8 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
                       double* grad_sigma);
 
 /* Tuning knobs (keys 0-4 give the same results to FP32 rounding for every setting):
- *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8)
+ *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8; the
+ *          3D one-step kernels use at most 4, the 3D adjoint kernels all four)
  *   key 1: replay the time loop as a captured hipGraph (0/1)
  *   key 2: 2D temporal depth K = steps per launch (0 = one-step kernel; 1..4
  *          with one wavefront per window, 4, 6 or 8 with four, key 16)

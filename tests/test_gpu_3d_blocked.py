@@ -89,7 +89,12 @@ def test_blocked_3d_equals_one_step(amd, kinds, magnetic, shape, T):
     sm = rng.uniform(0, 3e3, shape) if magnetic else None
     c = [s // 2 for s in shape]
     src = [tuple(c), (c[0] - 3, c[1] + 4, c[2] - 5), (n + 1, n + 1, n + 1)]
-    prb = src + [(c[0] + 2, c[1] - 6, c[2] + 7), (shape[0] - n - 2, c[1], shape[2] - n - 2)]
+    # interior probes, probes on the four kinds of box wall (first and last
+    # slab, last row, first column: E there never changes) and one in the
+    # CPML region outside the stepped box
+    prb = src + [(c[0] + 2, c[1] - 6, c[2] + 7), (shape[0] - n - 2, c[1], shape[2] - n - 2),
+                 (n, c[1], c[2]), (shape[0] - n, c[1], c[2]), (c[0], shape[1] - n, c[2]),
+                 (c[0], c[1], n), (1, 1, 1)]
     out = {}
     for form in (0, 4):
         s = _solver(amd, shape, n, T, dx, src, prb, w, eps, sig, m, kinds, mu, sm)

@@ -32,23 +32,6 @@ namespace {
 
 constexpr int kRows = 16;   // window rows: one wavefront each
 
-// lane l receives lane l+1 (0 for lane 63): the value at z+1
-__device__ __forceinline__ float from_next_lane(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
-}
-// lane l receives lane l-1 (0 for lane 0): the value at z-1
-__device__ __forceinline__ float from_prev_lane(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
-}
-
-// LDS exchange barrier: waits for this wavefront's LDS operations only, not
-// for its global loads in flight (the next slabs' prefetch)
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 template <int POLES>
 struct Cell3 {
   float e[3], h[3];
@@ -60,26 +43,6 @@ struct Cell3 {
 struct Maps3 {
   float ca, cb, dk1, dk2, lk1, lk2, lk3, da, dbi;
   int ph;
-};
-
-// Buffer accesses: resource = (base, size), lane byte offset in a VGPR
-// (kOffNone off the window: the range check loads 0 and drops the store),
-// uniform slab / set offset in an SGPR, constant component offsets folded
-// into the instruction.  No 64-bit address per access.
-__device__ __forceinline__ float ldb(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
-}
-__device__ __forceinline__ void stb(float v, __amdgpu_buffer_rsrc_t r, unsigned voff,
-                                    unsigned soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
-}
-__device__ __forceinline__ int ldb_short(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return (int)(short)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
-}
-
-template <int V>
-struct Phase {
-  static constexpr int value = V;
 };
 
 }  // namespace
@@ -174,7 +137,7 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
     const unsigned v = slab_ok(a) ? yzb : kOffNone;
     const unsigned so = in_b + (unsigned)max(a, 0) * sx_b;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) c.e[q] = ldb(re[q], v, so);
+    for (int q = 0; q < 3; ++q) c.e[q] = rload(re[q], v, so);
   };
   auto load_hj = [&](Cell3<POLES>& c, int a) {
     const bool ok = slab_ok(a);
@@ -183,11 +146,11 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
     const unsigned sj = (unsigned)max(a, 0) * sxj_b;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      c.h[q] = ldb(rh[q], v, so);
-      if (POLES & 1) c.jd[q] = ldb(rj_in, vj + 4u * q, sj);
+      c.h[q] = rload(rh[q], v, so);
+      if (POLES & 1) c.jd[q] = rload(rj_in, vj + 4u * q, sj);
       if (POLES & 2) {
-        c.jl[q] = ldb(rj_in, vj + 4u * (JL + q), sj);
-        c.jlp[q] = ldb(rj_in, vj + 4u * (JL + 3 + q), sj);
+        c.jl[q] = rload(rj_in, vj + 4u * (JL + q), sj);
+        c.jlp[q] = rload(rj_in, vj + 4u * (JL + 3 + q), sj);
       }
     }
   };
@@ -195,18 +158,18 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
     const bool ok = slab_ok(a);
     const unsigned v = ok ? yzm : kOffNone;
     const unsigned so = (unsigned)max(a, 0) * sxm_b;
-    m.ca = ldb(rm, v, so);
-    m.cb = ldb(rm, v + 4u, so);
-    if (POLES & 1) { m.dk1 = ldb(rm, v + 8u, so); m.dk2 = ldb(rm, v + 12u, so); }
+    m.ca = rload(rm, v, so);
+    m.cb = rload(rm, v + 4u, so);
+    if (POLES & 1) { m.dk1 = rload(rm, v + 8u, so); m.dk2 = rload(rm, v + 12u, so); }
     if (POLES & 2) {
-      m.lk1 = ldb(rm, v + 16u, so);
-      m.lk2 = ldb(rm, v + 20u, so);
-      m.lk3 = ldb(rm, v + 24u, so);
+      m.lk1 = rload(rm, v + 16u, so);
+      m.lk2 = rload(rm, v + 20u, so);
+      m.lk3 = rload(rm, v + 24u, so);
     }
-    if (MAG) { m.da = ldb(rm, v + 28u, so); m.dbi = ldb(rm, v + 32u, so); }
+    if (MAG) { m.da = rload(rm, v + 28u, so); m.dbi = rload(rm, v + 32u, so); }
     // probe head: read (and masked to the inner cells) where it is used, so
     // this prefetch is not waited for here
-    m.ph = probes ? ldb_short(rp, ok ? yzp : kOffNone, (unsigned)max(a, 0) * (unsigned)(p.sx * 2))
+    m.ph = probes ? rload_short(rp, ok ? yzp : kOffNone, (unsigned)max(a, 0) * (unsigned)(p.sx * 2))
                   : -1;
   };
 
@@ -257,7 +220,7 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
       // H (h3d_kernel): forward differences
       {
         const float e0 = c.e[0], e1 = c.e[1], e2 = c.e[2];
-        const float eyz = from_next_lane(e1), exz = from_next_lane(e0);
+        const float eyz = lane_next(e1), exz = lane_next(e0);
         const float ezx = cn.e[2], eyx = cn.e[1];
         const float da = m.da, cc = MAG ? m.dbi : p.ch;
         if (hx_ok) {
@@ -281,7 +244,7 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
       const float hxy = xh[k - 1][1][ws][lane];
       // E (e3d_kernel) on the interior; -x neighbours kept from the last slab
       const float hx0 = c.h[0], hy0 = c.h[1], hz0 = c.h[2];
-      const float hyz = from_prev_lane(hy0), hxz = from_prev_lane(hx0);
+      const float hyz = lane_prev(hy0), hxz = lane_prev(hx0);
       const bool interior = e_yz && a >= 1 && a <= p.Mx - 2;
       if (interior) {
         float cu[3];
@@ -331,12 +294,12 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
       const unsigned sj = (unsigned)a * sxj_b;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        stb(c.e[q], re[q], yzs, so);
-        stb(c.h[q], rh[q], yzs, so);
-        if (POLES & 1) stb(c.jd[q], rj_out, yzjs + 4u * q, sj);
+        rstore(c.e[q], re[q], yzs, so);
+        rstore(c.h[q], rh[q], yzs, so);
+        if (POLES & 1) rstore(c.jd[q], rj_out, yzjs + 4u * q, sj);
         if (POLES & 2) {
-          stb(c.jl[q], rj_out, yzjs + 4u * (JL + q), sj);
-          stb(c.jlp[q], rj_out, yzjs + 4u * (JL + 3 + q), sj);
+          rstore(c.jl[q], rj_out, yzjs + 4u * (JL + q), sj);
+          rstore(c.jlp[q], rj_out, yzjs + 4u * (JL + 3 + q), sj);
         }
       }
     }

@@ -47,6 +47,42 @@ __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsign
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
 }
 
+// Raw buffer access with a uniform SGPR offset (slab, set or component) on
+// top of the lane's VGPR offset; constant component offsets added to the
+// VGPR offset fold into the instruction.
+__device__ __forceinline__ float rload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void rstore(float v, __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                       unsigned soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ int rload_short(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return (int)(short)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+
+// DPP wave shifts: lane l receives lane l+1 (0 for lane 63), resp. lane l-1
+// (0 for lane 0)
+__device__ __forceinline__ float lane_next(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+
+// Workgroup barrier for LDS exchanges: waits for this wavefront's LDS
+// operations only, not for its global loads in flight (prefetches)
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int V>
+struct Phase {
+  static constexpr int value = V;
+};
+
 // Workgroups of a 1D grid are dealt round-robin over MI355X's 8 XCDs (each
 // with its own L2).  With `on`, physical block b is renumbered so that XCD x
 // runs one contiguous range of logical blocks: neighbouring tiles, which

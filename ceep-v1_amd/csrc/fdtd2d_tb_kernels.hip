@@ -1105,6 +1105,10 @@ static hipError_t launch_rev_mw_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
     case 3: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 3>), grd, blk, 0, s, a); break;
     case 4: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 4>), grd, blk, 0, s, a); break;
     case 5: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 5>), grd, blk, 0, s, a); break;
+    case 6:
+      if (RW < 8) return hipErrorInvalidValue;   // the window keeps at least 20 exact rows
+      hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 6>), grd, blk, 0, s, a);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

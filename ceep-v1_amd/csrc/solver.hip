@@ -165,8 +165,8 @@ struct ceep_fdtd {
   float* pin_dtr = nullptr;    // pinned host staging: dL/dtrace (FP32) ...
   float* pin_g = nullptr;      // ... and the two gradient planes
   size_t pin_dtr_n = 0, pin_g_n = 0;
-  int adj_k = 0;               // 2D blocked adjoint: reversed steps per launch (0: 5 for the
-                               // four-wavefront reversible kernel, 4 otherwise)
+  int adj_k = 0;               // 2D blocked adjoint: reversed steps per launch (0: 6 for the
+                               // four-wavefront reversible kernel on 32-row windows, 4 otherwise)
   int fwd_waves = 1;           // 2D blocked forward: wavefronts per window (1 or 4)
   int xcd3d = 0;               // 3D H/E kernels: block order (0 dispatch, 1/2 per XCD)
   int tb3d_k = 2;              // 3D blocked form: steps per launch (1 or 2)
@@ -1517,9 +1517,10 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
   const bool rev = h->ck_rev && h->use_rev();
   const bool mw_rev = rev && h->adj_waves == 4;
-  const int adj_k = h->adj_k ? h->adj_k : (mw_rev ? 5 : 4);
+  const int adj_k = h->adj_k ? h->adj_k
+                             : (mw_rev ? ((h->adj_r == 0 || h->adj_r == 32) ? 6 : 5) : 4);
   if (h->use_tb() && adj_k > 4 && !mw_rev)
-    return fail(CEEP_EINVAL, "adjoint temporal depth 5 needs the four-wavefront reversible adjoint");
+    return fail(CEEP_EINVAL, "adjoint temporal depth above 4 needs the four-wavefront reversible adjoint");
   if (!rev && !h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * nbox))) return rc;
   for (int i = 0; i < 2; ++i)
     for (int f = 0; f < 3; ++f)
@@ -1708,8 +1709,8 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value != 1 && value != 4) return fail(CEEP_EINVAL, "adjoint wavefronts per window must be 1 or 4");
     h->adj_waves = value;
   } else if (key == 15) {
-    if (value < 1 || value > 5)
-      return fail(CEEP_EINVAL, "adjoint temporal depth must be in [1, 5] (5: four-wavefront reversible form only)");
+    if (value < 1 || value > 6)
+      return fail(CEEP_EINVAL, "adjoint temporal depth must be in [1, 6] (5, 6: four-wavefront reversible form only)");
     h->adj_k = value;
   } else if (key == 17) {
     if (value != 1 && value != 2) return fail(CEEP_EINVAL, "3D blocked steps per launch must be 1 or 2");

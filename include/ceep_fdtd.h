@@ -185,8 +185,9 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *   key 14: 2D reversible adjoint: wavefronts per window, 1 or 4 (default 4:
  *          each wavefront holds a quarter of the window's rows, halo rows
  *          pass through LDS).
- *   key 15: 2D blocked adjoint: reversed steps per launch, 1..5 (5 only
- *          with the four-wavefront reversible kernel; default 5 there, 4
+ *   key 15: 2D blocked adjoint: reversed steps per launch, 1..6 (5 and 6
+ *          only with the four-wavefront reversible kernel, 6 only on 32-row
+ *          windows; default 6 there, 5 on smaller windows of that kernel, 4
  *          otherwise).
  *   key 16: 2D blocked forward: wavefronts per window, 1 (default) or 4.
  *          Results are bit-identical between the two forms.

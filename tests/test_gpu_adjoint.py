@@ -132,6 +132,9 @@ def test_fp64_forward_blocked_equals_one_step(amd, reference, every):
      "xcd_order": 0},
     {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_temporal_depth": 5, "adjoint_batch_per_wave": 5},
     {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_temporal_depth": 5, "adjoint_rows_per_wave": 24},
+    {"temporal_depth": 6, "forward_waves": 4, "rows_per_wave": 32, "batch_per_wave": 2,
+     "adjoint_waves": 4, "adjoint_temporal_depth": 6, "adjoint_batch_per_wave": 5},
+    {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_temporal_depth": 5},
 ])
 def test_adjoint_kernel_variants(amd, reference, tuning):
     """One-step and blocked kernels, reversible (default) and recompute + E store

@@ -277,7 +277,7 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
         s._build()
         solvers.append(s)
         grids.append(g)
-    shape, log = (6, 48, 4, 4), []    # K, window rows, TX planes per window, wavefronts (the tuner's pick)
+    shape, log = (6, 32, 4, 4), []    # K, window rows, TX planes per window, wavefronts (the tuner's pick)
     if autotune:
         shape, log = autotune_uneven(solvers)
     ub = UnevenBatch(solvers, *shape)

@@ -104,7 +104,7 @@ def autotune_2d(solver, reps: int = 2, candidates=None, verbose: bool = False) -
 
 def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 32),
                     batch_per_wave=(1, 2, 4), start=(4, 16, 2),
-                    four_wave=((4, 32, 1), (4, 32, 2), (4, 32, 4), (4, 48, 2), (6, 48, 1),
+                    four_wave=((4, 32, 2), (4, 32, 4), (6, 32, 2), (6, 32, 4), (4, 48, 2),
                                (6, 48, 2), (6, 48, 4)),
                     verbose: bool = False):
     """Launch shape (temporal depth K, window rows R, TX planes per window,

@@ -187,12 +187,19 @@ struct ceep_fdtd {
   long long box_floats() const { return (long long)B * plane; }
   // field buffer holding the current state (3D fields stay in buffer 0 with
   // the two-kernel in-place update)
-  int fbuf() const { return cpml() ? 0 : (ndim == 2 || fused3d) ? parity : 0; }
+  int fbuf() const { return cpml() ? 0 : (ndim == 2 || form3d()) ? parity : 0; }
   int poles() const { return (has_debye ? 1 : 0) | (has_lorentz ? 2 : 0); }
+  // the 3D step form in effect: the blocked form (4) addresses a TX's two
+  // state sets and its packed maps with 32-bit byte offsets, so beyond those
+  // limits the handle steps with the one-step kernels (form 0) instead
+  int form3d() const {
+    const bool fits = ((long long)B * plane + plane) * 4 < (1LL << 31) && plane * 36 < (1LL << 31);
+    return fused3d == 4 && !fits ? 0 : fused3d;
+  }
   // 3D temporally blocked form: every state array holds two sets (fields
-  // f[0]/f[1], J at +0 / +box_floats()); set `parity` is live, and each set
-  // keeps its Lorentz J as (jl[0] = newest, jl[1] = one step older)
-  bool tb3d() const { return ndim == 3 && !cpml() && fused3d == 4; }
+  // f[0]/f[1], J packed per cell at set * nj * box_floats()); set `parity` is
+  // live, and each set keeps its Lorentz J as (newest, one step older)
+  bool tb3d() const { return ndim == 3 && !cpml() && form3d() == 4; }
   // the temporally blocked 2D kernel covers the plain and Debye cases
   bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz && !cpml(); }
   // 2D gradient path: reversible reconstruction instead of recompute + E store
@@ -550,10 +557,6 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     ceep::TB3DArgs t{};
     t.Mx = h->M[0]; t.My = h->M[1]; t.Mz = h->M[2]; t.pitch = h->pitch; t.B = h->B;
     t.sx = h->row_block; t.bstride = h->plane; t.nbox = h->box_floats();
-    if ((t.nbox + t.bstride) * 4 >= (1LL << 31))
-      return fail(CEEP_EUNSUPPORTED, "3D blocked form: batch x volume above 2^29 floats; use step form 0");
-    if ((long long)t.bstride * 9 * 4 >= (1LL << 31))
-      return fail(CEEP_EUNSUPPORTED, "3D blocked form: volume per TX above 2^31 / 36 cells; use step form 0");
     for (int k = 0; k < 3; ++k) { t.e[k] = h->f[0][k]; t.h[k] = h->f[0][3 + k]; }
     t.jp = h->jpack;
     t.mp = h->mpack;
@@ -598,16 +601,16 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       for (int k = 0; k < 3; ++k) { a.jl_cur[k] = h->jl[i][k]; a.jl_prev[k] = h->jl[o][k]; }
       a.step = st;
       a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * 3 * h->box_floats() : nullptr;
-      if (h->fused3d) {
+      if (h->form3d()) {
         ceep::Step3DFArgs fa{};
         fa.a = a;
-        fa.xcd_order = h->fused3d == 1;
+        fa.xcd_order = h->form3d() == 1;
         fa.seg_len = h->march_len;
         for (int k = 0; k < 3; ++k) {
           fa.e_in[k] = h->f[i][k]; fa.h_in[k] = h->f[i][3 + k];
           fa.e_out[k] = h->f[o][k]; fa.h_out[k] = h->f[o][3 + k];
         }
-        if (h->fused3d == 3)
+        if (h->form3d() == 3)
           CEEP_HIP(ceep::launch_step3d_march(fa, poles, bb, s));
         else
           CEEP_HIP(ceep::launch_step3d_fused(fa, poles, bb, s));
@@ -1238,7 +1241,7 @@ int run_checkpointed64(ceep_fdtd* h, int nck, bool rev, double* traces_out) {
 // (Debye J per component, both Lorentz buffers per component).
 std::vector<float*> state3d(ceep_fdtd* h, int par) {
   std::vector<float*> v;
-  const int fb = h->fused3d ? par : 0;
+  const int fb = h->form3d() ? par : 0;
   for (int k = 0; k < 6; ++k) v.push_back(h->f[fb][k]);
   if (h->tb3d()) {   // set `par` of the packed currents, as nj chunks of box_floats()
     const long long nbox = h->box_floats();

@@ -168,7 +168,10 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          ordered so each XCD walks contiguous x-slabs; 2 = fused, dispatch
  *          order; 3 = fused, each thread marching along x over key-8 slabs.
  *          All forms give bit-identical results.  Switching between 0, 1-3
- *          and 4 (different state layouts) clears the fields.
+ *          and 4 (different state layouts) clears the fields.  Form 4 steps
+ *          with form 0 where a TX's two state sets exceed 32-bit byte
+ *          offsets ((B + 1) x box volume x 4 B >= 2^31, or box volume x 36 B
+ *          >= 2^31).
  *   key 7: 2D blocked adjoint form: 1 = reversible (the forward state is
  *          rebuilt backward inside the adjoint launches from the checkpoint at
  *          each segment's end; no recomputation, no E store; default),

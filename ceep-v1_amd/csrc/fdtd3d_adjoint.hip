@@ -15,12 +15,20 @@
 //
 // Mapping as the forward kernels: 64 z-cells x 4 y-rows per workgroup,
 // blockIdx.z = x-slab + Mx * batch chunk, BB TX planes per thread.
+//
+// Between two reversed steps C of step s + 1 and A of step s touch the same
+// cells only (C reads hbar at the -1 neighbours, then A is pointwise), so A
+// can run with C of the step before folded in (WITH_C): lam then crosses HBM
+// once per step instead of three times, and a step takes three launches only
+// at the ends of the reversed run (A alone first, C alone last).  Contraction
+// is off in A, B and C, so both orders give identical results.
 #include "fdtd3d_kernels.h"
 
 namespace ceep {
 
-template <int POLES, int BB>
+template <int POLES, int BB, bool WITH_C, bool MAGNETIC>
 __global__ __launch_bounds__(256) void adj3d_a_kernel(Adj3DArgs p) {
+#pragma clang fp contract(off)
   const int d = blockIdx.x * 64 + threadIdx.x;
   const int b = blockIdx.y * 4 + threadIdx.y;
   const int a = blockIdx.z % p.Mx;
@@ -67,6 +75,35 @@ __global__ __launch_bounds__(256) void adj3d_a_kernel(Adj3DArgs p) {
       es1[k][c] = p.e_s1[c * p.e_cstride + o];
     }
   }
+  if (WITH_C) {   // C of step s + 1 (adj3d_c_kernel), then the probe seed of step s
+    float c0 = p.ch, cmz = p.ch, cmy = p.ch, cmx = p.ch;
+    if (MAGNETIC) {
+      c0 = p.dbi[cell];
+      cmz = p.dbi[cell - 1];
+      cmy = p.dbi[cell - p.pitch];
+      cmx = p.dbi[cell - p.sx];
+    }
+    const int probe0 = p.probe_head ? p.probe_head[cell] : -1;
+    const float* __restrict__ hx = p.hbar[0];
+    const float* __restrict__ hy = p.hbar[1];
+    const float* __restrict__ hz = p.hbar[2];
+#pragma unroll
+    for (int k = 0; k < BB; ++k) {
+      if (k >= nb) break;
+      const int bt = b0 + k;
+      const long long o = (long long)bt * p.bstride + cell;
+      const float rx0 = c0 * hx[o], ry0 = c0 * hy[o], rz0 = c0 * hz[o];
+      const float rx_mz = cmz * hx[o - 1], rx_my = cmy * hx[o - p.pitch];
+      const float ry_mz = cmz * hy[o - 1], ry_mx = cmx * hy[o - p.sx];
+      const float rz_my = cmy * hz[o - p.pitch], rz_mx = cmx * hz[o - p.sx];
+      lam[k][0] += (ry0 - ry_mz) - (rz0 - rz_my);
+      lam[k][1] += (rz0 - rz_mx) - (rx0 - rx_mz);
+      float lz = lam[k][2] + ((rx0 - rx_my) - (ry0 - ry_mx));
+      for (int q = probe0; q >= 0; q = p.probe_next[q])
+        lz += p.dtr[((long long)bt * p.P + q) * p.T + p.step];
+      lam[k][2] = lz;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < BB; ++k) {
     if (k >= nb) break;
@@ -94,6 +131,7 @@ __global__ __launch_bounds__(256) void adj3d_a_kernel(Adj3DArgs p) {
 
 template <bool MAGNETIC, int BB>
 __global__ __launch_bounds__(256) void adj3d_b_kernel(Adj3DArgs p) {
+#pragma clang fp contract(off)
   const int d = blockIdx.x * 64 + threadIdx.x;
   const int b = blockIdx.y * 4 + threadIdx.y;
   const int a = blockIdx.z % p.Mx;
@@ -130,6 +168,7 @@ __global__ __launch_bounds__(256) void adj3d_b_kernel(Adj3DArgs p) {
 
 template <bool MAGNETIC, int BB>
 __global__ __launch_bounds__(256) void adj3d_c_kernel(Adj3DArgs p) {
+#pragma clang fp contract(off)
   const int d = blockIdx.x * 64 + threadIdx.x;
   const int b = blockIdx.y * 4 + threadIdx.y;
   const int a = blockIdx.z % p.Mx;
@@ -174,37 +213,57 @@ static dim3 adj_grid3(const Adj3DArgs& a, int bb) {
   return dim3((a.Mz + 63) / 64, (a.My + 3) / 4, a.Mx * ((a.B + bb - 1) / bb));
 }
 
+// one reversed step: A (with C of the step before folded in when with_c), then B
 template <int POLES, int BB>
-static hipError_t launch_adj3d_pb(const Adj3DArgs& a, hipStream_t s) {
+static hipError_t launch_adj3d_pb(const Adj3DArgs& a, bool with_c, hipStream_t s) {
   const dim3 blk(64, 4, 1), grd = adj_grid3(a, BB);
-  hipLaunchKernelGGL((adj3d_a_kernel<POLES, BB>), grd, blk, 0, s, a);
   if (a.da) {
+    if (with_c) hipLaunchKernelGGL((adj3d_a_kernel<POLES, BB, true, true>), grd, blk, 0, s, a);
+    else hipLaunchKernelGGL((adj3d_a_kernel<POLES, BB, false, true>), grd, blk, 0, s, a);
     hipLaunchKernelGGL((adj3d_b_kernel<true, BB>), grd, blk, 0, s, a);
-    hipLaunchKernelGGL((adj3d_c_kernel<true, BB>), grd, blk, 0, s, a);
   } else {
+    if (with_c) hipLaunchKernelGGL((adj3d_a_kernel<POLES, BB, true, false>), grd, blk, 0, s, a);
+    else hipLaunchKernelGGL((adj3d_a_kernel<POLES, BB, false, false>), grd, blk, 0, s, a);
     hipLaunchKernelGGL((adj3d_b_kernel<false, BB>), grd, blk, 0, s, a);
-    hipLaunchKernelGGL((adj3d_c_kernel<false, BB>), grd, blk, 0, s, a);
   }
   return hipGetLastError();
 }
 
 template <int POLES>
-static hipError_t launch_adj3d_p(const Adj3DArgs& a, int bb, hipStream_t s) {
+static hipError_t launch_adj3d_p(const Adj3DArgs& a, int bb, bool with_c, hipStream_t s) {
   switch (bb) {
-    case 1: return launch_adj3d_pb<POLES, 1>(a, s);
-    case 2: return launch_adj3d_pb<POLES, 2>(a, s);
-    case 4: return launch_adj3d_pb<POLES, 4>(a, s);
-    case 8: return launch_adj3d_pb<POLES, 8>(a, s);
+    case 1: return launch_adj3d_pb<POLES, 1>(a, with_c, s);
+    case 2: return launch_adj3d_pb<POLES, 2>(a, with_c, s);
+    case 4: return launch_adj3d_pb<POLES, 4>(a, with_c, s);
+    case 8: return launch_adj3d_pb<POLES, 8>(a, with_c, s);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_adj3d(const Adj3DArgs& a, int poles, int bb, hipStream_t s) {
+template <int BB>
+static hipError_t launch_adj3d_c_b(const Adj3DArgs& a, hipStream_t s) {
+  const dim3 blk(64, 4, 1), grd = adj_grid3(a, BB);
+  if (a.da) hipLaunchKernelGGL((adj3d_c_kernel<true, BB>), grd, blk, 0, s, a);
+  else hipLaunchKernelGGL((adj3d_c_kernel<false, BB>), grd, blk, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj3d_c(const Adj3DArgs& a, int bb, hipStream_t s) {
+  switch (bb) {
+    case 1: return launch_adj3d_c_b<1>(a, s);
+    case 2: return launch_adj3d_c_b<2>(a, s);
+    case 4: return launch_adj3d_c_b<4>(a, s);
+    case 8: return launch_adj3d_c_b<8>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj3d(const Adj3DArgs& a, int poles, int bb, bool with_c, hipStream_t s) {
   switch (poles) {
-    case 0: return launch_adj3d_p<0>(a, bb, s);
-    case 1: return launch_adj3d_p<1>(a, bb, s);
-    case 2: return launch_adj3d_p<2>(a, bb, s);
-    case 3: return launch_adj3d_p<3>(a, bb, s);
+    case 0: return launch_adj3d_p<0>(a, bb, with_c, s);
+    case 1: return launch_adj3d_p<1>(a, bb, with_c, s);
+    case 2: return launch_adj3d_p<2>(a, bb, with_c, s);
+    case 3: return launch_adj3d_p<3>(a, bb, with_c, s);
   }
   return hipErrorInvalidValue;
 }

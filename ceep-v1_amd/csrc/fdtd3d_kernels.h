@@ -118,7 +118,11 @@ struct Adj3DArgs {
   long long gstride;
 };
 
-hipError_t launch_adj3d(const Adj3DArgs& a, int poles, int bb, hipStream_t s);
+// A of step a.step (with C of step a.step + 1 folded in when with_c: the
+// probe seed it adds is dL/dtrace(a.step)), then B.
+hipError_t launch_adj3d(const Adj3DArgs& a, int poles, int bb, bool with_c, hipStream_t s);
+// C of step a.step alone (the last reversed step)
+hipError_t launch_adj3d_c(const Adj3DArgs& a, int bb, hipStream_t s);
 // lam_z[b][probe cell] += dtr[b][q][step] for every TX and probe
 hipError_t launch_adj3d_seed(float* lam_z, const long long* probe_cell, const float* dtr,
                              int B, int P, int T, int step, long long bstride, hipStream_t s);

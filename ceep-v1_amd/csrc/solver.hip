@@ -1408,10 +1408,13 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
       a.e_s = h->estore + (size_t)(s - s0) * slot;
       a.e_s1 = h->estore + (size_t)(s + 1 - s0) * slot;
       a.step = s;
-      CEEP_HIP(ceep::launch_adj3d(a, h->poles(), bb, st));
+      // C of step s + 1 runs inside A of step s (none before the last step)
+      CEEP_HIP(ceep::launch_adj3d(a, h->poles(), bb, s < T - 1, st));
       for (int c = 0; c < 3; ++c) std::swap(a.al[c], a.al_prev[c]);   // the new al is in al_prev
     }
   }
+  a.step = 0;   // C of the first step, alone
+  CEEP_HIP(ceep::launch_adj3d_c(a, bb, st));
   if (nchunks > 1) {   // fold the chunk partials into plane 0 (in place: plane 0 is read first)
     CEEP_HIP(ceep::launch_sum_parts(h->g_ca, nchunks, h->plane, h->plane, h->g_ca, st));
     CEEP_HIP(ceep::launch_sum_parts(h->g_cbi, nchunks, h->plane, h->plane, h->g_cbi, st));

@@ -180,6 +180,38 @@ struct TB3DArgs {
 
 hipError_t launch_step3d_tb(const TB3DArgs& a, int K, int poles, hipStream_t s);
 
+// Temporally blocked time-reversed 3D step: reversed steps step, step - 1,
+// ..., step - K + 1 (K = 1 or 2; each A, B and C of the three-kernel form)
+// on 16 x 64 (y, z) windows marching along x.  lam and hbar per component
+// are one allocation of two nbox-float sets, the adjoint currents one packed
+// array [set][B][cell][nj] like the forward's; a launch reads in_set and
+// writes the other.  The gradient sums go to one plane per TX and coefficient.
+struct AdjTB3DArgs {
+  int Mx, My, Mz, pitch, B;
+  long long sx, bstride, nbox;
+  int in_set;
+  float* lam[3]; float* hbar[3];  // set-0 bases
+  float* jp;                      // packed adjoint currents (null without poles)
+  const float* mp;                // packed maps [cell][9] as TB3DArgs::mp
+  int magnetic;
+  float ch, idx;
+  const float* e_lo;              // E store slot of step - K + 1 ([3][nbox] per slot), then
+                                  // the K slots after it
+  const long long* src_cell;
+  const float* wave;
+  long long wave_stride_b;
+  int step, T, P;
+  const float* dtr;               // [B][P][T]
+  const short* probe_head;
+  const short* probe_next;
+  float* g;                       // [2][B][gstride]: sums for ca, then for cb (as Adj3DArgs)
+  long long gstride;
+  int tiles_y, tiles_z;           // ceil((My-1)/(16-2K)), ceil((Mz-1)/(64-2K))
+  int order;                      // 0: dispatch order, 1: contiguous tile ranges per XCD
+};
+
+hipError_t launch_adj3d_tb(const AdjTB3DArgs& a, int K, int poles, hipStream_t s);
+
 hipError_t launch_h3d(const Step3DArgs& a, int bb, hipStream_t s);
 hipError_t launch_e3d(const Step3DArgs& a, int poles, int bb, hipStream_t s);
 

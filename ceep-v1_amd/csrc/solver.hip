@@ -133,6 +133,12 @@ struct ceep_fdtd {
   float* g_cbi = nullptr;
   float* d_dtr = nullptr;
   float* adj3[6][3] = {{nullptr}};  // 3D: lam, hbar, q, Debye, Lorentz, Lorentz-prev adjoints
+  // 3D blocked adjoint: lam then hbar, each component two nbox-float sets
+  // (12 nbox), the packed adjoint currents (2 nj nbox) and the gradient sums
+  // per TX ([2][B][plane])
+  float* adjtb = nullptr;
+  float* adjtb_j = nullptr;
+  float* g3tb = nullptr;
   bool ck_valid = false;
   int estore_seg = -1;         // segment whose forward E the store holds, or -1
 
@@ -171,6 +177,7 @@ struct ceep_fdtd {
   int xcd3d = 0;               // 3D H/E kernels: block order (0 dispatch, 1/2 per XCD)
   int tb3d_k = 2;              // 3D blocked form: steps per launch (1 or 2)
   int tx_group3d = 4;          // 3D blocked form, block order 3: TX per group
+  int adj3d_k = 2;             // 3D adjoint: reversed steps per blocked launch (0: the A/B/C kernels)
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
@@ -200,6 +207,11 @@ struct ceep_fdtd {
   // f[0]/f[1], J packed per cell at set * nj * box_floats()); set `parity` is
   // live, and each set keeps its Lorentz J as (newest, one step older)
   bool tb3d() const { return ndim == 3 && !cpml() && form3d() == 4; }
+  // 3D blocked adjoint: runs with the blocked forward (its packed maps) while
+  // an E-store slot of three components fits its 32-bit byte offsets
+  bool adjtb3d() const {
+    return tb3d() && adj3d_k > 0 && (2 * box_floats() + plane) * 4 < (1LL << 31);
+  }
   // the temporally blocked 2D kernel covers the plain and Debye cases
   bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz && !cpml(); }
   // 2D gradient path: reversible reconstruction instead of recompute + E store
@@ -228,6 +240,7 @@ struct ceep_fdtd {
       for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
     for (auto& row : adj3)
       for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
+    for (float** p : {&adjtb, &adjtb_j, &g3tb}) { if (*p) (void)hipFree(*p); *p = nullptr; }
     if (g_ca) (void)hipFree(g_ca);
     if (g_cbi) (void)hipFree(g_cbi);
     if (d_dtr) (void)hipFree(d_dtr);
@@ -1339,10 +1352,14 @@ void chain_rule(double eps_r, double eps_add, double sigma, double dt, double dL
   *gs = dL_dca * dca_ds * ds_dsig + dL_dcb * dcb_dsig;
 }
 
+int adjoint3d_abc(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<float>& gca,
+                  std::vector<float>& gcb);
+int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<float>& gca,
+                      std::vector<float>& gcb);
+
 int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* grad_sigma) {
   int rc;
-  const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
-  const long long nbox = h->box_floats();
+  const int B = h->B, P = h->P, T = h->T;
   const size_t ntr = (size_t)B * P * T;
   double mx = 0.0;
   for (size_t i = 0; i < ntr; ++i) mx = std::max(mx, std::fabs(dtraces[i]));
@@ -1354,6 +1371,34 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
   const double scale = std::ldexp(1.0, -std::ilogb(mx));
   std::vector<float> dtr(ntr);
   for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
+  std::vector<float> gca(h->plane), gcb(h->plane);
+  if (h->adjtb3d()) {
+    if ((rc = adjoint3d_blocked(h, dtr, gca, gcb))) return rc;
+  } else if ((rc = adjoint3d_abc(h, dtr, gca, gcb))) {
+    return rc;
+  }
+  hipStream_t st = h->stream;
+  if ((rc = zero_state(h))) return rc;   // the recomputation overwrote the fields
+  CEEP_HIP(hipStreamSynchronize(st));
+  for (int x = 1; x < h->M[0] - 1; ++x)
+    for (int y = 1; y < h->M[1] - 1; ++y)
+      for (int z = 1; z < h->M[2] - 1; ++z) {
+        const long long o = h->box_index(x, y, z);
+        const long long g = h->full_index(x + h->org, y + h->org, z + h->org);
+        chain_rule(h->eps_r[g], eps_add_at(h, g), h->sigma_e[g], h->dt, gca[o] / scale,
+                   gcb[o] / scale, &grad_eps_r[g], &grad_sigma[g]);
+      }
+  return CEEP_OK;
+}
+
+// The three-kernel adjoint (A, B, C per reversed step): gradient sums of ca
+// and cb into gca / gcb (host, one plane, scaled like dtr).
+int adjoint3d_abc(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<float>& gca,
+                  std::vector<float>& gcb) {
+  int rc;
+  const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
+  const long long nbox = h->box_floats();
+  const size_t ntr = dtr.size();
   const int bb = h->bb >= 8 ? 8 : h->bb >= 4 ? 4 : (h->bb >= 2 ? 2 : 1);
   const int nchunks = (B + bb - 1) / bb;
   const bool use[6] = {true, true, true, h->has_debye, h->has_lorentz, h->has_lorentz};
@@ -1419,19 +1464,80 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
     CEEP_HIP(ceep::launch_sum_parts(h->g_ca, nchunks, h->plane, h->plane, h->g_ca, st));
     CEEP_HIP(ceep::launch_sum_parts(h->g_cbi, nchunks, h->plane, h->plane, h->g_cbi, st));
   }
-  std::vector<float> gca(h->plane), gcb(h->plane);
   CEEP_HIP(hipMemcpyAsync(gca.data(), h->g_ca, gca.size() * sizeof(float), hipMemcpyDeviceToHost, st));
   CEEP_HIP(hipMemcpyAsync(gcb.data(), h->g_cbi, gcb.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-  if ((rc = zero_state(h))) return rc;   // the recomputation overwrote the fields
-  CEEP_HIP(hipStreamSynchronize(st));
-  for (int x = 1; x < h->M[0] - 1; ++x)
-    for (int y = 1; y < h->M[1] - 1; ++y)
-      for (int z = 1; z < h->M[2] - 1; ++z) {
-        const long long o = h->box_index(x, y, z);
-        const long long g = h->full_index(x + h->org, y + h->org, z + h->org);
-        chain_rule(h->eps_r[g], eps_add_at(h, g), h->sigma_e[g], h->dt, gca[o] / scale,
-                   gcb[o] / scale, &grad_eps_r[g], &grad_sigma[g]);
-      }
+  return CEEP_OK;
+}
+
+// The blocked adjoint: adj3d_k reversed steps per launch (the segment's last
+// launch takes what is left), each a full A, B and C, the state in two sets.
+int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<float>& gca,
+                      std::vector<float>& gcb) {
+  int rc;
+  const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
+  const long long nbox = h->box_floats();
+  const size_t ntr = dtr.size();
+  const int nj = h->nj();
+  hipStream_t st = h->stream;
+  if (!h->adjtb && (rc = dalloc(&h->adjtb, (size_t)12 * nbox))) return rc;
+  if (nj && !h->adjtb_j && (rc = dalloc(&h->adjtb_j, (size_t)2 * nj * nbox))) return rc;
+  if (!h->g3tb && (rc = dalloc(&h->g3tb, (size_t)2 * nbox))) return rc;
+  if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
+  CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr.data(), ntr * sizeof(float), hipMemcpyHostToDevice, st));
+  CEEP_HIP(ceep::launch_zero(h->adjtb, (size_t)12 * nbox * sizeof(float), st));
+  if (nj) CEEP_HIP(ceep::launch_zero(h->adjtb_j, (size_t)2 * nj * nbox * sizeof(float), st));
+  CEEP_HIP(ceep::launch_zero(h->g3tb, (size_t)2 * nbox * sizeof(float), st));
+  ceep::AdjTB3DArgs t{};
+  t.Mx = h->M[0]; t.My = h->M[1]; t.Mz = h->M[2]; t.pitch = h->pitch; t.B = B;
+  t.sx = h->row_block; t.bstride = h->plane; t.nbox = nbox;
+  for (int c = 0; c < 3; ++c) {
+    t.lam[c] = h->adjtb + (size_t)2 * c * nbox;
+    t.hbar[c] = h->adjtb + (size_t)(6 + 2 * c) * nbox;
+  }
+  t.jp = nj ? h->adjtb_j : nullptr;
+  t.mp = h->mpack;
+  t.magnetic = h->da != nullptr;
+  t.ch = h->ch; t.idx = h->idx;
+  t.src_cell = h->src3; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
+  t.T = T; t.P = P; t.dtr = h->d_dtr;
+  t.probe_head = h->probe_head; t.probe_next = h->probe_next;
+  t.g = h->g3tb; t.gstride = h->plane;
+  t.order = h->xcd3d ? 1 : 0;
+  // lam_z^T = dL/dtrace(T-1) at the probes (set 0)
+  CEEP_HIP(ceep::launch_adj3d_seed(t.lam[2], h->probe_cell3, h->d_dtr, B, P, T, T - 1, h->plane, st));
+  const size_t nstate = state3d(h, 0).size();
+  const size_t slot = (size_t)3 * nbox;
+  int set = 0;
+  for (int k = h->ck_count - 1; k >= 0; --k) {
+    const int s0 = k * every, n = std::min(every, T - s0);
+    if (k != h->estore_seg) {   // recompute the segment from its checkpoint, storing E
+      const std::vector<float*> live = state3d(h, 0);
+      for (size_t i = 0; i < nstate; ++i)
+        CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + i], live[i], nbox, st));
+      for (int c = 0; c < 3; ++c)
+        CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + c], h->estore + (size_t)c * nbox, nbox, st));
+      if ((rc = launch_range(h, 0, s0, n, h->estore))) return rc;
+      h->estore_seg = k;
+    }
+    for (int s = s0 + n - 1; s >= s0;) {
+      const int K = std::min(h->adj3d_k, s - s0 + 1);
+      t.in_set = set;
+      t.step = s;
+      t.e_lo = h->estore + (size_t)(s - K + 1 - s0) * slot;
+      t.tiles_y = (t.My - 1 + (16 - 2 * K) - 1) / (16 - 2 * K);
+      t.tiles_z = (t.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K);
+      CEEP_HIP(ceep::launch_adj3d_tb(t, K, h->poles(), st));
+      set ^= 1;
+      s -= K;
+    }
+  }
+  if (B > 1)   // fold the per-TX planes into plane 0 of each (in place: plane 0 is read first)
+    for (int c = 0; c < 2; ++c)
+      CEEP_HIP(ceep::launch_sum_parts(h->g3tb + (size_t)c * nbox, B, h->plane, h->plane,
+                                      h->g3tb + (size_t)c * nbox, st));
+  CEEP_HIP(hipMemcpyAsync(gca.data(), h->g3tb, gca.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipMemcpyAsync(gcb.data(), h->g3tb + nbox, gcb.size() * sizeof(float),
+                          hipMemcpyDeviceToHost, st));
   return CEEP_OK;
 }
 
@@ -1721,6 +1827,10 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 17) {
     if (value != 1 && value != 2) return fail(CEEP_EINVAL, "3D blocked steps per launch must be 1 or 2");
     h->tb3d_k = value;
+  } else if (key == 19) {
+    if (value < 0 || value > 2)
+      return fail(CEEP_EINVAL, "3D adjoint steps per launch must be 0 (three-kernel form), 1 or 2");
+    h->adj3d_k = value;
   } else if (key == 16) {
     if (value != 1 && value != 4) return fail(CEEP_EINVAL, "forward wavefronts per window must be 1 or 4");
     h->fwd_waves = value;

@@ -200,7 +200,15 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          chunk) fastest, 3 = blocked form only: per XCD in groups of key-18
  *          TX, then y tiles.  Results are identical; 0 measured fastest.
  *   key 17: 3D blocked form: steps per launch, 1 or 2 (default 2).
- *   key 18: 3D blocked form, order 3: TX per group (default 4). */
+ *   key 18: 3D blocked form, order 3: TX per group (default 4).
+ *   key 19: 3D gradient with the blocked forward (key 6 = 4): reversed steps
+ *          per adjoint launch, 1 or 2 (default 2; one launch runs A, B and C
+ *          of each of its steps on 16 x 64 windows marching along x), or 0 =
+ *          three kernels per reversed step.  The adjoint fields are
+ *          bit-identical between the forms; the blocked launches sum the
+ *          gradient per TX and add the TX at the end, the three-kernel order
+ *          when key 0 is 1.  The three kernels run instead where
+ *          (2 x B + 1) x box volume x 4 B reaches 2^31. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,

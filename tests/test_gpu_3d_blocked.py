@@ -1,5 +1,6 @@
-"""The temporally blocked 3D step (two steps per launch, step form 4) against
-the one-step H-then-E kernels and the FP64 oracle (needs a GPU).
+"""The temporally blocked 3D step (two steps per launch, step form 4) and the
+blocked time-reversed step against the one-step H-then-E kernels, the
+three-kernel adjoint and the FP64 oracle (needs a GPU).
 
 The blocked kernel performs the same operations in the same order as the
 one-step kernels, so traces, fields and gradients are compared bit for bit;
@@ -117,36 +118,87 @@ def test_blocked_3d_equals_one_step(amd, kinds, magnetic, shape, T):
     assert out[4][4] == out[0][4]
 
 
-@pytest.mark.parametrize("every", [17, 30])
-def test_blocked_3d_gradient_equals_one_step(amd, every):
-    """Checkpoints, the last segment's E store and the recomputation of the
-    other segments all run on the blocked kernel: dL/d eps_r and dL/d sigma
-    equal the one-step form's bit for bit."""
+def _gradients(amd, shape, n, T, every, kinds, magnetic, tunings, prb=None, seed=0):
+    """dL/d eps_r and dL/d sigma of an S-matrix misfit under each tuning."""
     from ceep_amd import adjoint
-    shape, n, T, dx = (26, 24, 70), 3, 75, 1e-3
+    dx = 1e-3
     dt = W.dt_3d_courant(dx, 0.5)
     w = W.modulated_gaussian(T, dt, 30e9, 40e9, delay_factor=3.0)
-    eps, sig, m = _maps(shape, 0)
-    eps_t, sig_t, _ = _maps(shape, 1)
-    kinds = ("debye", "lorentz")
-    src = [(10, 11, 30), (15, 12, 40)]
-    prb = [(16, 9, 33), (9, 14, 41), (12, 16, 25)]
+    eps, sig, m = _maps(shape, seed)
+    eps_t, sig_t, _ = _maps(shape, seed + 1)
+    rng = np.random.default_rng(9)
+    mu = 1.0 + rng.uniform(0, 1, shape) if magnetic else None
+    sm = rng.uniform(0, 3e3, shape) if magnetic else None
+    c = [s // 2 for s in shape]
+    src = [tuple(c), (c[0] - 3, c[1] + 2, c[2] + 5)]
+    prb = prb or [(c[0] + 3, c[1] - 2, c[2] + 4), (c[0] - 4, c[1] + 3, c[2] - 6)]
     freqs = np.linspace(10e9, 45e9, 6)
-    res = {}
-    for form in (0, 4):
-        s = _solver(amd, shape, n, T, dx, src, prb, w, eps_t, sig_t, m, kinds)
-        s.tuning = {"fused_3d": form}
+    res = []
+    for tun in tunings:
+        s = _solver(amd, shape, n, T, dx, src, prb, w, eps_t, sig_t, m, kinds, mu, sm)
+        s.tuning = dict(tun)
         s.run()
         St = s.s_parameters(freqs).s_matrix
-        s = _solver(amd, shape, n, T, dx, src, prb, w, eps, sig, m, kinds)
-        s.tuning = {"fused_3d": form}
-        print(f"3D blocked gradient check: every={every} form={form}", flush=True)
-        res[form] = adjoint.gradient(s, eps, sig, freqs, St, every=every)
-    print(f"  loss {res[4][0]:.6e} vs {res[0][0]:.6e}; bitwise grads "
-          f"{np.array_equal(res[4][1], res[0][1]) and np.array_equal(res[4][2], res[0][2])}")
-    assert res[4][0] == res[0][0]
-    assert np.array_equal(res[4][1], res[0][1]) and np.array_equal(res[4][2], res[0][2])
-    assert np.abs(res[4][1]).max() > 0
+        s = _solver(amd, shape, n, T, dx, src, prb, w, eps, sig, m, kinds, mu, sm)
+        s.tuning = dict(tun)
+        print(f"3D gradient: {kinds} magnetic={magnetic} shape={shape} T={T} every={every} {tun}",
+              flush=True)
+        res.append(adjoint.gradient(s, eps, sig, freqs, St, every=every))
+    return res
+
+
+# one TX per chunk (key 0 = 1): the three-kernel adjoint then sums the
+# gradient in the blocked launches' order, so every pair is bit-identical
+ONE_STEP = {"fused_3d": 0, "batch_per_thread": 1}
+
+
+@pytest.mark.parametrize("every,adj_depth", [(17, 2), (30, 2), (17, 1), (30, 0)])
+def test_blocked_3d_gradient_equals_one_step(amd, every, adj_depth):
+    """Checkpoints, the last segment's E store, the recomputation of the other
+    segments and the time-reversed steps (blocked launches of adj_depth
+    reversed steps, segment lengths 17 and 30 leaving one-step remainders,
+    or the three kernels) against the one-step forms: dL/d eps_r and
+    dL/d sigma equal bit for bit."""
+    blocked = {"fused_3d": 4, "batch_per_thread": 1, "adjoint_temporal_depth_3d": adj_depth}
+    r0, r4 = _gradients(amd, (26, 24, 70), 3, 75, every, ("debye", "lorentz"), False,
+                        [ONE_STEP, blocked])
+    print(f"  loss {r4[0]:.6e} vs {r0[0]:.6e}; bitwise grads "
+          f"{np.array_equal(r4[1], r0[1]) and np.array_equal(r4[2], r0[2])} (rel-L2 "
+          f"{rel_l2(r4[1], r0[1]):.2e}, {rel_l2(r4[2], r0[2]):.2e})")
+    assert r4[0] == r0[0]
+    assert np.array_equal(r4[1], r0[1]) and np.array_equal(r4[2], r0[2])
+    assert np.abs(r4[1]).max() > 0
+
+
+@pytest.mark.parametrize("kinds,magnetic,shape,T,every", [
+    ((), False, (30, 44, 90), 41, 41),
+    (("debye",), True, (26, 40, 75), 40, 13),
+    (("lorentz",), False, (24, 22, 20), 37, 37),
+    (("debye", "lorentz"), True, (30, 44, 90), 43, 20),
+], ids=["plain", "debye-mag", "lorentz-small", "debye+lorentz-mag"])
+def test_blocked_3d_adjoint_cases(amd, kinds, magnetic, shape, T, every):
+    """The blocked adjoint on ragged windows in y and z, every pole set, with
+    and without mu_r / sigma_m maps, and probes on the box walls (their seeds
+    meet E cells that never change) and outside the stepped box: gradients
+    equal the three-kernel adjoint's bit for bit, and the default
+    chunking (key 0 = 4, sums per four TX) agrees to FP32 rounding."""
+    c = [s // 2 for s in shape]
+    n = 3
+    prb = [(c[0] + 3, c[1] - 2, c[2] + 4), (n, c[1], c[2]), (c[0], shape[1] - n, c[2]),
+           (c[0], c[1], n), (1, 1, 1), (c[0] - 4, c[1] + 3, c[2] - 6)]
+    blocked = {"fused_3d": 4, "batch_per_thread": 1}
+    three = {"fused_3d": 4, "batch_per_thread": 1, "adjoint_temporal_depth_3d": 0}
+    three4 = {"fused_3d": 4, "batch_per_thread": 4, "adjoint_temporal_depth_3d": 0}
+    rb, r3, r34 = _gradients(amd, shape, n, T, every, kinds, magnetic, [blocked, three, three4],
+                             prb=prb, seed=2)
+    d = max(rel_l2(rb[1], r34[1]), rel_l2(rb[2], r34[2]))
+    print(f"  loss {rb[0]:.6e}; bitwise vs three kernels "
+          f"{np.array_equal(rb[1], r3[1]) and np.array_equal(rb[2], r3[2])}; "
+          f"rel-L2 vs four-TX chunks {d:.2e}")
+    assert rb[0] == r3[0]
+    assert np.array_equal(rb[1], r3[1]) and np.array_equal(rb[2], r3[2])
+    assert np.abs(rb[1]).max() > 0 and np.abs(rb[2]).max() > 0
+    assert d < 1e-5
 
 
 @pytest.mark.parametrize("tuning", [{"xcd_order_3d": 1}, {"xcd_order_3d": 2},

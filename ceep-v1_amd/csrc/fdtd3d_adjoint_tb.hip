@@ -39,6 +39,12 @@ struct ASlab {
   float et[K][3];   // the adjoint of each level's new E, for the gradient sums
 };
 
+template <int K>
+struct EGSlab {
+  float e[K + 1][3];   // E-store slots 0..K
+  float gca, gcb;
+};
+
 struct AMaps {
   float ca, cb, dk1, dk2, lk1, lk2, lk3, da, dbi;
   int ph;
@@ -63,10 +69,18 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int L = xcd_block(p.order != 0);
-  const int ty = L % p.tiles_y;
-  const int r0 = L / p.tiles_y;
-  const int tz = r0 % p.tiles_z;
-  const int bt = r0 / p.tiles_z;
+  int ty, tz, bt;
+  if (p.order == 2) {   // TX fastest: the TX of one tile share its maps in an L2
+    bt = L % p.B;
+    const int t = L / p.B;
+    ty = t % p.tiles_y;
+    tz = t / p.tiles_y;
+  } else {              // y tiles fastest
+    ty = L % p.tiles_y;
+    const int r0 = L / p.tiles_y;
+    tz = r0 % p.tiles_z;
+    bt = r0 / p.tiles_z;
+  }
   const int y = ty * (R - 2 * K) - K + w;
   const int z = tz * (64 - 2 * K) - K + lane;
   const bool yz_in = y >= 0 && y < p.My && z >= 0 && z < p.Mz;
@@ -221,7 +235,21 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   // the gradient sums of every level (level k is reversed step S - k, which
   // reads E-store slots S - k and S - k + 1; slot j of the launch is step
   // S - K + 1 + j)
-  auto finish_a = [&](const ASlab<POLES, K>& c, const AMaps& m, int a) __attribute__((always_inline)) {
+  // the E-store slots and gradient sums of slab a, issued ahead of use
+  // (vector memory counters complete in order: loads issued after the next
+  // slab's prefetch would make their wait wait for the prefetch too)
+  auto load_eg = [&](EGSlab<K>& g, int a) __attribute__((always_inline)) {
+    const unsigned v = slab_ok(a) ? yzs : kOffNone;
+    const unsigned sa = (unsigned)max(a, 0) * sx_b;
+    g.gca = rload(rg, v, sa);
+    g.gcb = rload(rg, v, sa + gcb_b);
+#pragma unroll
+    for (int j = 0; j <= K; ++j)
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) g.e[j][c3] = rload(re[j], v, sa + c3 * nbox_b);
+  };
+  auto finish_a = [&](const ASlab<POLES, K>& c, const AMaps& m, int a,
+                      const EGSlab<K>& g) __attribute__((always_inline)) {
     if (!slab_ok(a)) return;
     const unsigned sa = (unsigned)a * sx_b;
     const unsigned sj = (unsigned)a * sxj_b;
@@ -233,13 +261,9 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
         rstore(c.alp[c3], rj_out, yzjs + 4u * (JL + 3 + c3), sj);
       }
     }
-    float gca = rload(rg, yzs, sa), gcb = rload(rg, yzs, sa + gcb_b);
+    float gca = g.gca, gcb = g.gcb;
     if (e_yz && a >= 1 && a <= p.Mx - 2) {
-      float e[K + 1][3];
-#pragma unroll
-      for (int j = 0; j <= K; ++j)
-#pragma unroll
-        for (int c3 = 0; c3 < 3; ++c3) e[j][c3] = rload(re[j], yzs, sa + c3 * nbox_b);
+      const float(&e)[K + 1][3] = g.e;
       const bool at_src = src_col && a == src_a;
       const int pb = a & 1;
 #pragma unroll
@@ -277,13 +301,20 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   load_maps(mring[3], -1);  // run on it; their results are never stored)
 #pragma unroll
   for (int c3 = 0; c3 < 3; ++c3) ring[3].q[c3] = 0.f;
+  // slab x's E-store slots and gradient sums live in eg[x & 1]: K = 2 loads
+  // them at the start of the iteration that finishes x, K = 1 one iteration
+  // earlier with the slab prefetch
+  EGSlab<K> eg[2];
+  if (K == 1) {
+    load_eg(eg[0], 0);
+    load_eg(eg[1], 1);
+  }
   step_a(ring[0], mring[0], 0, 0);
-  if (K == 1) finish_a(ring[0], mring[0], 0);
+  if (K == 1) finish_a(ring[0], mring[0], 0, eg[0]);
 
   auto iteration = [&](auto phase, int i) __attribute__((always_inline)) {
     constexpr int U = decltype(phase)::value;   // i % 4: slab x lives in ring[x % 4]
-    load_slab(ring[(U + 2) % 4], i + 2);
-    load_maps(mring[(U + 2) % 4], i + 2);
+    if (K == 2) load_eg(eg[U & 1], i);
 #pragma unroll
     for (int k = 0; k < K; ++k) {       // level k + 1: reversed step S - k
       const int xa = i + 1 - k;         // A's slab
@@ -291,7 +322,14 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
       ASlab<POLES, K>& sA = ring[(U + 5 - k) % 4];
       ASlab<POLES, K>& sB = ring[(U + 4 - k) % 4];
       step_a(sA, mring[(U + 5 - k) % 4], xa, k);
-      if (k == K - 1) finish_a(sA, mring[(U + 5 - k) % 4], xa);
+      if (k == K - 1) {
+        finish_a(sA, mring[(U + 5 - k) % 4], xa, eg[(U + 1 - k) & 1]);
+        // the next slab's prefetch, issued after every load the rest of
+        // this iteration waits for
+        load_slab(ring[(U + 2) % 4], i + 2);
+        load_maps(mring[(U + 2) % 4], i + 2);
+        if (K == 1) load_eg(eg[U & 1], i + 2);
+      }
       // B: q at +1 in z (DPP), y (LDS), x (slab xb + 1 = A's)
       xq[k][0][w][lane] = sB.q[0];
       xq[k][1][w][lane] = sB.q[2];

@@ -207,7 +207,8 @@ struct AdjTB3DArgs {
   float* g;                       // [2][B][gstride]: sums for ca, then for cb (as Adj3DArgs)
   long long gstride;
   int tiles_y, tiles_z;           // ceil((My-1)/(16-2K)), ceil((Mz-1)/(64-2K))
-  int order;                      // 0: dispatch order, 1: contiguous tile ranges per XCD
+  int order;                      // 0: dispatch order (y tiles fastest), 1: the same with
+                                  // contiguous ranges per XCD, 2: per XCD with the TX fastest
 };
 
 hipError_t launch_adj3d_tb(const AdjTB3DArgs& a, int K, int poles, hipStream_t s);

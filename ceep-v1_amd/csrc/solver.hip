@@ -178,6 +178,7 @@ struct ceep_fdtd {
   int tb3d_k = 2;              // 3D blocked form: steps per launch (1 or 2)
   int tx_group3d = 4;          // 3D blocked form, block order 3: TX per group
   int adj3d_k = 2;             // 3D adjoint: reversed steps per blocked launch (0: the A/B/C kernels)
+  int adj3d_order = 1;         // 3D blocked adjoint: workgroup order (0 dispatch, 1 per XCD, 2 TX fastest)
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
@@ -1502,7 +1503,7 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
   t.T = T; t.P = P; t.dtr = h->d_dtr;
   t.probe_head = h->probe_head; t.probe_next = h->probe_next;
   t.g = h->g3tb; t.gstride = h->plane;
-  t.order = h->xcd3d ? 1 : 0;
+  t.order = h->adj3d_order;
   // lam_z^T = dL/dtrace(T-1) at the probes (set 0)
   CEEP_HIP(ceep::launch_adj3d_seed(t.lam[2], h->probe_cell3, h->d_dtr, B, P, T, T - 1, h->plane, st));
   const size_t nstate = state3d(h, 0).size();
@@ -1831,6 +1832,9 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 0 || value > 2)
       return fail(CEEP_EINVAL, "3D adjoint steps per launch must be 0 (three-kernel form), 1 or 2");
     h->adj3d_k = value;
+  } else if (key == 20) {
+    if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D adjoint block order must be 0, 1 or 2");
+    h->adj3d_order = value;
   } else if (key == 16) {
     if (value != 1 && value != 4) return fail(CEEP_EINVAL, "forward wavefronts per window must be 1 or 4");
     h->fwd_waves = value;

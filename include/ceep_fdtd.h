@@ -208,7 +208,11 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
  *          bit-identical between the forms; the blocked launches sum the
  *          gradient per TX and add the TX at the end, the three-kernel order
  *          when key 0 is 1.  The three kernels run instead where
- *          (2 x B + 1) x box volume x 4 B reaches 2^31. */
+ *          (2 x B + 1) x box volume x 4 B reaches 2^31.
+ *   key 20: 3D blocked adjoint: workgroup order, 0 = dispatch order (y tiles
+ *          fastest), 1 = the same renumbered so each XCD runs a contiguous
+ *          range (default; y-neighbour tiles share halo rows in an L2),
+ *          2 = per XCD with the TX fastest.  Results are identical. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,

@@ -201,6 +201,18 @@ def test_blocked_3d_adjoint_cases(amd, kinds, magnetic, shape, T, every):
     assert d < 1e-5
 
 
+def test_blocked_3d_adjoint_orders_agree(amd):
+    """Workgroup orders of the blocked adjoint (dispatch, per XCD, TX fastest
+    per XCD) only permute which workgroup owns which window: the
+    gradients equal the default order's bit for bit."""
+    tun = [{"adjoint_order_3d": o} for o in (1, 0, 2)]
+    res = _gradients(amd, (26, 30, 70), 3, 39, 20, ("debye", "lorentz"), False, tun, seed=4)
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        assert np.array_equal(r[1], res[0][1]) and np.array_equal(r[2], res[0][2])
+    assert np.abs(res[0][1]).max() > 0
+
+
 @pytest.mark.parametrize("tuning", [{"xcd_order_3d": 1}, {"xcd_order_3d": 2},
                                     {"xcd_order_3d": 3, "tx_group_3d": 2},
                                     {"xcd_order_3d": 3, "tx_group_3d": 4}])

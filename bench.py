@@ -330,12 +330,13 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
                          "run one after another with the one-step kernel")}
 
 
-def _bench_3d_gradient(T3=1000, every=100):
+def _bench_3d_gradient(T3=1000, every=100, single=False):
     """dL/d eps_r and dL/d sigma over the whole 200^3 volume of the 3D
     workload (Debye + Lorentz head seed 0, 16 TX, 1000 steps), loss = sum
     |S - S_target|^2 over the 101-frequency S-matrix against head seed 1,
     checkpoints every 100 steps with recomputation.  Ratio = gradient wall
-    time over the forward's (reset, run, DFT)."""
+    time over the forward's (reset, run, DFT).  With `single`: one gradient
+    call and nothing timed (for rocprofv3 --pmc)."""
     import ceep_amd
     from ceep_amd import adjoint
     from ceep_amd.constants import cfl_timestep_3d
@@ -360,6 +361,11 @@ def _bench_3d_gradient(T3=1000, every=100):
     S_t = tgt.s_parameters(freqs).s_matrix
     tgt.close()
     s = solver(0)
+    if single:
+        adjoint.gradient(s, s._eps_r.copy(), s._sigma_e.copy(), freqs, S_t, every=every)
+        _device_sync()
+        s.close()
+        return None
     s._build()
     h = s._handle
 
@@ -427,6 +433,9 @@ def main():
                     help="with --single-run: JSON dict of launch settings applied on top of the defaults")
     ap.add_argument("--single-run-3d", action="store_true",
                     help="one 3D simulation (200^3, Debye + Lorentz, 100 steps), no output (for rocprofv3 --pmc)")
+    ap.add_argument("--single-gradient-3d", action="store_true",
+                    help="one 3D gradient (200^3, Debye + Lorentz, 100 steps, checkpoints every 20), "
+                         "no output (for rocprofv3 --pmc)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of 256 MiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -439,6 +448,9 @@ def main():
         return
     if args.single_run_3d:
         _bench_3d(steps_timed=1, T3=100)
+        return
+    if args.single_gradient_3d:
+        _bench_3d_gradient(T3=100, every=20, single=True)
         return
     if args.single_run:
         s, *_ = _workload()

@@ -56,20 +56,51 @@ def traces(solver, eps_r: torch.Tensor, sigma_e: torch.Tensor, every: int = 50,
     return BatchedTraces.apply(eps_r, sigma_e, solver, every, fp64_forward)
 
 
+_BASIS: dict = {}
+
+
+def _dft_basis(T: int, freqs, dt: float) -> torch.Tensor:
+    """(T, 2F) real matrix [cos(w_f s dt) | -sin(w_f s dt)] * dt, cached per
+    (T, freqs, dt): the running DFT as one matrix product."""
+    f64 = np.asarray(freqs, dtype=np.float64)
+    key = (int(T), f64.tobytes(), float(dt))
+    m = _BASIS.get(key)
+    if m is None:
+        f = torch.as_tensor(f64)
+        ph = 2.0 * np.pi * f[None, :] * (torch.arange(T, dtype=torch.float64) * dt)[:, None]
+        m = torch.cat([torch.cos(ph), -torch.sin(ph)], dim=1) * dt
+        if len(_BASIS) >= 8:
+            _BASIS.clear()
+        _BASIS[key] = m
+    return m
+
+
+def _dft_parts(traces_: torch.Tensor, freqs, dt: float) -> torch.Tensor:
+    """(B * P, 2F): real then imaginary parts of dt * sum_s trace[b, p, s] exp(-i w_f s dt)."""
+    B, P, T = traces_.shape
+    return traces_.reshape(B * P, T) @ _dft_basis(T, freqs, dt)
+
+
 def s_matrix(traces_: torch.Tensor, freqs, dt: float) -> torch.Tensor:
     """(F, RX, TX) complex S-matrix: dt * sum_s trace[tx, rx, s] exp(-i w_f s dt)."""
-    T = traces_.shape[-1]
-    f = torch.as_tensor(np.asarray(freqs, dtype=np.float64))
-    ph = 2.0 * np.pi * f[:, None] * (torch.arange(T, dtype=torch.float64) * dt)[None, :]
-    re = torch.einsum("bpt,ft->fpb", traces_, torch.cos(ph)) * dt
-    im = torch.einsum("bpt,ft->fpb", traces_, -torch.sin(ph)) * dt
-    return torch.complex(re, im)
+    B, P, _ = traces_.shape
+    F = len(freqs)
+    y = _dft_parts(traces_, freqs, dt).reshape(B, P, 2, F).permute(2, 3, 1, 0)
+    return torch.complex(y[0], y[1])
 
 
 def s_misfit(traces_: torch.Tensor, freqs, dt: float, s_target: torch.Tensor) -> torch.Tensor:
-    """L = sum over (f, rx, tx) of |S - S_target|^2."""
-    d = s_matrix(traces_, freqs, dt) - s_target
-    return (d.real ** 2 + d.imag ** 2).sum()
+    """L = sum over (f, rx, tx) of |S - S_target|^2 (computed on real and
+    imaginary parts: the same sum without complex autograd)."""
+    B, P, _ = traces_.shape
+    F = len(freqs)
+    st = torch.as_tensor(s_target)
+    if not st.is_complex():
+        st = st.to(torch.complex128)
+    tgt = torch.cat([st.real.permute(2, 1, 0).reshape(B * P, F),
+                     st.imag.permute(2, 1, 0).reshape(B * P, F)], dim=1).to(traces_.dtype)
+    d = _dft_parts(traces_, freqs, dt) - tgt
+    return (d * d).sum()
 
 
 def gradient(solver, eps_r, sigma_e, freqs, s_target, every: int = 50,

@@ -118,7 +118,7 @@ def test_blocked_3d_equals_one_step(amd, kinds, magnetic, shape, T):
     assert out[4][4] == out[0][4]
 
 
-def _gradients(amd, shape, n, T, every, kinds, magnetic, tunings, prb=None, seed=0):
+def _gradients(amd, shape, n, T, every, kinds, magnetic, tunings, prb=None, seed=0, src=None):
     """dL/d eps_r and dL/d sigma of an S-matrix misfit under each tuning."""
     from ceep_amd import adjoint
     dx = 1e-3
@@ -130,7 +130,7 @@ def _gradients(amd, shape, n, T, every, kinds, magnetic, tunings, prb=None, seed
     mu = 1.0 + rng.uniform(0, 1, shape) if magnetic else None
     sm = rng.uniform(0, 3e3, shape) if magnetic else None
     c = [s // 2 for s in shape]
-    src = [tuple(c), (c[0] - 3, c[1] + 2, c[2] + 5)]
+    src = src or [tuple(c), (c[0] - 3, c[1] + 2, c[2] + 5)]
     prb = prb or [(c[0] + 3, c[1] - 2, c[2] + 4), (c[0] - 4, c[1] + 3, c[2] - 6)]
     freqs = np.linspace(10e9, 45e9, 6)
     res = []
@@ -199,6 +199,28 @@ def test_blocked_3d_adjoint_cases(amd, kinds, magnetic, shape, T, every):
     assert np.array_equal(rb[1], r3[1]) and np.array_equal(rb[2], r3[2])
     assert np.abs(rb[1]).max() > 0 and np.abs(rb[2]).max() > 0
     assert d < 1e-5
+
+
+@pytest.mark.parametrize("ntx", [1, 5])
+def test_blocked_3d_adjoint_batch_sizes(amd, ntx):
+    """One TX and five TX (per-TX gradient planes folded at the end), a
+    source on the first interior cell next to the walls, segments of 7
+    steps and a last one of 3 (each ends in a one-step launch), Debye +
+    Lorentz with mu_r / sigma_m maps: bit-identical to the three kernels."""
+    shape, n = (22, 26, 69), 3
+    c = [s // 2 for s in shape]
+    src = [(n + 1, n + 1, n + 1), tuple(c), (c[0] + 2, c[1] - 3, c[2] + 7),
+           (c[0] - 3, c[1] + 4, c[2] - 9), (c[0] + 1, c[1] + 5, c[2] + 3)][:ntx]
+    prb = [(c[0] + 3, c[1] - 2, c[2] + 4), (n + 2, n + 1, n + 3), (c[0] - 4, c[1] + 3, c[2] - 6)]
+    tun = [{"fused_3d": 4, "batch_per_thread": 1},
+           {"fused_3d": 4, "batch_per_thread": 1, "adjoint_temporal_depth_3d": 0}]
+    rb, r3 = _gradients(amd, shape, n, 45, 7, ("debye", "lorentz"), True, tun, prb=prb, seed=6,
+                        src=src)
+    print(f"  {ntx} TX: loss {rb[0]:.6e}; bitwise "
+          f"{np.array_equal(rb[1], r3[1]) and np.array_equal(rb[2], r3[2])}")
+    assert rb[0] == r3[0]
+    assert np.array_equal(rb[1], r3[1]) and np.array_equal(rb[2], r3[2])
+    assert np.abs(rb[1]).max() > 0
 
 
 def test_blocked_3d_adjoint_orders_agree(amd):

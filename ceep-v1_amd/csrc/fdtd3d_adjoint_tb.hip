@@ -94,7 +94,8 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   const unsigned yzs = own ? yzb : kOffNone;          // inner-cell stores, E store, gradient
   const unsigned yzj = yz_in ? yz * (4u * NJ) : kOffNone;
   const unsigned yzjs = own ? yzj : kOffNone;
-  const unsigned yzm = yz_in ? yz * 36u : kOffNone;
+  constexpr MapPack MPK = map_pack(POLES, MAG);
+  const unsigned yzm = yz_in ? yz * (4u * MPK.n) : kOffNone;
   // probe heads of every window cell: with K = 2 level 1's halo cells feed
   // level 2, so their lam needs its seeds too
   const unsigned yzp = yz_in ? yz * 2u : kOffNone;
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   const unsigned nbox_b = (unsigned)(p.nbox * 4);
   const unsigned sx_b = (unsigned)(p.sx * 4);
   const unsigned sxj_b = sx_b * NJ;
-  const unsigned sxm_b = sx_b * 9;
+  const unsigned sxm_b = sx_b * MPK.n;
   __amdgpu_buffer_rsrc_t rl[3], rh[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   const __amdgpu_buffer_rsrc_t rj_out =
       plane_rsrc(NJ ? p.jp + ((long long)(p.in_set ^ 1) * p.B + bt) * p.bstride * NJ : p.mp,
                  jbytes);
-  const __amdgpu_buffer_rsrc_t rm = plane_rsrc(p.mp, (unsigned)(p.bstride * 36));
+  const __amdgpu_buffer_rsrc_t rm = plane_rsrc(p.mp, (unsigned)(p.bstride * 4 * MPK.n));
   // E-store slots of steps S - K + 1 .. S + 1, one resource each (component
   // c at + c * nbox; the host keeps (2 nbox + bstride) * 4 below 2^31)
   __amdgpu_buffer_rsrc_t re[K + 1];
@@ -170,13 +171,19 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
     m.dbi = p.ch;
     m.ca = rload(rm, v, so);
     m.cb = rload(rm, v + 4u, so);
-    if (POLES & 1) { m.dk1 = rload(rm, v + 8u, so); m.dk2 = rload(rm, v + 12u, so); }
-    if (POLES & 2) {
-      m.lk1 = rload(rm, v + 16u, so);
-      m.lk2 = rload(rm, v + 20u, so);
-      m.lk3 = rload(rm, v + 24u, so);
+    if (POLES & 1) {
+      m.dk1 = rload(rm, v + 4u * MPK.dk, so);
+      m.dk2 = rload(rm, v + 4u * (MPK.dk + 1), so);
     }
-    if (MAG) { m.da = rload(rm, v + 28u, so); m.dbi = rload(rm, v + 32u, so); }
+    if (POLES & 2) {
+      m.lk1 = rload(rm, v + 4u * MPK.lk, so);
+      m.lk2 = rload(rm, v + 4u * (MPK.lk + 1), so);
+      m.lk3 = rload(rm, v + 4u * (MPK.lk + 2), so);
+    }
+    if (MAG) {
+      m.da = rload(rm, v + 4u * MPK.mag, so);
+      m.dbi = rload(rm, v + 4u * (MPK.mag + 1), so);
+    }
     m.ph = probes ? rload_short(rp, ok ? yzp : kOffNone, (unsigned)max(a, 0) * (unsigned)(sx * 2))
                   : -1;
   };

@@ -161,7 +161,7 @@ struct TB3DArgs {
   int in_set;
   float* e[3]; float* h[3];       // set-0 bases
   float* jp;                      // packed ADE currents (null without poles)
-  const float* mp;                // packed maps [cell][9]: ca, cb, dk1, dk2dt, lk1, lk2dt, lk3, da, dbi
+  const float* mp;                // packed maps [cell][map_pack(poles, magnetic).n]
   int magnetic;                   // da / dbi vary per cell (else da = 1, dbi = ch)
   float ch, idx;
   const long long* src_cell;
@@ -180,6 +180,17 @@ struct TB3DArgs {
 
 hipError_t launch_step3d_tb(const TB3DArgs& a, int K, int poles, hipStream_t s);
 
+// Packed 3D maps of the blocked kernels: per cell ca, cb, then Debye k1,
+// k2/dt (poles & 1), Lorentz k1, k2/dt, k3 (poles & 2), da, dbi (magnetic);
+// absent groups take no room.  Offsets and record length in floats.
+struct MapPack {
+  int dk, lk, mag, n;
+};
+constexpr MapPack map_pack(int poles, bool magnetic) {
+  return MapPack{2, 2 + ((poles & 1) ? 2 : 0), 2 + ((poles & 1) ? 2 : 0) + ((poles & 2) ? 3 : 0),
+                 2 + ((poles & 1) ? 2 : 0) + ((poles & 2) ? 3 : 0) + (magnetic ? 2 : 0)};
+}
+
 // Temporally blocked time-reversed 3D step: reversed steps step, step - 1,
 // ..., step - K + 1 (K = 1 or 2; each A, B and C of the three-kernel form)
 // on 16 x 64 (y, z) windows marching along x.  lam and hbar per component
@@ -192,7 +203,7 @@ struct AdjTB3DArgs {
   int in_set;
   float* lam[3]; float* hbar[3];  // set-0 bases
   float* jp;                      // packed adjoint currents (null without poles)
-  const float* mp;                // packed maps [cell][9] as TB3DArgs::mp
+  const float* mp;                // packed maps as TB3DArgs::mp
   int magnetic;
   float ch, idx;
   const float* e_lo;              // E store slot of step - K + 1 ([3][nbox] per slot), then

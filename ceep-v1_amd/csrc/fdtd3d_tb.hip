@@ -92,7 +92,8 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
   const unsigned yzs = own ? yzb : kOffNone;
   const unsigned yzj = yz_in ? yz * (4u * NJ) : kOffNone;
   const unsigned yzjs = own ? yzj : kOffNone;
-  const unsigned yzm = yz_in ? yz * 36u : kOffNone;
+  constexpr MapPack MPK = map_pack(POLES, MAG);
+  const unsigned yzm = yz_in ? yz * (4u * MPK.n) : kOffNone;
   const unsigned yzp = own ? yz * 2u : kOffNone;   // probe heads: inner cells only
 
   // resources (the host keeps (nbox + bstride) * 4 and bstride * 36 below 2^31)
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
   const unsigned out_b = (unsigned)((p.in_set ^ 1) * p.nbox * 4);
   const unsigned sx_b = (unsigned)(p.sx * 4);
   const unsigned sxj_b = sx_b * NJ;
-  const unsigned sxm_b = sx_b * 9;
+  const unsigned sxm_b = sx_b * MPK.n;
   __amdgpu_buffer_rsrc_t re[3], rh[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
   const __amdgpu_buffer_rsrc_t rj_out =
       plane_rsrc(NJ ? p.jp + ((long long)(p.in_set ^ 1) * p.B + bt) * p.bstride * NJ : p.mp,
                  jbytes);
-  const __amdgpu_buffer_rsrc_t rm = plane_rsrc(p.mp, (unsigned)(p.bstride * 36));
+  const __amdgpu_buffer_rsrc_t rm = plane_rsrc(p.mp, (unsigned)(p.bstride * 4 * MPK.n));
   const bool probes = p.probe_head != nullptr;
   const __amdgpu_buffer_rsrc_t rp = plane_rsrc(
       probes ? reinterpret_cast<const float*>(p.probe_head) : p.mp, (unsigned)(p.bstride * 2));
@@ -160,13 +161,19 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
     const unsigned so = (unsigned)max(a, 0) * sxm_b;
     m.ca = rload(rm, v, so);
     m.cb = rload(rm, v + 4u, so);
-    if (POLES & 1) { m.dk1 = rload(rm, v + 8u, so); m.dk2 = rload(rm, v + 12u, so); }
-    if (POLES & 2) {
-      m.lk1 = rload(rm, v + 16u, so);
-      m.lk2 = rload(rm, v + 20u, so);
-      m.lk3 = rload(rm, v + 24u, so);
+    if (POLES & 1) {
+      m.dk1 = rload(rm, v + 4u * MPK.dk, so);
+      m.dk2 = rload(rm, v + 4u * (MPK.dk + 1), so);
     }
-    if (MAG) { m.da = rload(rm, v + 28u, so); m.dbi = rload(rm, v + 32u, so); }
+    if (POLES & 2) {
+      m.lk1 = rload(rm, v + 4u * MPK.lk, so);
+      m.lk2 = rload(rm, v + 4u * (MPK.lk + 1), so);
+      m.lk3 = rload(rm, v + 4u * (MPK.lk + 2), so);
+    }
+    if (MAG) {
+      m.da = rload(rm, v + 4u * MPK.mag, so);
+      m.dbi = rload(rm, v + 4u * (MPK.mag + 1), so);
+    }
     // probe head: read (and masked to the inner cells) where it is used, so
     // this prefetch is not waited for here
     m.ph = probes ? rload_short(rp, ok ? yzp : kOffNone, (unsigned)max(a, 0) * (unsigned)(p.sx * 2))

@@ -357,14 +357,16 @@ int commit_materials(ceep_fdtd* h) {
   dfree(h->mpack);
   dfree(h->jpack);
   if (h->tb3d()) {
-    std::vector<float> mp((size_t)nbox * 9, 0.f);
+    // only the groups the kernels read for this material set (no room for
+    // absent poles or, without mu_r / sigma_m maps, for da and dbi)
+    const ceep::MapPack mk = ceep::map_pack(h->poles(), !da.empty());
+    std::vector<float> mp((size_t)nbox * mk.n, 0.f);
     for (long long o = 0; o < nbox; ++o) {
-      float* q = &mp[(size_t)o * 9];
+      float* q = &mp[(size_t)o * mk.n];
       q[0] = ca[o]; q[1] = cb[o];
-      if (h->has_debye) { q[2] = dk1[o]; q[3] = dk2[o]; }
-      if (h->has_lorentz) { q[4] = lk1[o]; q[5] = lk2[o]; q[6] = lk3[o]; }
-      q[7] = da.empty() ? 1.f : da[o];
-      q[8] = da.empty() ? h->ch : dbi[o];
+      if (h->has_debye) { q[mk.dk] = dk1[o]; q[mk.dk + 1] = dk2[o]; }
+      if (h->has_lorentz) { q[mk.lk] = lk1[o]; q[mk.lk + 1] = lk2[o]; q[mk.lk + 2] = lk3[o]; }
+      if (!da.empty()) { q[mk.mag] = da[o]; q[mk.mag + 1] = dbi[o]; }
     }
     if ((rc = dalloc(&h->mpack, mp.size())) || (rc = upload_f32(h->mpack, mp, s))) return rc;
     CEEP_HIP(hipStreamSynchronize(s));   // mp goes out of scope

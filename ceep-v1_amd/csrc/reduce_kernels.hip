@@ -24,6 +24,9 @@ __global__ __launch_bounds__(256) void dft_kernel(const float* __restrict__ trac
   const int tx = pair / P, rx = pair % P;
   const float* __restrict__ tr = traces + (long long)pair * T;
   float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
+  // unrolled so the trace and twiddle loads of several samples are in flight
+  // together (the sums keep their sample order)
+#pragma unroll 32
   for (int s = 0; s < T; ++s) {
     const float v = tr[s];
     const float2 w = tw[(long long)s * F + f];

@@ -16,6 +16,14 @@
 // done, from the et of every level and the K + 1 E-store slots.  Slab Mx-1,
 // row My-1 and column Mz-1 hold zeros in every step.
 //
+// With K = 2, level 1's adjoint currents and et wait in LDS (one slot per
+// thread, so no barrier) from its A on a slab to level 2's A on that slab an
+// iteration later, which keeps the Debye + Lorentz case within 128 VGPRs.
+// The E-store and gradient loads of a slab are issued before the next
+// slab's prefetch: vector memory counters complete in order, so waiting for
+// loads issued after the prefetch would wait for the prefetch as well.  The
+// maps are the forward's packed records (map_pack).
+//
 // State: lam and hbar per component in two sets (one allocation of
 // 2 * nbox floats each), the adjoint currents packed per cell in two sets
 // ([set][B][cell][nj]: Debye x, y, z; Lorentz x, y, z; Lorentz one reversed

@@ -8,7 +8,11 @@ Workload (one bench "step" = one full simulation of one batch): 2D TMz,
 the running DFT of all 256 traces at 301 frequencies over 0.5-3 GHz into the
 (301, 16, 16) S-matrix.  Inputs are resident on the device before timing.
 
-Prints one JSON line (see DESIGN.md for the byte model behind `roofline`).
+Prints one JSON line.  Its `roofline` counts the algorithmic bytes of the
+dominant kernel's launch: every stepped cell's fields and ADE current read and
+written once per TX, plus its maps once, times the steps per launch, over the
+launch's mean HIP-event time; `traffic` is the measured HBM bytes per launch
+from the committed rocprofv3 counter summary.
 """
 from __future__ import annotations
 

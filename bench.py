@@ -128,13 +128,11 @@ def _cpu_openmp(ph, pos, w, dt, steps=100):
             "sample": f"first {steps} of {T} steps, FP64 C restatement, {threads} OpenMP threads, {el:.2f} s"}
 
 
-def _adjoint_ratio(every=50):
-    """fwd+adjoint / fwd on the gradient workload: 600x600, 16 TX/RX ring, 800
-    steps, seeded lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells),
-    loss = |S - S_target|^2 at 16 frequencies in 1-2.5 GHz against a second
-    seed, checkpoints every `every` steps with recomputation."""
+def _gradient_workload():
+    """The 2D gradient workload: 600x600, 16 TX/RX ring r=200, 800 steps, seeded
+    lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells; seed 0) and the
+    S-matrix at 16 frequencies in 1-2.5 GHz of the target disk (seed 1)."""
     import ceep_amd
-    from ceep_amd import adjoint
     from ceep_amd.constants import cfl_timestep_2d
     from ceep_amd.waveforms import modulated_gaussian
 
@@ -157,8 +155,23 @@ def _adjoint_ratio(every=50):
     tgt = solver(eps_t, sig_t)
     tgt.run()
     S_t = tgt.s_parameters(freqs).s_matrix
+    tgt.close()
     s = solver(eps, sig)
     s._build()
+    return s, eps, sig, freqs, S_t
+
+
+def _adjoint_ratio(every=50):
+    """fwd+adjoint / fwd on the gradient workload: 600x600, 16 TX/RX ring, 800
+    steps, seeded lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells),
+    loss = |S - S_target|^2 at 16 frequencies in 1-2.5 GHz against a second
+    seed, checkpoints every `every` steps with recomputation."""
+    import ceep_amd
+    from ceep_amd import adjoint
+    from ceep_amd.constants import cfl_timestep_2d
+    from ceep_amd.waveforms import modulated_gaussian
+
+    s, eps, sig, freqs, S_t = _gradient_workload()
     for _ in range(2):
         s._handle.reset()
         s._handle.run(copy=False)
@@ -440,6 +453,9 @@ def main():
     ap.add_argument("--single-gradient-3d", action="store_true",
                     help="one 3D gradient (200^3, Debye + Lorentz, 100 steps, checkpoints every 20), "
                          "no output (for rocprofv3 --pmc)")
+    ap.add_argument("--single-gradient", action="store_true",
+                    help="one 2D gradient call on the gradient workload (600^2 disk, 16 TX, 800 steps, "
+                         "checkpoints every 50), no output (for rocprofv3 --pmc); --tuning applies")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of 256 MiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -455,6 +471,14 @@ def main():
         return
     if args.single_gradient_3d:
         _bench_3d_gradient(T3=100, every=20, single=True)
+        return
+    if args.single_gradient:
+        from ceep_amd import adjoint
+        s, eps, sig, freqs, S_t = _gradient_workload()
+        if args.tuning:
+            s._handle.apply_tuning(json.loads(args.tuning))
+        adjoint.gradient(s, eps, sig, freqs, S_t, every=50)
+        _device_sync()
         return
     if args.single_run:
         s, *_ = _workload()

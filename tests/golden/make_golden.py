@@ -24,6 +24,12 @@ Vectors written (all float64, ``numpy.savez_compressed``, no pickles):
 * ``cpml3d_*.npz`` -- ``FDTD3D`` with a ``CPML`` boundary
   (boundaries/absorbing.py:567-943) and mu_r / sigma_m: the 3D physical
   absorbing-boundary mode, traces and final fields.
+* ``runcpu_*.npz`` -- ``BatchedFDTD2D.run_cpu()`` (fdtd_2d_batched.py:540-611):
+  interior sources with a lossy disk, and sources / probes on and next to the
+  PEC walls (including corners and duplicated probes).
+* ``waveforms.npz`` -- ``ModulatedGaussianSource.waveform`` and ``value_at``
+  (sources/waveforms.py:97-132) at the configurations' parameters, plus the
+  CFL time steps and constants of core/constants.py:21-27,105-175.
 """
 from __future__ import annotations
 
@@ -291,6 +297,65 @@ def golden_cpml3d():
                             kappa_max=cpml.kappa_max)
 
 
+def golden_run_cpu():
+    """``BatchedFDTD2D.run_cpu()`` (fdtd_2d_batched.py:540-611), the full-grid
+    PEC box: (a) interior sources, a lossy disk, a duplicated probe; (b)
+    sources on each wall, in a corner and next to the walls, probes on the
+    walls and in the corners."""
+    from ceep.core.backend import set_backend
+    from ceep.solvers.fdtd_2d_batched import BatchedFDTD2D
+    set_backend("numpy")
+
+    cases = {
+        "interior": dict(nx=40, ny=34, dx=0.5e-3, T=120, f=4e9, df=5.0,
+                         src=[(20, 17), (5, 30), (33, 9)],
+                         prb=[(20, 17), (5, 30), (33, 8), (20, 17), (10, 25)],
+                         disk=(18, 15, 6, 9.0, 0.8)),
+        "walls": dict(nx=30, ny=28, dx=1e-3, T=90, f=5e9, df=3.0,
+                      src=[(0, 10), (29, 14), (12, 0), (1, 1), (28, 26), (15, 27)],
+                      prb=[(0, 10), (29, 14), (12, 0), (1, 1), (28, 26), (15, 27), (14, 14),
+                           (0, 0), (29, 27), (1, 10)],
+                      disk=(14, 13, 5, 4.0, 0.3)),
+    }
+    for name, c in cases.items():
+        s = BatchedFDTD2D(nx=c["nx"], ny=c["ny"], dx=c["dx"], total_steps=c["T"],
+                          cpml_thickness=4, source_positions=c["src"],
+                          probe_positions=c["prb"], frequency=c["f"], delay_factor=c["df"])
+        cx, cy, r, e, sg = c["disk"]
+        s.set_material_circle(cx, cy, r, eps_r=e, sigma_e=sg)
+        out = s.run_cpu()
+        tr = np.array([[out[b][p] for p in range(len(c["prb"]))] for b in range(len(c["src"]))],
+                      dtype=np.float64)
+        np.savez_compressed(os.path.join(HERE, f"runcpu_{name}.npz"), traces=tr, dt=s.dt,
+                            eps_r=s._eps_r, sigma_e=s._sigma_e, sources=np.array(c["src"]),
+                            probes=np.array(c["prb"]), nx=c["nx"], ny=c["ny"], dx=c["dx"],
+                            frequency=c["f"], delay_factor=c["df"])
+
+
+def golden_waveforms():
+    """``ModulatedGaussianSource.waveform`` / ``value_at`` (sources/waveforms.py:
+    97-132) at the configurations' pulse (2 GHz, 1.5 GHz bandwidth, default
+    amplitude and delay factor) on the 2D (dx 0.5 mm, Courant 0.5) and 3D
+    (dx 1 mm, 0.5 * 0.99) time steps, long enough to pass the envelope's peak,
+    and at non-default amplitude / delay factor; the CFL steps and constants."""
+    from ceep.core import constants as K
+    from ceep.sources.waveforms import ModulatedGaussianSource
+
+    dt2 = K.cfl_timestep_2d(0.5e-3, 0.5e-3, 0.5)
+    dt3 = K.cfl_timestep_3d(1e-3, 1e-3, 1e-3, 0.5) * 0.99
+    out = dict(dt2=dt2, dt3=dt3, c0=K.C_0, eps0=K.EPS_0, mu0=K.MU_0,
+               dt2_aniso=K.cfl_timestep_2d(0.5e-3, 0.7e-3, 0.45),
+               dt3_aniso=K.cfl_timestep_3d(1e-3, 0.8e-3, 1.2e-3, 0.4))
+    src = ModulatedGaussianSource(x=0, y=0, frequency=2e9, bandwidth=1.5e9)
+    out["cfg2d"] = src.waveform(4400, dt2)
+    out["cfg3d"] = src.waveform(2800, dt3)
+    src = ModulatedGaussianSource(x=0, y=0, frequency=18e9, bandwidth=20e9, amplitude=2.5,
+                                  delay_factor=3.0)
+    out["custom"] = src.waveform(300, dt2)
+    out["custom_value_at"] = np.array([src.value_at(k, dt2) for k in range(300)])
+    np.savez_compressed(os.path.join(HERE, "waveforms.npz"), **out)
+
+
 def golden_das():
     """``DelayAndSumBeamformer.reconstruct`` (imaging/beamforming.py:86-313),
     DAS and DMAS, raw and normalised, on a seeded S-matrix."""
@@ -325,4 +390,6 @@ if __name__ == "__main__":
     golden_cpml2d()
     golden_cpml3d()
     golden_das()
+    golden_run_cpu()
+    golden_waveforms()
     print("golden vectors written to", HERE)

@@ -130,8 +130,8 @@ def _cpu_openmp(ph, pos, w, dt, steps=100):
 
 def _gradient_workload():
     """The 2D gradient workload: 600x600, 16 TX/RX ring r=200, 800 steps, seeded
-    lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells; seed 0) and the
-    S-matrix at 16 frequencies in 1-2.5 GHz of the target disk (seed 1)."""
+    lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells; seed 0); the
+    misfit's target is the disk of seed 1, S at 16 frequencies in 1-2.5 GHz."""
     import ceep_amd
     from ceep_amd.constants import cfl_timestep_2d
     from ceep_amd.waveforms import modulated_gaussian
@@ -144,77 +144,125 @@ def _gradient_workload():
     eps, sig = ceep_amd.phantoms.DiskPhantom2D(NX, NY, 0, **kw).get_eps_map(2e9)
     eps_t, sig_t = ceep_amd.phantoms.DiskPhantom2D(NX, NY, 1, **kw).get_eps_map(2e9)
 
-    def solver(e, sg):
+    def solver(e, sg, tuning=None):
         s = ceep_amd.BatchedFDTD2D(nx=NX, ny=NY, dx=DX, total_steps=T, cpml_thickness=N_CPML,
                                    source_positions=pos, probe_positions=pos, frequency=2e9,
                                    courant=COURANT, waveform=w, cpml_R=1e-6)
         s._eps_r[:] = e
         s._sigma_e[:] = sg
+        if tuning is not None:
+            from ceep_amd.autotune import tuning_for
+            s.tuning = {**tuning_for(2, (NX, NY), N_CPML, B, False, False), **tuning}
         return s
 
-    tgt = solver(eps_t, sig_t)
-    tgt.run()
-    S_t = tgt.s_parameters(freqs).s_matrix
-    tgt.close()
     s = solver(eps, sig)
     s._build()
-    return s, eps, sig, freqs, S_t
+    return s, eps, sig, freqs, (eps_t, sig_t), solver, dt
+
+
+def _fp64_fd(solver, dt, freqs, eps, sig, eps_t, sig_t, ge, gs, n_pix=4, seed=6, h=1e-4):
+    """Worst relative deviation of (ge, gs) from FP64 central differences
+    (h = 1e-4) of L = sum |S - S_target|^2 on n_pix seeded pixels among the 400
+    largest |gradient| cells (half eps_r, half sigma; the pixel choice of the
+    600^2 finite-difference test).  The losses come from the engine's FP64
+    forward kernels (bitwise the FP64 oracle) and an FP64 DFT on the host."""
+    s64 = solver(eps, sig, {"fp64_forward": 1})
+    s64._build()
+    ph = np.exp(-2j * np.pi * freqs[:, None] * (np.arange(T) * dt)[None, :]) * dt
+
+    def S_of(e, sg):
+        s64.update_materials(e, sg)
+        return np.einsum("ft,bpt->fpb", ph, s64._handle.run_checkpointed(T))
+
+    St = S_of(eps_t, sig_t)
+
+    def loss(e, sg):
+        return float(np.sum(np.abs(S_of(e, sg) - St) ** 2))
+
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    L64 = loss(eps, sig)
+    for grad, which in ((ge, "eps"), (gs, "sigma")):
+        top = np.argsort(np.abs(grad).ravel())[-400:]
+        for flat in rng.choice(top, size=n_pix // 2, replace=False):
+            a, b = np.unravel_index(flat, grad.shape)
+            base = eps if which == "eps" else sig
+            p, m = base.copy(), base.copy()
+            p[a, b] += h
+            m[a, b] -= h
+            fd = ((loss(p, sig) - loss(m, sig)) if which == "eps"
+                  else (loss(eps, p) - loss(eps, m))) / (2 * h)
+            worst = max(worst, abs(grad[a, b] - fd) / abs(fd))
+    s64.close()
+    return worst, L64, St
 
 
 def _adjoint_ratio(every=50):
-    """fwd+adjoint / fwd on the gradient workload: 600x600, 16 TX/RX ring, 800
-    steps, seeded lossy disk (eps_r 10-60, sigma 0.2-1.5, radius 30-90 cells),
-    loss = |S - S_target|^2 at 16 frequencies in 1-2.5 GHz against a second
-    seed, checkpoints every `every` steps with recomputation."""
-    import ceep_amd
+    """fwd+adjoint / fwd on the gradient workload (see _gradient_workload):
+    loss = sum |S - S_target|^2 at 16 frequencies in 1-2.5 GHz against the
+    target disk, checkpoints every `every` steps, FP32 difference form (the
+    engine steps E(eps) - E(target); the variant the 600^2 finite-difference
+    test checks).  The target's own run is taken once per target and timed
+    apart."""
     from ceep_amd import adjoint
-    from ceep_amd.constants import cfl_timestep_2d
-    from ceep_amd.waveforms import modulated_gaussian
 
-    s, eps, sig, freqs, S_t = _gradient_workload()
+    s, eps, sig, freqs, (eps_t, sig_t), solver, dt = _gradient_workload()
+    h = s._handle
     for _ in range(2):
-        s._handle.reset()
-        s._handle.run(copy=False)
-        s._handle.dft(freqs)
+        h.reset()
+        h.run(copy=False)
+        h.dft(freqs)
     _device_sync()
     t0 = time.perf_counter()
     reps = 3
     for _ in range(reps):
-        s._handle.reset()
-        s._handle.run(copy=False)
-        s._handle.dft(freqs)
+        h.reset()
+        h.run(copy=False)
+        h.dft(freqs)
     _device_sync()
     t_fwd = (time.perf_counter() - t0) / reps
-    def timed(ev, fp64=False):
-        adjoint.gradient(s, eps, sig, freqs, S_t, every=ev, fp64_forward=fp64)        # warm
+
+    def timed(fn):
+        fn()      # warm (allocations)
         _device_sync()
         t0 = time.perf_counter()
         for _ in range(reps):
-            adjoint.gradient(s, eps, sig, freqs, S_t, every=ev, fp64_forward=fp64)
+            out = fn()
         _device_sync()
-        return (time.perf_counter() - t0) / reps
+        return (time.perf_counter() - t0) / reps, out
 
-    t_grad = timed(every)
-    t_grad64 = timed(every, True)
-    s._handle.set_tuning("adjoint_reversible", 0)
-    t_store = timed(every)
-    t_whole = timed(T)
-    s._handle.set_tuning("adjoint_reversible", 1)
+    # the target's run (once per target): the first residual run minus a later one
+    adjoint.set_target(s, eps_t, sig_t)
+    _device_sync()
+    t0 = time.perf_counter()
+    h.run_residual(every)
+    _device_sync()
+    t_first = time.perf_counter() - t0
+    t_res, _ = timed(lambda: h.run_residual(every))
+    t_grad, (loss, ge, gs) = timed(lambda: adjoint.gradient(s, eps, sig, freqs, every=every,
+                                                            target=(eps_t, sig_t)))
+    worst, L64, St64 = _fp64_fd(solver, dt, freqs, eps, sig, eps_t, sig_t, ge, gs)
+    t_grad64, _ = timed(lambda: adjoint.gradient(s, eps, sig, freqs, St64, every=every,
+                                                 fp64_forward=True))
+    s.close()
     return {"ratio": round(t_grad / t_fwd, 3), "ms_forward": round(t_fwd * 1e3, 3),
             "ms_forward_plus_adjoint": round(t_grad * 1e3, 3), "checkpoint_every": every,
-            "ratio_single_segment": round(t_whole / t_fwd, 3),
-            "ratio_recompute_and_e_store": round(t_store / t_fwd, 3),
+            "fd_worst_rel_dev": float(f"{worst:.3e}"),
+            "fd_pixels": 4,
+            "loss": float(f"{loss:.6e}"), "loss_fp64": float(f"{L64:.6e}"),
+            "ms_target_run_once": round((t_first - t_res) * 1e3, 3),
             "ratio_fp64_forward": round(t_grad64 / t_fwd, 3),
-            "note": ("ratio: checkpoints every 50 steps; the adjoint launches rebuild the "
-                     "forward state backward from each segment's end checkpoint (reversible, "
-                     "no recomputation pass); ratio_recompute_and_e_store: each segment "
-                     "recomputed into an E store instead; ratio_single_segment: E of all 800 "
-                     "steps kept from the forward pass (16.7 GB, no recomputation); "
-                     "ratio_fp64_forward: forward and "
-                     "recomputation on FP64 fields (what this misfit needs for an accurate "
-                     "gradient: its residual is 3e-10 of |S|), adjoint FP32, over the FP32 forward"),
+            "note": ("ratio: the FP32 difference form (forward of E(eps) - E(target) with "
+                     "checkpoints every 50 steps, reversible adjoint, chain rule on the device); "
+                     "fd_worst_rel_dev: that gradient against FP64 central differences "
+                     "(h = 1e-4, 2 eps_r + 2 sigma pixels) from the engine's FP64 forward; "
+                     "ms_target_run_once: the target's own run with its E kept at every step, "
+                     "taken once per target (like S_target itself) and outside the ratio; "
+                     "ratio_fp64_forward: the other accurate variant, FP64 forward fields "
+                     "against an FP64 S_target"),
             "workload": ("2D 600x600, 16 TX/RX ring r=200, 800 steps, lossy disk phantom; "
-                         "L = sum |S - S_target|^2 at 16 freqs in 1-2.5 GHz; dL/deps_r, dL/dsigma")}
+                         "L = sum |S - S_target|^2 at 16 freqs in 1-2.5 GHz against the disk "
+                         "of seed 1; dL/deps_r, dL/dsigma")}
 
 
 def _bench_3d(steps_timed=2, T3=1000, tuning=None):
@@ -474,10 +522,10 @@ def main():
         return
     if args.single_gradient:
         from ceep_amd import adjoint
-        s, eps, sig, freqs, S_t = _gradient_workload()
+        s, eps, sig, freqs, tgt, *_ = _gradient_workload()
         if args.tuning:
             s._handle.apply_tuning(json.loads(args.tuning))
-        adjoint.gradient(s, eps, sig, freqs, S_t, every=50)
+        adjoint.gradient(s, eps, sig, freqs, every=50, target=tgt)
         _device_sync()
         return
     if args.single_run:

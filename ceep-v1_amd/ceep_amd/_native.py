@@ -57,6 +57,8 @@ SIGNATURES = {
     "ceep_fdtd_time_runs": (_I, [_P, _I, _PD]),
     "ceep_fdtd_run_checkpointed": (_I, [_P, _I, _PD]),
     "ceep_fdtd_adjoint": (_I, [_P, _PD, _PD, _PD]),
+    "ceep_fdtd_set_reference": (_I, [_P, _PD, _PD]),
+    "ceep_fdtd_run_residual": (_I, [_P, _I, _PD, _PD]),
     "ceep_fdtd_set_tuning": (_I, [_P, _I, _I]),
     "ceep_fdtd_multi_create": (_I, [_I, ctypes.POINTER(_P), _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "ceep_fdtd_multi_run": (_I, [_P, _I]),
@@ -143,6 +145,7 @@ TUNE_TB3D_DEPTH = 17
 TUNE_TX_GROUP_3D = 18
 TUNE_ADJ3D_DEPTH = 19
 TUNE_ADJ3D_ORDER = 20
+TUNE_ADJ_PREFETCH = 21
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
@@ -154,7 +157,7 @@ TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "adjoint_temporal_depth": TUNE_ADJ_DEPTH, "forward_waves": TUNE_FWD_WAVES,
                "temporal_depth_3d": TUNE_TB3D_DEPTH, "tx_group_3d": TUNE_TX_GROUP_3D,
                "adjoint_temporal_depth_3d": TUNE_ADJ3D_DEPTH,
-               "adjoint_order_3d": TUNE_ADJ3D_ORDER}
+               "adjoint_order_3d": TUNE_ADJ3D_ORDER, "adjoint_prefetch": TUNE_ADJ_PREFETCH}
 
 
 class Handle:
@@ -243,6 +246,25 @@ class Handle:
         out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)
         check(self._lib.ceep_fdtd_run_checkpointed(self._h, int(every), dptr(out)))
         return out
+
+    def set_reference(self, eps_r=None, sigma_e=None):
+        """Reference medium of the difference form (None, None clears it)."""
+        if eps_r is None and sigma_e is None:
+            check(self._lib.ceep_fdtd_set_reference(self._h, None, None))
+            return
+        e = self._f64(eps_r, self.shape)
+        s = self._f64(sigma_e, self.shape)
+        check(self._lib.ceep_fdtd_set_reference(self._h, dptr(e), dptr(s)))
+
+    def run_residual(self, every):
+        """(residual, reference) traces: trace(eps) - trace(eps_ref) stepped
+        as a difference, and the reference run's traces; checkpoints for the
+        adjoint as run_checkpointed."""
+        shp = (self.batch, self.n_probes, self.total_steps)
+        res = np.zeros(shp, dtype=np.float64)
+        ref = np.zeros(shp, dtype=np.float64)
+        check(self._lib.ceep_fdtd_run_residual(self._h, int(every), dptr(res), dptr(ref)))
+        return res, ref
 
     def adjoint(self, dtraces):
         g = np.ascontiguousarray(np.asarray(dtraces, dtype=np.float64))

@@ -24,6 +24,13 @@ the trace is dominated by the source pulse itself, and a scattered signal of
 600 x 600, 800-step workload: 3e-10) is below FP32 trace accuracy (~1e-6), so
 an FP32 forward returns a residual -- and a gradient -- dominated by rounding.
 
+``target=(eps_r_target, sigma_target)`` is the FP32 alternative for a misfit
+against the S-matrix of a known target medium.  The engine steps the
+difference E(eps) - E(target) itself (``ceep_fdtd_run_residual``: the target's
+run is taken once and its E read back where the media differ), so the
+residual ``S - S_target`` is rounded as a quantity of its own size instead of
+as the difference of two nearly equal FP32 totals.  The adjoint is unchanged.
+
 Import torch before the first solver is built (see ``_native``).
 """
 from __future__ import annotations
@@ -41,6 +48,7 @@ class BatchedTraces(torch.autograd.Function):
         solver._handle.set_tuning("fp64_forward", int(bool(fp64_forward)))
         tr = solver._handle.run_checkpointed(int(every))
         solver.probe_data = tr
+        solver._residual_run = False
         ctx.solver = solver
         return torch.from_numpy(tr.copy())
 
@@ -48,6 +56,51 @@ class BatchedTraces(torch.autograd.Function):
     def backward(ctx, g):
         ge, gs = ctx.solver._handle.adjoint(g.detach().cpu().numpy())
         return torch.from_numpy(ge), torch.from_numpy(gs), None, None, None
+
+
+class BatchedResidual(torch.autograd.Function):
+    """residual = F(eps_r, sigma) - F(eps_target, sigma_target) for a built
+    ``BatchedFDTD2D`` whose handle holds the target as its reference medium."""
+
+    @staticmethod
+    def forward(ctx, eps_r, sigma_e, solver, every):
+        solver.update_materials(eps_r.detach().cpu().numpy(), sigma_e.detach().cpu().numpy())
+        res, ref = solver._handle.run_residual(int(every))
+        solver.probe_data = ref + res
+        solver._residual_run = True
+        ctx.solver = solver
+        return torch.from_numpy(res)
+
+    @staticmethod
+    def backward(ctx, g):
+        ge, gs = ctx.solver._handle.adjoint(g.detach().cpu().numpy())
+        return torch.from_numpy(ge), torch.from_numpy(gs), None, None
+
+
+def set_target(solver, eps_r, sigma_e) -> None:
+    """Make (eps_r, sigma_e) the reference medium of ``solver``'s difference
+    form (kept until changed; its run is taken at the next residual run)."""
+    if not solver._built:
+        solver._build()
+    e = np.ascontiguousarray(eps_r, dtype=np.float64)
+    s = np.ascontiguousarray(sigma_e, dtype=np.float64)
+    solver._handle.set_reference(e, s)
+    solver._target_maps = (e.copy(), s.copy())
+
+
+def _ensure_target(solver, target) -> None:
+    eps_t, sig_t = target
+    cur = getattr(solver, "_target_maps", None)
+    if cur is not None and (cur[0] is eps_t or np.array_equal(cur[0], eps_t)) and \
+            (cur[1] is sig_t or np.array_equal(cur[1], sig_t)):
+        return
+    set_target(solver, eps_t, sig_t)
+
+
+def residual_traces(solver, eps_r: torch.Tensor, sigma_e: torch.Tensor,
+                    every: int = 50) -> torch.Tensor:
+    """(B, P, T) float64 trace(eps) - trace(target), differentiable in eps_r and sigma_e."""
+    return BatchedResidual.apply(eps_r, sigma_e, solver, every)
 
 
 def traces(solver, eps_r: torch.Tensor, sigma_e: torch.Tensor, every: int = 50,
@@ -103,12 +156,30 @@ def s_misfit(traces_: torch.Tensor, freqs, dt: float, s_target: torch.Tensor) ->
     return (d * d).sum()
 
 
-def gradient(solver, eps_r, sigma_e, freqs, s_target, every: int = 50,
-             fp64_forward: bool = False):
-    """(L, dL/d eps_r, dL/d sigma) as NumPy for the S misfit at ``freqs``."""
+def s_norm2(traces_: torch.Tensor, freqs, dt: float) -> torch.Tensor:
+    """sum over (f, rx, tx) of |S|^2 of the traces' S-matrix."""
+    d = _dft_parts(traces_, freqs, dt)
+    return (d * d).sum()
+
+
+def gradient(solver, eps_r, sigma_e, freqs, s_target=None, every: int = 50,
+             fp64_forward: bool = False, target=None):
+    """(L, dL/d eps_r, dL/d sigma) as NumPy for L = sum |S - S_target|^2 at ``freqs``.
+
+    ``s_target``: the target S-matrix (F, RX, TX); the residual is S minus it.
+    ``target=(eps_r_t, sigma_t)`` instead: S_target is the S-matrix of that
+    medium, and the residual is stepped as a difference in FP32."""
     e = torch.as_tensor(np.asarray(eps_r, dtype=np.float64)).clone().requires_grad_(True)
     s = torch.as_tensor(np.asarray(sigma_e, dtype=np.float64)).clone().requires_grad_(True)
-    tr = traces(solver, e, s, every, fp64_forward)
-    loss = s_misfit(tr, freqs, solver.dt, torch.as_tensor(s_target))
+    if target is not None:
+        if s_target is not None or fp64_forward:
+            raise ValueError("target= replaces s_target and the FP64 forward")
+        _ensure_target(solver, target)
+        loss = s_norm2(residual_traces(solver, e, s, every), freqs, solver.dt)
+    else:
+        if s_target is None:
+            raise ValueError("give s_target or target")
+        tr = traces(solver, e, s, every, fp64_forward)
+        loss = s_misfit(tr, freqs, solver.dt, torch.as_tensor(s_target))
     loss.backward()
     return float(loss.detach()), e.grad.numpy(), s.grad.numpy()

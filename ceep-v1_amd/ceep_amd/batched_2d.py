@@ -212,6 +212,8 @@ class BatchedFDTD2D:
         self.cpml_c_x = self.cpml_b_x - 1.0
         self.cpml_b_y, self.cpml_c_y = self.cpml_b_x, self.cpml_c_x
         self.probe_data = np.zeros((self.batch, len(self.probe_positions), self.total_steps))
+        self._target_maps = None          # reference medium of the difference form
+        self._residual_run = False        # device traces hold trace - reference trace
         self._built = True
 
     def update_materials(self, eps_r, sigma_e) -> None:
@@ -234,6 +236,7 @@ class BatchedFDTD2D:
         if not self._built:
             self._build()
         self.probe_data = self._handle.run()
+        self._residual_run = False
         return self._as_dict(self.probe_data)
 
     def run_cpu(self) -> Dict[int, Dict[int, np.ndarray]]:
@@ -260,7 +263,13 @@ class BatchedFDTD2D:
         if self._handle is None or self.probe_data is None:
             raise RuntimeError("run() first")
         f = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
-        S = self._handle.dft(f)
+        if getattr(self, "_residual_run", False):
+            # after a difference-form run the device holds the residual
+            # traces; probe_data (reference + residual) is transformed here
+            ph = np.exp(-2j * np.pi * f[:, None] * (np.arange(self.total_steps) * self.dt)[None, :])
+            S = np.einsum("ft,bpt->fpb", ph, self.probe_data) * self.dt
+        else:
+            S = self._handle.dft(f)
         return MultistaticSParameters(f, S, [tuple(p) for p in self.probe_positions])
 
     # the (1, nx, ny) coefficient maps of _build (:162-165), formed on access

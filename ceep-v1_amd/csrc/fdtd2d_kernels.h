@@ -195,13 +195,27 @@ struct TB2DArgs {
   float* e_save;                  // if set: E after local step s -> e_save + s * slot ([B][plane])
   long long slot;                 // floats per e_save slot
   int xcd_order;                  // contiguous tile ranges per XCD (xcd_block)
+  // difference form (poles == kPolesDiff): the state is E - E_ref of a
+  // reference run with coefficients ca_r, cb_r; its E^s sits at
+  // e_ref + s * ref_slot ([B][plane]).  The E step adds
+  // d_a E_ref^s + d_b (E_ref^{s+1} - source^s) with d_a = ca - cb ca_r / cb_r,
+  // d_b = cb / cb_r - 1 (zero where the two media agree), only in the tiles
+  // diff_tile flags; no source is injected (it cancels).
+  const float* d_a;
+  const float* d_b;
+  const float* e_ref;
+  long long ref_slot;
+  const unsigned char* diff_tile; // [tiles_x * tiles_y]
 };
+
+// The difference form's `poles` value (non-dispersive media only).
+constexpr int kPolesDiff = 4;
 
 // poles: 0 or 1 (Debye); R in {16, 24, 32}; K in {1, 2, 3, 4}.
 hipError_t launch_step2d_tb(const TB2DArgs& a, int poles, int R, int K, hipStream_t s);
 // The same update with an R x 64 window split over the four wavefronts of a
 // workgroup (R / 4 rows each, halo rows through LDS); one workgroup per task.
-// R in {32, 48, 64}, K in {4, 6, 8}, poles 0 or 1.
+// R in {32, 48, 64}, K in {4, 6, 8}, poles 0, 1 or kPolesDiff.
 hipError_t launch_step2d_mw(const TB2DArgs& a, int poles, int R, int K, hipStream_t s);
 
 // Temporally blocked adjoint: K adjoint steps (step_hi-1 down to step_hi-ksteps)
@@ -252,7 +266,8 @@ hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s
 // The same reversed steps with an R x 64 window split over the four
 // wavefronts of a workgroup (R / 4 rows each; halo rows through LDS); one
 // workgroup per task, R in {16, 24, 32}.
-hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
+// prefetch: the next TX's planes go to LDS while the current TX is stepped.
+hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, bool prefetch, hipStream_t s);
 
 // One problem of an uneven batch: its blocked-kernel arguments (buffers set
 // per round from the ping-pong pairs) and the number of K-step rounds.

@@ -20,6 +20,7 @@
 #include "fdtd3d_kernels.h"
 #include "reduce_kernels.h"
 #include "imaging.h"
+#include "coef2d.h"
 
 namespace {
 
@@ -152,6 +153,28 @@ struct ceep_fdtd {
   bool maps64_dirty = true;
   bool traces64_valid = false; // the last run was the FP64 forward
 
+  // 2D non-dispersive path: the FP64 material maps (full grid) on the device;
+  // the coefficient maps and the gradient chain rule are formed from them there
+  double* eps64 = nullptr;
+  double* sig64 = nullptr;
+  double* grad64 = nullptr;    // [2][full_cells] dL/d eps_r, dL/d sigma
+
+  // difference (residual) form of the 2D gradient path: a reference medium
+  // (the misfit's target), its run's E at every step and its state at every
+  // checkpoint, and the maps of the source term E - E_ref is stepped with
+  bool has_ref = false;
+  bool ref_dirty = true;       // the reference run is missing or stale
+  int ref_every = 0;
+  double* ref_eps64 = nullptr;
+  double* ref_sig64 = nullptr;
+  float *ca_ref = nullptr, *cb_ref = nullptr, *d_a = nullptr, *d_b = nullptr;
+  unsigned char* diff_tile = nullptr;
+  int diff_tile_n = 0;
+  float* e_ref = nullptr;      // (T + 1) slots of [B][plane]: E_ref^s
+  std::vector<float*> ref_ck;  // (nck + 1) x 3: reference state every ref_every steps
+  std::vector<float> ref_traces;   // [B][P][T]
+  bool diff_active = false;    // launch_range steps the difference form
+
   bool mats_dirty = true;
   bool has_run = false;
   int bb = 4;
@@ -180,6 +203,7 @@ struct ceep_fdtd {
   int adj3d_k = 2;             // 3D adjoint: reversed steps per blocked launch (0: the A/B/C kernels)
   int adj3d_order = 1;         // 3D blocked adjoint: workgroup order (0 dispatch, 1 per XCD, 2 TX fastest)
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
+  int adj_prefetch = 0;        // 2D four-wavefront reversible adjoint: next TX's planes into LDS
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
 
@@ -251,6 +275,17 @@ struct ceep_fdtd {
     pin_dtr = pin_g = nullptr;
     pin_dtr_n = pin_g_n = 0;
   }
+  void free_reference() {
+    for (auto& p : ref_ck) if (p) (void)hipFree(p);
+    ref_ck.clear();
+    for (float** p : {&e_ref, &ca_ref, &cb_ref, &d_a, &d_b}) { if (*p) (void)hipFree(*p); *p = nullptr; }
+    if (diff_tile) (void)hipFree(diff_tile);
+    diff_tile = nullptr;
+    diff_tile_n = 0;
+    ref_traces.clear();
+    ref_dirty = true;
+    ref_every = 0;
+  }
   void drop_graph() {
     for (auto& g : gexec) {
       if (g) (void)hipGraphExecDestroy(g);
@@ -268,17 +303,45 @@ int upload_f32(float* dst, const std::vector<float>& v, hipStream_t s) {
 
 // FP64 coefficient maps (fdtd_2d_batched.py:152-165, fdtd_3d_batched.py:282-296,
 // dispersive.py:253-299, core/grid.py:178-184) over the box, stored as FP32.
+// 2D without poles: ca / cb (and, with a reference medium, its maps and the
+// difference-form maps) formed on the device from the FP64 maps
+int commit_materials_2d_device(ceep_fdtd* h) {
+  hipStream_t s = h->stream;
+  int rc;
+  const size_t nf = (size_t)h->full_cells;
+  if (!h->eps64 && (rc = dalloc(&h->eps64, nf))) return rc;
+  if (!h->sig64 && (rc = dalloc(&h->sig64, nf))) return rc;
+  CEEP_HIP(hipMemcpyAsync(h->eps64, h->eps_r.data(), nf * sizeof(double), hipMemcpyHostToDevice, s));
+  CEEP_HIP(hipMemcpyAsync(h->sig64, h->sigma_e.data(), nf * sizeof(double), hipMemcpyHostToDevice, s));
+  ceep::Coef2DArgs a{};
+  a.eps_r = h->eps64; a.sigma = h->sig64;
+  a.ny_full = h->N[1]; a.org = h->org;
+  a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch;
+  a.cpml_interior = h->cpml() ? 1 : 0;
+  a.dt = h->dt;
+  a.ca = h->ca; a.cb = h->cb;
+  if (h->has_ref) {
+    for (float** p : {&h->ca_ref, &h->cb_ref, &h->d_a, &h->d_b})
+      if (!*p && (rc = dalloc(p, (size_t)h->plane))) return rc;
+    a.eps_ref = h->ref_eps64; a.sigma_ref = h->ref_sig64;
+    a.ca_ref = h->ca_ref; a.cb_ref = h->cb_ref; a.d_a = h->d_a; a.d_b = h->d_b;
+  }
+  CEEP_HIP(ceep::launch_coef2d(a, s));
+  return CEEP_OK;
+}
+
 int commit_materials(ceep_fdtd* h) {
   if (!h->mats_dirty) return CEEP_OK;
   const long long nbox = h->plane;
-  std::vector<float> ca(nbox, 0.f), cb(nbox, 0.f);
+  const bool dev2d = h->ndim == 2 && !h->has_debye && !h->has_lorentz;
+  std::vector<float> ca(dev2d ? 0 : nbox, 0.f), cb(dev2d ? 0 : nbox, 0.f);
   std::vector<float> da, dbi, dk1, dk2, lk1, lk2, lk3;
   if (h->ndim == 3 && h->magnetic) { da.assign(nbox, 1.f); dbi.assign(nbox, 0.f); }
   if (h->has_debye) { dk1.assign(nbox, 0.f); dk2.assign(nbox, 0.f); }
   if (h->has_lorentz) { lk1.assign(nbox, 0.f); lk2.assign(nbox, 0.f); lk3.assign(nbox, 0.f); }
   const double dt = h->dt, inv = 1.0 / h->dx;
   const int mz = h->ndim == 3 ? h->M[2] : 1;
-  for (int a = 0; a < h->M[0]; ++a)
+  for (int a = 0; a < (dev2d ? 0 : h->M[0]); ++a)
     for (int b = 0; b < h->M[1]; ++b)
       for (int c = 0; c < mz; ++c) {
         const long long g = h->full_index(a + h->org, b + h->org, h->ndim == 3 ? c + h->org : 0);
@@ -334,7 +397,11 @@ int commit_materials(ceep_fdtd* h) {
       }
   hipStream_t s = h->stream;
   int rc;
-  if ((rc = upload_f32(h->ca, ca, s)) || (rc = upload_f32(h->cb, cb, s))) return rc;
+  if (dev2d) {
+    if ((rc = commit_materials_2d_device(h))) return rc;
+  } else if ((rc = upload_f32(h->ca, ca, s)) || (rc = upload_f32(h->cb, cb, s))) {
+    return rc;
+  }
   dfree(h->da); dfree(h->dbi); dfree(h->dk1); dfree(h->dk2);
   dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
   if (!da.empty()) {
@@ -485,6 +552,16 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     a.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
     a.bpw = h->tb_bpw;
     a.ntasks = a.tiles_x * a.tiles_y * ((h->B + a.bpw - 1) / a.bpw);
+    int kpoles = h->poles();
+    if (h->diff_active) {   // E - E_ref with the reference run's source term
+      if (h->fwd_waves != 4 || kpoles != 0 || !h->e_ref || !h->diff_tile ||
+          h->diff_tile_n != a.tiles_x * a.tiles_y)
+        return fail(CEEP_ESTATE, "difference form: needs the four-wavefront blocked forward, "
+                                 "non-dispersive media and a reference run for this launch shape");
+      kpoles = ceep::kPolesDiff;
+      a.d_a = h->d_a; a.d_b = h->d_b; a.e_ref = h->e_ref; a.ref_slot = h->box_floats();
+      a.diff_tile = h->diff_tile;
+    }
     int cur = start;
     for (int st = step0; st < step_end; st += K) {
       const int o = cur ^ 1;
@@ -496,9 +573,9 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       a.slot = h->box_floats();
       a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
       if (h->fwd_waves == 4)
-        CEEP_HIP(ceep::launch_step2d_mw(a, h->poles(), R, K, s));
+        CEEP_HIP(ceep::launch_step2d_mw(a, kpoles, R, K, s));
       else
-        CEEP_HIP(ceep::launch_step2d_tb(a, h->poles(), R, K, s));
+        CEEP_HIP(ceep::launch_step2d_tb(a, kpoles, R, K, s));
       cur = o;
     }
   } else if (h->cpml() && h->ndim == 2) {
@@ -915,6 +992,9 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   for (int i = 0; i < 2; ++i)
     for (int k = 0; k < 3; ++k) dfree(h->f64[i][k]);
   dfree(h->ca64); dfree(h->cb64); dfree(h->wave64); dfree(h->traces64);
+  h->free_reference();
+  dfree(h->eps64); dfree(h->sig64); dfree(h->grad64);
+  dfree(h->ref_eps64); dfree(h->ref_sig64);
   for (int k = 0; k < 12; ++k) dfree(h->psi[k]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1546,6 +1626,60 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
 
 }  // namespace
 
+namespace {
+
+// The difference form's reference run: the reference medium's forward over
+// all T steps with this handle's sources, probes and launch shape, keeping
+// E after every step (slot s of e_ref holds E_ref^s; slot 0 = 0), the state
+// at every multiple of `every` and the traces.
+int run_reference(ceep_fdtd* h, int every) {
+  int rc;
+  hipStream_t st = h->stream;
+  const size_t nbox = (size_t)h->box_floats();
+  const int nck = (h->T + every - 1) / every;
+  if (!h->e_ref) {
+    const size_t n = (size_t)(h->T + 1) * nbox;
+    if (dalloc(&h->e_ref, n))
+      return fail(CEEP_ENOMEM, "difference form: the reference run's E store (" +
+                                   std::to_string(n * sizeof(float)) + " B) does not fit");
+  }
+  if (h->ref_ck.size() != (size_t)(nck + 1) * 3) {
+    for (auto& p : h->ref_ck) dfree(p);
+    h->ref_ck.assign((size_t)(nck + 1) * 3, nullptr);
+    for (size_t i = 3; i < h->ref_ck.size(); ++i)   // the state at step 0 is zero: not kept
+      if ((rc = dalloc(&h->ref_ck[i], nbox))) return rc;
+  }
+  if ((rc = zero_state(h))) return rc;
+  CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), st));
+  CEEP_HIP(hipMemsetAsync(h->e_ref, 0, nbox * sizeof(float), st));
+  std::swap(h->ca, h->ca_ref);
+  std::swap(h->cb, h->cb_ref);
+  int cur = 0;
+  for (int k = 0; k < nck && !rc; ++k) {
+    if (k > 0)
+      for (int f = 0; f < 3 && !rc; ++f)
+        if (ceep::launch_copy(h->f[cur][f], h->ref_ck[(size_t)k * 3 + f], (long long)nbox, st) != hipSuccess)
+          rc = fail(CEEP_EHIP, "reference run: checkpoint copy failed");
+    const int s0 = k * every, n = std::min(every, h->T - s0);
+    if (!rc) rc = launch_range(h, cur, s0, n, h->e_ref + (size_t)s0 * nbox);
+    cur ^= ((n + h->tb_k - 1) / h->tb_k) & 1;
+  }
+  std::swap(h->ca, h->ca_ref);
+  std::swap(h->cb, h->cb_ref);
+  if (rc) return rc;
+  for (int f = 0; f < 3; ++f)
+    CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ref_ck[(size_t)nck * 3 + f], (long long)nbox, st));
+  h->ref_traces.resize((size_t)h->B * h->P * h->T);
+  CEEP_HIP(hipMemcpyAsync(h->ref_traces.data(), h->traces, h->ref_traces.size() * sizeof(float),
+                          hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipStreamSynchronize(st));
+  h->ref_every = every;
+  h->ref_dirty = false;
+  return CEEP_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
@@ -1607,6 +1741,109 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out) {
   h->ck_rev = rev;
   h->estore_seg = rev ? -1 : nck - 1;
   if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
+  return CEEP_OK;
+}
+
+int ceep_fdtd_set_reference(ceep_fdtd* h, const double* eps_r, const double* sigma_e) {
+  if (!h) return fail(CEEP_EINVAL, "null handle");
+  if (!eps_r && !sigma_e) {
+    h->free_reference();
+    dfree(h->ref_eps64);
+    dfree(h->ref_sig64);
+    h->has_ref = false;
+    h->mats_dirty = true;
+    return CEEP_OK;
+  }
+  if (!eps_r || !sigma_e) return fail(CEEP_EINVAL, "reference medium: eps_r and sigma_e are both needed");
+  if (h->ndim != 2 || h->cpml())
+    return fail(CEEP_EUNSUPPORTED, "the difference form covers the 2D cavity modes");
+  for (long long i = 0; i < h->full_cells; ++i)
+    if (!(eps_r[i] > 0) || !std::isfinite(sigma_e[i]))
+      return fail(CEEP_EINVAL, "reference eps_r must be > 0 and sigma_e finite everywhere");
+  int rc;
+  const size_t nf = (size_t)h->full_cells;
+  if (!h->ref_eps64 && (rc = dalloc(&h->ref_eps64, nf))) return rc;
+  if (!h->ref_sig64 && (rc = dalloc(&h->ref_sig64, nf))) return rc;
+  CEEP_HIP(hipMemcpyAsync(h->ref_eps64, eps_r, nf * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  CEEP_HIP(hipMemcpyAsync(h->ref_sig64, sigma_e, nf * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  h->has_ref = true;
+  h->ref_dirty = true;
+  h->mats_dirty = true;
+  return CEEP_OK;
+}
+
+int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double* reference_out) {
+  int rc = ready_check(h);
+  if (rc) return rc;
+  if (!h->has_ref) return fail(CEEP_ESTATE, "no reference medium (ceep_fdtd_set_reference)");
+  if (h->ndim != 2 || h->cpml() || h->poles())
+    return fail(CEEP_EUNSUPPORTED, "the difference form covers 2D non-dispersive cavity runs");
+  if (!h->use_rev() || h->fwd_waves != 4 || h->fwd64)
+    return fail(CEEP_EUNSUPPORTED, "the difference form needs the four-wavefront blocked FP32 "
+                                   "forward and the reversible adjoint");
+  if (every < 1) return fail(CEEP_EINVAL, "checkpoint interval must be >= 1");
+  hipStream_t st = h->stream;
+  const size_t nbox = (size_t)h->box_floats();
+  if (h->ref_dirty || h->ref_every != every) {
+    h->ck_valid = false;
+    if ((rc = run_reference(h, every))) return rc;
+  }
+  // tiles whose window meets cells where the media differ
+  const int R = h->tb_r, K = h->tb_k;
+  const int tx = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
+  const int ty = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
+  if (h->diff_tile_n != tx * ty) {
+    dfree(h->diff_tile);
+    h->diff_tile_n = 0;
+    if ((rc = dalloc(&h->diff_tile, (size_t)tx * ty))) return rc;
+    h->diff_tile_n = tx * ty;
+  }
+  CEEP_HIP(ceep::launch_diff_tiles(h->d_a, h->d_b, h->M[0], h->M[1], h->pitch, R, K, tx, ty,
+                                   h->diff_tile, st));
+  const int nck = (h->T + every - 1) / every;
+  h->ck_valid = false;
+  if (h->ck.size() != (size_t)(nck + 1) * 3) {
+    for (auto& p : h->ck) dfree(p);
+    h->ck.assign((size_t)(nck + 1) * 3, nullptr);
+    for (auto& p : h->ck)
+      if ((rc = dalloc(&p, nbox))) { h->free_adjoint(); return rc; }
+  }
+  h->ck_count = nck;
+  if (every != h->ck_every) dfree(h->estore);
+  h->ck_every = every;
+  h->estore_seg = -1;
+  if ((rc = zero_state(h))) return rc;
+  CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), st));
+  h->traces64_valid = false;
+  int cur = 0;
+  h->diff_active = true;
+  for (int k = 0; k < nck && !rc; ++k) {
+    if (k > 0)
+      for (int f = 0; f < 3 && !rc; ++f)
+        if (ceep::launch_copy(h->f[cur][f], h->ck[(size_t)k * 3 + f], (long long)nbox, st) != hipSuccess)
+          rc = fail(CEEP_EHIP, "difference run: checkpoint copy failed");
+    const int s0 = k * every, n = std::min(every, h->T - s0);
+    if (!rc) rc = launch_range(h, cur, s0, n, nullptr);
+    cur ^= ((n + h->tb_k - 1) / h->tb_k) & 1;
+  }
+  h->diff_active = false;
+  if (rc) return rc;
+  for (int f = 0; f < 3; ++f)
+    CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ck[(size_t)nck * 3 + f], (long long)nbox, st));
+  // the adjoint reverses the medium's own total field: checkpoint = E_ref + (E - E_ref)
+  for (int k = 1; k <= nck; ++k)
+    for (int f = 0; f < 3; ++f)
+      CEEP_HIP(ceep::launch_add(h->ck[(size_t)k * 3 + f], h->ref_ck[(size_t)k * 3 + f],
+                                h->ck[(size_t)k * 3 + f], (long long)nbox, st));
+  h->parity = cur;
+  CEEP_HIP(hipStreamSynchronize(st));
+  h->has_run = true;
+  h->ck_valid = true;
+  h->ck_rev = true;
+  if (reference_out)
+    for (size_t i = 0; i < h->ref_traces.size(); ++i) reference_out[i] = h->ref_traces[i];
+  if (residual_out) return ceep_fdtd_get_traces(h, residual_out);
   return CEEP_OK;
 }
 
@@ -1693,7 +1930,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       t.step_hi = hi;
       t.ksteps = std::min(K, hi - s0);
       if (h->adj_waves == 4)
-        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, st));
+        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, h->adj_prefetch != 0, st));
       else
         CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, st));
       acur = o;
@@ -1759,41 +1996,27 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     CEEP_HIP(ceep::launch_sum_parts(h->g_ca, nchunks, h->plane, h->plane, h->g_ca, st));
     CEEP_HIP(ceep::launch_sum_parts(h->g_cbi, nchunks, h->plane, h->plane, h->g_cbi, st));
   }
-  if ((rc = pinned(&h->pin_g, &h->pin_g_n, (size_t)2 * h->plane))) return rc;
-  const float* gca = h->pin_g;
-  const float* gcb = h->pin_g + h->plane;
-  CEEP_HIP(hipMemcpyAsync(h->pin_g, h->g_ca, (size_t)h->plane * sizeof(float), hipMemcpyDeviceToHost, st));
-  CEEP_HIP(hipMemcpyAsync(h->pin_g + h->plane, h->g_cbi, (size_t)h->plane * sizeof(float),
-                          hipMemcpyDeviceToHost, st));
+  // chain rule to eps_r and sigma on the device, in FP64 (ca, cb of
+  // fdtd_2d_batched.py:153-157; g_cbi sums lam * (cb / dx) * curl = cb * dL/dcb)
+  const size_t nf = (size_t)h->full_cells;
+  if (!h->grad64 && (rc = dalloc(&h->grad64, 2 * nf))) return rc;
+  ceep::Chain2DArgs c{};
+  c.eps_r = h->eps64; c.sigma = h->sig64;
+  c.g_ca = h->g_ca; c.g_cbi = h->g_cbi;
+  c.inv_scale = 1.0 / scale;   // scale is a power of two: exact
+  c.dt = h->dt;
+  c.nx = (int)h->N[0]; c.ny = (int)h->N[1]; c.org = h->org;
+  c.Mx = h->M[0]; c.My = h->M[1]; c.pitch = h->pitch;
+  c.grad_eps_r = h->grad64; c.grad_sigma = h->grad64 + nf;
+  CEEP_HIP(ceep::launch_chain2d(c, st));
   // the recomputation overwrote the fields: leave a defined (zero) state
   if ((rc = zero_state(h))) return rc;
+  CEEP_HIP(hipMemcpyAsync(grad_eps_r, h->grad64, nf * sizeof(double), hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipMemcpyAsync(grad_sigma, h->grad64 + nf, nf * sizeof(double), hipMemcpyDeviceToHost, st));
   CEEP_HIP(hipStreamSynchronize(st));
-  // chain rule to eps_r and sigma in FP64 (ca, cb of fdtd_2d_batched.py:153-157):
-  //   s = sigma dt / (2 eps),  ca = (1 - s) / (1 + s),  cb = (dt / eps) / (1 + s)
-  //   dca/ds = -2 / (1 + s)^2,  ds/deps = -s / eps,  ds/dsigma = dt / (2 eps)
-  //   dcb/deps = -cb / eps - cb / (1 + s) ds/deps,  dcb/dsigma = -cb / (1 + s) ds/dsigma
-  // g_cbi sums lam * (cb / dx) * curl = cb * dL/dcb, hence the division by cb.
-  const double dt = h->dt, inv_scale = 1.0 / scale;   // scale is a power of two: exact
-  for (int x = 1; x < h->M[0] - 1; ++x)
-    for (int y = 1; y < h->M[1] - 1; ++y) {
-      const long long o = h->box_index(x, y, 0);
-      const long long g = h->full_index(x + h->org, y + h->org, 0);
-      const double eps = h->eps_r[g] * kEps0, sg = h->sigma_e[g];
-      const double inv_eps = 1.0 / eps;
-      const double sv = 0.5 * sg * dt * inv_eps;
-      const double inv_1s = 1.0 / (1.0 + sv);
-      const double dL_dca = gca[o] * inv_scale;
-      const double dL_dcb_cb = gcb[o] * inv_scale;        // cb * dL/dcb
-      const double dca_ds = -2.0 * inv_1s * inv_1s;
-      const double ds_deps = -sv * inv_eps, ds_dsig = 0.5 * dt * inv_eps;
-      // dL/dcb * dcb/dq = (cb dL/dcb) * (dcb/dq / cb)
-      const double rdcb_deps = -inv_eps - inv_1s * ds_deps;
-      const double rdcb_dsig = -inv_1s * ds_dsig;
-      grad_eps_r[g] = kEps0 * (dL_dca * dca_ds * ds_deps + dL_dcb_cb * rdcb_deps);
-      grad_sigma[g] = dL_dca * dca_ds * ds_dsig + dL_dcb_cb * rdcb_dsig;
-    }
   return CEEP_OK;
 }
+
 
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   if (!h) return fail(CEEP_EINVAL, "null handle");
@@ -1834,6 +2057,9 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 0 || value > 2)
       return fail(CEEP_EINVAL, "3D adjoint steps per launch must be 0 (three-kernel form), 1 or 2");
     h->adj3d_k = value;
+  } else if (key == 21) {
+    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "adjoint prefetch must be 0 or 1");
+    h->adj_prefetch = value;
   } else if (key == 20) {
     if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D adjoint block order must be 0, 1 or 2");
     h->adj3d_order = value;

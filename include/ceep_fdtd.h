@@ -145,6 +145,28 @@ int ceep_fdtd_run_checkpointed(ceep_fdtd* h, int every, double* traces_out);
 int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
                       double* grad_sigma);
 
+/* Reference medium of the difference form (2D, cavity modes): eps_r and
+ * sigma_e (full grid, float64) of the medium whose traces the misfit is
+ * measured against -- e.g. the target phantom of L = sum |S - S_target|^2.
+ * Both NULL clears it.  The next ceep_fdtd_run_residual steps that medium once
+ * with this handle's sources, probes and launch shape, keeping its E after
+ * every step ((total_steps + 1) x batch x box plane floats of device memory)
+ * and its state at every checkpoint.  No counterpart in the reference (it has
+ * no gradient path); the target is what its S-matrix workflow compares
+ * against (src/ceep/solvers/s_params.py:186-231). */
+int ceep_fdtd_set_reference(ceep_fdtd* h, const double* eps_r, const double* sigma_e);
+
+/* Checkpointed forward of the difference E - E_ref (FP32, 2D non-dispersive,
+ * four-wavefront blocked forward, reversible adjoint): the state is the
+ * difference itself, stepped with this medium's coefficients plus the source
+ * term d_a E_ref^s + d_b (E_ref^{s+1} - source^s) where the media differ, so
+ * FP32 rounds the residual rather than two nearly equal totals.  residual_out
+ * (batch, n_probes, total_steps) gets trace(eps) - trace(eps_ref) (or NULL);
+ * reference_out the reference run's traces (or NULL).  The checkpoints it
+ * leaves hold the medium's total state, so ceep_fdtd_adjoint runs after it as
+ * after ceep_fdtd_run_checkpointed.  No counterpart in the reference. */
+int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double* reference_out);
+
 /* Tuning knobs (keys 0-4 give the same results to FP32 rounding for every setting):
  *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8; the
  *          3D one-step kernels use at most 4, the 3D adjoint kernels all four)

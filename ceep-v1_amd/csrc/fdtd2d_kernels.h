@@ -266,8 +266,7 @@ hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s
 // The same reversed steps with an R x 64 window split over the four
 // wavefronts of a workgroup (R / 4 rows each; halo rows through LDS); one
 // workgroup per task, R in {16, 24, 32}.
-// prefetch: the next TX's planes go to LDS while the current TX is stepped.
-hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, bool prefetch, hipStream_t s);
+hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
 
 // One problem of an uneven batch: its blocked-kernel arguments (buffers set
 // per round from the ping-pong pairs) and the number of K-step rounds.

@@ -876,20 +876,11 @@ hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s
 // or more wavefronts share a SIMD and makes R = 32 windows affordable, whose
 // (R - 2K) / R output fraction is 75 % at K = 4.
 //
-// With PF, a window's next TX plane set (the six planes, this wavefront's
-// rows) is fetched into LDS by buffer-to-LDS loads issued before the current
-// TX's reversed steps, so its HBM latency hides behind them (48 KB of LDS per
-// workgroup, no registers).  The loads of the current TX's dL/dtrace samples
-// are issued before that prefetch, the prefetch is waited for before the
-// current TX's stores, and the LDS copies are read before the next prefetch
-// is issued; rows off the box are zeroed on the way into registers.
-template <int RW, int K, bool PF>
+template <int RW, int K>
 __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
   constexpr int R = 4 * RW;
   constexpr int NO = R - 2 * K;
   __shared__ float x_fy[4][64], x_es[4][64], x_q[4][64], x_my[4][64];
-  __shared__ float pf_buf[PF ? 4 : 1][6][PF ? RW : 1][64];
-  __shared__ float pf_gq[PF ? 4 : 1][PF ? K : 1][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int task = xcd_block(p.xcd_order);
@@ -949,19 +940,7 @@ __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
   float le[RW], mx[RW], my[RW], fe[RW], fx[RW], fy[RW];
   for (int bt = b_begin; bt < b_end; ++bt) {
     const long long base = (long long)bt * p.bstride;
-    if (PF && bt > b_begin) {
-      // the prefetch of this TX landed before the previous TX's stores
-#pragma unroll
-      for (int i = 0; i < RW; ++i) {
-        const bool ok = row_off(i) < kOffNone;
-        le[i] = ok ? pf_buf[w][0][i][lane] : 0.f;
-        mx[i] = ok ? pf_buf[w][1][i][lane] : 0.f;
-        my[i] = ok ? pf_buf[w][2][i][lane] : 0.f;
-        fe[i] = ok ? pf_buf[w][3][i][lane] : 0.f;
-        fx[i] = ok ? pf_buf[w][4][i][lane] : 0.f;
-        fy[i] = ok ? pf_buf[w][5][i][lane] : 0.f;
-      }
-    } else {
+    {
       const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_in + base, plane_bytes);
       const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_in + base, plane_bytes);
       const __amdgpu_buffer_rsrc_t r_my = plane_rsrc(p.my_in + base, plane_bytes);
@@ -994,46 +973,9 @@ __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
       const int step = p.step_hi - 1 - k;
       const bool live = k < p.ksteps;
       wsrc[k] = live ? wv[step] : 0.f;
-      if (PF && bt > b_begin)
-        gq[k] = (live && step >= 1 && pq) ? pf_gq[w][k][lane] : 0.f;
-      else
-        gq[k] = (live && step >= 1 && pq)
-                    ? p.dtr[((long long)bt * p.P + lane) * p.T + (step - 1)] : 0.f;
+      gq[k] = (live && step >= 1 && pq)
+                  ? p.dtr[((long long)bt * p.P + lane) * p.T + (step - 1)] : 0.f;
     }
-    const bool prefetch = PF && bt + 1 < b_end;
-    // the LDS copies of this TX are read (the next prefetch overwrites them)
-    // and its dL/dtrace loads are done before the prefetch goes out
-    if (PF) {
-      __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-      __builtin_amdgcn_s_waitcnt(kWaitVm0);
-    }
-    asm volatile("" ::: "memory");
-    if (prefetch) {
-      const long long nb = (long long)(bt + 1) * p.bstride;
-      const float* src[6] = {p.le_in + nb, p.mx_in + nb, p.my_in + nb,
-                             p.fe_in + nb, p.fhx_in + nb, p.fhy_in + nb};
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        const __amdgpu_buffer_rsrc_t r = plane_rsrc(src[j], plane_bytes);
-#pragma unroll
-        for (int i = 0; i < RW; ++i)
-          __builtin_amdgcn_raw_ptr_buffer_load_lds(
-              r, (__attribute__((address_space(3))) void*)&pf_buf[w][j][i][0], 4,
-              (int)row_off(i), 0, 0, 0);
-      }
-      const __amdgpu_buffer_rsrc_t rq =
-          plane_rsrc(p.dtr, (unsigned)((long long)p.B * p.P * p.T * 4));
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int step = p.step_hi - 1 - k;
-        const bool ok = k < p.ksteps && step >= 1 && pq;
-        const unsigned off =
-            ok ? (unsigned)((((long long)(bt + 1) * p.P + lane) * p.T + (step - 1)) * 4) : kOffNone;
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rq, (__attribute__((address_space(3))) void*)&pf_gq[w][k][0], 4, (int)off, 0, 0, 0);
-      }
-    }
-    asm volatile("" ::: "memory");
 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1128,11 +1070,6 @@ __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
         }
       }
     }
-    if (prefetch) {   // the next TX is in LDS
-      asm volatile("" ::: "memory");
-      __builtin_amdgcn_s_waitcnt(kWaitVm0);
-      asm volatile("" ::: "memory");
-    }
     {
       const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_out + base, plane_bytes);
       const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_out + base, plane_bytes);
@@ -1164,30 +1101,30 @@ __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
   }
 }
 
-template <int RW, bool PF>
+template <int RW>
 static hipError_t launch_rev_mw_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
   const dim3 blk(256, 1, 1), grd((unsigned)a.ntasks, 1, 1);
   switch (K) {
-    case 1: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 1, PF>), grd, blk, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 2, PF>), grd, blk, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 3, PF>), grd, blk, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 4, PF>), grd, blk, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 5, PF>), grd, blk, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 1>), grd, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 2>), grd, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 3>), grd, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 4>), grd, blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 5>), grd, blk, 0, s, a); break;
     case 6:
       if (RW < 8) return hipErrorInvalidValue;   // the window keeps at least 20 exact rows
-      hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 6, PF>), grd, blk, 0, s, a);
+      hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 6>), grd, blk, 0, s, a);
       break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, bool prefetch, hipStream_t s) {
+hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
   if (a.ntasks < 1) return hipSuccess;
   switch (R) {
-    case 16: return prefetch ? launch_rev_mw_r<4, true>(a, K, s) : launch_rev_mw_r<4, false>(a, K, s);
-    case 24: return prefetch ? launch_rev_mw_r<6, true>(a, K, s) : launch_rev_mw_r<6, false>(a, K, s);
-    case 32: return prefetch ? launch_rev_mw_r<8, true>(a, K, s) : launch_rev_mw_r<8, false>(a, K, s);
+    case 16: return launch_rev_mw_r<4>(a, K, s);
+    case 24: return launch_rev_mw_r<6>(a, K, s);
+    case 32: return launch_rev_mw_r<8>(a, K, s);
   }
   return hipErrorInvalidValue;
 }
@@ -1305,12 +1242,21 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
 #pragma unroll
         for (int i = 0; i < RW; ++i) jd[i] = bload(r_jd, row_off(i));
       }
-      if (dif) {   // jd holds the reference run's E at the current step
-        const __amdgpu_buffer_rsrc_t r_er =
-            plane_rsrc(p.e_ref + (long long)p.step0 * p.ref_slot + base, plane_bytes);
+    }
+    // difference form: the reference run's E at the current and the next
+    // step (er0, er1), loaded only on cells where the media differ (the
+    // source term is 0 elsewhere); each step fetches the one after next
+    float er0[RW], er1[RW];
+    auto load_ref = [&](float* dst, int step) {
+      const __amdgpu_buffer_rsrc_t r_er =
+          plane_rsrc(p.e_ref + (long long)step * p.ref_slot + base, plane_bytes);
 #pragma unroll
-        for (int i = 0; i < RW; ++i) jd[i] = bload(r_er, row_off(i));
-      }
+      for (int i = 0; i < RW; ++i)
+        dst[i] = bload(r_er, (mk1[i] != 0.f || mk2[i] != 0.f) ? row_off(i) : kOffNone);
+    };
+    if (dif) {
+      load_ref(er0, p.step0);
+      load_ref(er1, p.step0 + 1);
     }
     const int sc = p.src_cell[bt];
     const int s_row = sc / p.pitch - (a0 + r0);        // local row
@@ -1321,13 +1267,8 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
     for (int s = 0; s < K; ++s) {
       if (s >= p.ksteps) break;                        // uniform per workgroup
       const int step = p.step0 + s;
-      float er1[RW];                                   // difference form: E_ref^{step+1}
-      if (dif) {
-        const __amdgpu_buffer_rsrc_t r_er =
-            plane_rsrc(p.e_ref + (long long)(step + 1) * p.ref_slot + base, plane_bytes);
-#pragma unroll
-        for (int i = 0; i < RW; ++i) er1[i] = bload(r_er, row_off(i));
-      }
+      float er2[RW];                                   // difference form: E_ref^{step+2}
+      if (dif && s + 1 < p.ksteps) load_ref(er2, step + 2);
       // H half step: the E row below the last row comes from the wavefront below
       x_e[w][lane] = ez[0];
       __syncthreads();
@@ -1362,22 +1303,13 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
         // the source is common to both runs: it cancels in E - E_ref, and
         // E_ref^{s+1} - source^s is the reference's ca_r E_ref^s + cb_r curl
         if (dif) {
-          if (src_here) {
-            const float wv = p.wave[(long long)bt * p.wave_stride_b + step];
-#pragma unroll
-            for (int i = 0; i < RW; ++i)
-              if (i == s_row && lane == s_lane) er1[i] -= wv;
-          }
+          const float wv = src_here ? p.wave[(long long)bt * p.wave_stride_b + step] : 0.f;
 #pragma unroll
           for (int i = 0; i < RW; ++i) {
-            ez[i] += mk1[i] * jd[i] + mk2[i] * er1[i];
-            jd[i] = er1[i];
-          }
-          if (src_here) {   // the next step reads E_ref^{s+1} itself
-            const float wv = p.wave[(long long)bt * p.wave_stride_b + step];
-#pragma unroll
-            for (int i = 0; i < RW; ++i)
-              if (i == s_row && lane == s_lane) jd[i] += wv;
+            const float e1 = (src_here && i == s_row && lane == s_lane) ? er1[i] - wv : er1[i];
+            ez[i] += mk1[i] * er0[i] + mk2[i] * e1;
+            er0[i] = er1[i];
+            er1[i] = er2[i];
           }
         }
       } else if (src_here) {

@@ -70,12 +70,6 @@ __device__ __forceinline__ float lane_prev(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 
-// s_waitcnt immediates (gfx9 encoding: vmcnt [3:0] + [15:14], expcnt [6:4],
-// lgkmcnt [11:8]): wait for every vector-memory operation, resp. for every
-// LDS / scalar-memory operation, leaving the other counters free
-constexpr int kWaitVm0 = 0x0F70;
-constexpr int kWaitLgkm0 = 0xC07F;
-
 // Workgroup barrier for LDS exchanges: waits for this wavefront's LDS
 // operations only, not for its global loads in flight (prefetches)
 __device__ __forceinline__ void lds_barrier() {

@@ -203,7 +203,6 @@ struct ceep_fdtd {
   int adj3d_k = 2;             // 3D adjoint: reversed steps per blocked launch (0: the A/B/C kernels)
   int adj3d_order = 1;         // 3D blocked adjoint: workgroup order (0 dispatch, 1 per XCD, 2 TX fastest)
   int adj_rev = 1;             // 2D FP32 blocked adjoint rebuilds the forward state backward
-  int adj_prefetch = 0;        // 2D four-wavefront reversible adjoint: next TX's planes into LDS
   bool ck_rev = false;         // the checkpoints were taken for the reversible adjoint
   float ch = 0.f, idx = 0.f;
 
@@ -1818,11 +1817,14 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
   h->traces64_valid = false;
   int cur = 0;
   h->diff_active = true;
+  // the adjoint reverses the medium's own total field: each checkpoint is
+  // written as E_ref + (E - E_ref), the reference state plus the difference
   for (int k = 0; k < nck && !rc; ++k) {
     if (k > 0)
       for (int f = 0; f < 3 && !rc; ++f)
-        if (ceep::launch_copy(h->f[cur][f], h->ck[(size_t)k * 3 + f], (long long)nbox, st) != hipSuccess)
-          rc = fail(CEEP_EHIP, "difference run: checkpoint copy failed");
+        if (ceep::launch_add(h->f[cur][f], h->ref_ck[(size_t)k * 3 + f], h->ck[(size_t)k * 3 + f],
+                             (long long)nbox, st) != hipSuccess)
+          rc = fail(CEEP_EHIP, "difference run: checkpoint failed");
     const int s0 = k * every, n = std::min(every, h->T - s0);
     if (!rc) rc = launch_range(h, cur, s0, n, nullptr);
     cur ^= ((n + h->tb_k - 1) / h->tb_k) & 1;
@@ -1830,12 +1832,8 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
   h->diff_active = false;
   if (rc) return rc;
   for (int f = 0; f < 3; ++f)
-    CEEP_HIP(ceep::launch_copy(h->f[cur][f], h->ck[(size_t)nck * 3 + f], (long long)nbox, st));
-  // the adjoint reverses the medium's own total field: checkpoint = E_ref + (E - E_ref)
-  for (int k = 1; k <= nck; ++k)
-    for (int f = 0; f < 3; ++f)
-      CEEP_HIP(ceep::launch_add(h->ck[(size_t)k * 3 + f], h->ref_ck[(size_t)k * 3 + f],
-                                h->ck[(size_t)k * 3 + f], (long long)nbox, st));
+    CEEP_HIP(ceep::launch_add(h->f[cur][f], h->ref_ck[(size_t)nck * 3 + f],
+                              h->ck[(size_t)nck * 3 + f], (long long)nbox, st));
   h->parity = cur;
   CEEP_HIP(hipStreamSynchronize(st));
   h->has_run = true;
@@ -1930,7 +1928,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       t.step_hi = hi;
       t.ksteps = std::min(K, hi - s0);
       if (h->adj_waves == 4)
-        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, h->adj_prefetch != 0, st));
+        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, st));
       else
         CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, st));
       acur = o;
@@ -2057,9 +2055,6 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 0 || value > 2)
       return fail(CEEP_EINVAL, "3D adjoint steps per launch must be 0 (three-kernel form), 1 or 2");
     h->adj3d_k = value;
-  } else if (key == 21) {
-    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "adjoint prefetch must be 0 or 1");
-    h->adj_prefetch = value;
   } else if (key == 20) {
     if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D adjoint block order must be 0, 1 or 2");
     h->adj3d_order = value;

@@ -52,6 +52,7 @@ SIGNATURES = {
     "ceep_fdtd_run": (_I, [_P, _PD]),
     "ceep_fdtd_get_traces": (_I, [_P, _PD]),
     "ceep_fdtd_dft": (_I, [_P, _PD, _I, _PD]),
+    "ceep_fdtd_dft_noalloc": (_I, [_P, _PD, _I, _PD]),
     "ceep_fdtd_get_field": (_I, [_P, _I, _PD]),
     "ceep_fdtd_energy": (_I, [_P, _PD]),
     "ceep_fdtd_time_runs": (_I, [_P, _I, _PD]),
@@ -283,6 +284,14 @@ class Handle:
         f = np.ascontiguousarray(np.atleast_1d(np.asarray(freqs, dtype=np.float64)))
         out = np.zeros((f.size, self.n_probes, self.batch, 2), dtype=np.float64)
         check(self._lib.ceep_fdtd_dft(self._h, dptr(f), int(f.size), dptr(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def dft_noalloc(self, freqs):
+        """``dft`` through the entry point that allocates nothing (static
+        device scratch of 16 MiB per GPU; overlapping calls on one GPU race)."""
+        f = np.ascontiguousarray(np.atleast_1d(np.asarray(freqs, dtype=np.float64)))
+        out = np.zeros((f.size, self.n_probes, self.batch, 2), dtype=np.float64)
+        check(self._lib.ceep_fdtd_dft_noalloc(self._h, dptr(f), int(f.size), dptr(out)))
         return out[..., 0] + 1j * out[..., 1]
 
     def field(self, comp):

@@ -21,4 +21,11 @@ hipError_t launch_to_f32(const double* src, float* dst, long long n, hipStream_t
 hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks,
                         hipStream_t s);
 
+// tw[s][f] = (cos(2 pi freqs[f] s dt), -sin(...)) for s < T, f < F (freqs on the device).
+hipError_t launch_twiddles(const double* freqs, int F, int T, double dt, float2* tw,
+                           hipStream_t s);
+
+// out = interleaved (re, im) float64 of S[0:n)
+hipError_t launch_c2z(const float2* S, double* out, long long n, hipStream_t s);
+
 }  // namespace ceep

@@ -122,4 +122,42 @@ hipError_t launch_sumsq(const float* x, long long n, float* partial, int nblocks
   return hipGetLastError();
 }
 
+// Twiddle table [T][F] = (cos(w_f s dt), -sin(w_f s dt)), w_f = 2 pi f, formed
+// in FP64 and rounded once (the host's table for ceep_fdtd_dft uses the same
+// expressions).
+__global__ __launch_bounds__(256) void twiddle_kernel(const double* __restrict__ freqs, int F,
+                                                      int T, double dt, float2* __restrict__ tw) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)T * F) return;
+  const int s = (int)(i / F), f = (int)(i % F);
+  const double t = s * dt;
+  const double w = 2.0 * 3.14159265358979323846 * freqs[f];
+  tw[i] = make_float2((float)cos(w * t), (float)(-sin(w * t)));
+}
+
+hipError_t launch_twiddles(const double* freqs, int F, int T, double dt, float2* tw,
+                           hipStream_t s) {
+  const long long n = (long long)T * F;
+  if (n < 1) return hipSuccess;
+  hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, freqs, F,
+                     T, dt, tw);
+  return hipGetLastError();
+}
+
+// out[2i], out[2i+1] = (double)S[i].x, (double)S[i].y
+__global__ __launch_bounds__(256) void c2z_kernel(const float2* __restrict__ S, double* __restrict__ out,
+                                                  long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = S[i];
+  out[2 * i] = (double)v.x;
+  out[2 * i + 1] = (double)v.y;
+}
+
+hipError_t launch_c2z(const float2* S, double* out, long long n, hipStream_t s) {
+  if (n < 1) return hipSuccess;
+  hipLaunchKernelGGL(c2z_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, out, n);
+  return hipGetLastError();
+}
+
 }  // namespace ceep

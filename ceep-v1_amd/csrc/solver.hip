@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -76,8 +77,42 @@ void dfree(T*& p) {
 
 }  // namespace
 
+// Scratch of the no-allocation DFT (ceep_fdtd_dft_noalloc): the frequency
+// list, the twiddle table and the S staging of one call, in one file-scope
+// device array.  Every GPU the library runs on holds its own copy for the
+// life of the process; host code reaches it only through the per-device
+// address hipGetSymbolAddress returns (scratch_address), never through the
+// array's name.  All calls on one GPU share it, and it keeps the last call's
+// contents.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, resolved once per device.
+int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(CEEP_EHIP, "no usable current HIP device for the DFT scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CEEP_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return CEEP_OK;
+}
+}  // namespace
+
 struct ceep_fdtd {
   int ndim = 2, box_mode = 0, n = 0, B = 0, P = 0, T = 0;
+  unsigned char* scratch = nullptr;   // this handle's GPU's copy of the DFT scratch
   int64_t N[3] = {1, 1, 1};
   double dx = 0, dt = 0;
   int org = 0;                 // grid index of box cell 0 (same on every axis)
@@ -933,6 +968,12 @@ int ceep_fdtd_create(int ndim, const int64_t* shape, int cpml_n, int box_mode, i
   h->eps_r.assign(h->full_cells, 1.0);
   h->sigma_e.assign(h->full_cells, 0.0);
   int rc = CEEP_OK;
+  {
+    void* sp = nullptr;
+    const int src = scratch_address(&sp);
+    if (src) { delete h; return src; }
+    h->scratch = static_cast<unsigned char*>(sp);
+  }
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete h;
@@ -1197,6 +1238,36 @@ int ceep_fdtd_dft(ceep_fdtd* h, const double* freqs, int n_freq, double* s_out) 
   CEEP_HIP(hipMemcpyAsync(S.data(), h->S, ns * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
   CEEP_HIP(hipStreamSynchronize(h->stream));
   for (size_t i = 0; i < ns; ++i) { s_out[2 * i] = S[i].x; s_out[2 * i + 1] = S[i].y; }
+  return CEEP_OK;
+}
+
+int ceep_fdtd_dft_noalloc(ceep_fdtd* h, const double* freqs, int n_freq, double* s_out) {
+  if (!h || !freqs || !s_out || n_freq < 1) return fail(CEEP_EINVAL, "bad argument");
+  if (!h->scratch) return fail(CEEP_ESTATE, "no DFT scratch on this handle's device");
+  const size_t F = (size_t)n_freq, T = (size_t)h->T, ns = F * h->P * h->B;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t off_tw = up(F * sizeof(double));
+  const size_t off_s = off_tw + up(T * F * sizeof(float2));
+  const size_t off_z = off_s + up(ns * sizeof(float2));
+  const size_t need = off_z + ns * 2 * sizeof(double);
+  if (need > kScratchBytes)
+    return fail(CEEP_EINVAL, "ceep_fdtd_dft_noalloc: " + std::to_string(n_freq) + " frequencies x " +
+                                 std::to_string(h->T) + " steps need " + std::to_string(need) +
+                                 " B of scratch; the static scratch holds " +
+                                 std::to_string(kScratchBytes) + " B (ceep_fdtd_dft has no limit)");
+  if (!h->has_run) return fail(CEEP_ESTATE, "no run yet");
+  if (h->P == 0) return CEEP_OK;
+  hipStream_t st = h->stream;
+  double* d_f = reinterpret_cast<double*>(h->scratch);
+  float2* tw = reinterpret_cast<float2*>(h->scratch + off_tw);
+  float2* S = reinterpret_cast<float2*>(h->scratch + off_s);
+  double* Z = reinterpret_cast<double*>(h->scratch + off_z);
+  CEEP_HIP(hipMemcpyAsync(d_f, freqs, F * sizeof(double), hipMemcpyHostToDevice, st));
+  CEEP_HIP(ceep::launch_twiddles(d_f, n_freq, h->T, h->dt, tw, st));
+  CEEP_HIP(ceep::launch_dft(h->traces, tw, h->B, h->P, h->T, n_freq, (float)h->dt, S, st));
+  CEEP_HIP(ceep::launch_c2z(S, Z, (long long)ns, st));
+  CEEP_HIP(hipMemcpyAsync(s_out, Z, ns * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipStreamSynchronize(st));
   return CEEP_OK;
 }
 

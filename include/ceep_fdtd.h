@@ -112,6 +112,17 @@ int ceep_fdtd_get_traces(ceep_fdtd* h, double* traces_out);
  * float64 (re, im). */
 int ceep_fdtd_dft(ceep_fdtd* h, const double* freqs, int n_freq, double* s_out);
 
+/* ceep_fdtd_dft without any device or host allocation: the frequency list, the
+ * twiddle table (formed on the device from the same FP64 expressions) and the
+ * S staging live in one static device array of 16 MiB per GPU, which every
+ * GPU the library runs on keeps until the process ends.  Rejects (CEEP_EINVAL)
+ * a call whose n_freq x total_steps twiddles and S staging exceed it.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other
+ * threads or streams) race.  No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+int ceep_fdtd_dft_noalloc(ceep_fdtd* h, const double* freqs, int n_freq, double* s_out);
+
 /* Copy the current field component comp (2D: 0 = ez, 1 = hx, 2 = hy; 3D:
  * 0..5 = ex, ey, ez, hx, hy, hz) to a full-grid (batch, nx, ny[, nz])
  * float64 array (cells outside the stepped box are exactly 0 in the

@@ -13,6 +13,8 @@ solvers turn into the ADE.
   CSF, grey, white matter) plus a hemorrhage disk inside the white matter.
 * ``DebyeHeadPhantom3D`` -- the same layers as ellipsoids, a spherical
   hemorrhage and an optional Lorentz pole in every tissue cell.
+* ``BrainPhantom2D``     -- the reference's concentric Cole-Cole head
+  (``get_eps_map`` with the Gabriel tissues of ``ceep_amd.tissues``).
 """
 from __future__ import annotations
 
@@ -170,3 +172,70 @@ class DebyeHeadPhantom3D:
             out["lorentz_omega_0"] = np.where(tissue, w0, 0.0)
             out["lorentz_delta"] = np.where(tissue, dl, 0.0)
         return out
+
+
+class BrainPhantom2D:
+    """Concentric circular head cross-section with an optional hemorrhage disk
+    (src/ceep/phantoms/head_models.py:497-631): skin, skull, CSF, grey and
+    white matter of 3, 7, 2 and 10 mm thickness inside a ``head_radius_cm``
+    circle centred at (nx // 2, ny // 2).
+
+    ``get_eps_map(f)`` returns (eps_r, sigma) maps.  With the Gabriel tissues,
+    eps_r = Re eps*(f) and sigma = -Im eps*(f) 2 pi f 8.854e-12 -- the
+    reference's rounded vacuum permittivity at this one place (:581), kept so
+    the maps agree bit for bit; otherwise fixed (eps_r, sigma) per layer.
+    """
+
+    FIXED = {"skin": (40.0, 1.0), "skull": (12.0, 0.2), "csf": (68.0, 2.5),
+             "gray": (52.0, 1.8), "white": (38.0, 1.2), "blood": (61.0, 2.0)}
+    TISSUES = {"skin": "skin_wet", "skull": "skull_cortical_bone", "csf": "cerebrospinal_fluid",
+               "gray": "brain_gray_matter", "white": "brain_white_matter", "blood": "hemorrhage"}
+
+    def __init__(self, nx: int = 600, ny: int = 600, dx: float = 0.5e-3,
+                 hemorrhage_location=None, hemorrhage_radius: float = 1.0,
+                 head_radius_cm: float = 9.0, use_gabriel_database: bool = True):
+        self.nx, self.ny, self.dx = nx, ny, dx
+        self.head_radius = head_radius_cm * 1e-2
+        self.hemorrhage_location = hemorrhage_location
+        self.hemorrhage_radius = hemorrhage_radius * 1e-2
+        self.use_gabriel = use_gabriel_database
+        self.skin_thickness = 3e-3
+        self.skull_thickness = 7e-3
+        self.csf_thickness = 2e-3
+        self.gray_thickness = 10e-3
+
+    def _layer_values(self, frequency: float):
+        if not self.use_gabriel:
+            return dict(self.FIXED)
+        from .tissues import tissue
+        f = np.array([frequency])
+        out = {}
+        for layer, key in self.TISSUES.items():
+            e = tissue(key).permittivity(f)[0]
+            out[layer] = (e.real, -e.imag * 2 * np.pi * frequency * 8.854e-12)
+        return out
+
+    def get_eps_map(self, frequency: float):
+        vals = self._layer_values(frequency)
+        eps_r = np.ones((self.nx, self.ny), dtype=np.float64)
+        sigma = np.zeros((self.nx, self.ny), dtype=np.float64)
+        cx, cy = self.nx // 2, self.ny // 2
+        gx, gy = np.meshgrid(np.arange(self.nx), np.arange(self.ny), indexing="ij")
+        r = np.sqrt(((gx - cx) * self.dx) ** 2 + ((gy - cy) * self.dx) ** 2)
+        # layer boundaries, outermost first
+        edges = [self.head_radius]
+        for t in (self.skin_thickness, self.skull_thickness, self.csf_thickness,
+                  self.gray_thickness):
+            edges.append(edges[-1] - t)
+        for k, layer in enumerate(("skin", "skull", "csf", "gray")):
+            m = (r <= edges[k]) & (r > edges[k + 1])
+            eps_r[m], sigma[m] = vals[layer]
+        m = r <= edges[4]
+        eps_r[m], sigma[m] = vals["white"]
+        if self.hemorrhage_location is not None:
+            hx, hy = self.hemorrhage_location
+            rh = np.sqrt(((gx - cx) * self.dx - hx * 1e-2) ** 2 +
+                         ((gy - cy) * self.dx - hy * 1e-2) ** 2)
+            m = rh <= self.hemorrhage_radius
+            eps_r[m], sigma[m] = vals["blood"]
+        return eps_r, sigma

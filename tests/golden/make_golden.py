@@ -27,6 +27,13 @@ Vectors written (all float64, ``numpy.savez_compressed``, no pickles):
 * ``runcpu_*.npz`` -- ``BatchedFDTD2D.run_cpu()`` (fdtd_2d_batched.py:540-611):
   interior sources with a lossy disk, and sources / probes on and next to the
   PEC walls (including corners and duplicated probes).
+* ``brain2d.npz`` -- ``BrainPhantom2D.get_eps_map`` (phantoms/head_models.py:
+  497-631) with and without the Gabriel tissues and a hemorrhage, and
+  ``TissueProperties.permittivity`` / ``conductivity``
+  (materials/tissue_database.py:92-137) of its six tissues.
+* ``sparams.npz`` -- ``MultistaticSParameters`` accessors, ``to_db``,
+  ``subtract_background``, ``MultistaticDataCollector`` and
+  ``extract_s_parameters`` (solvers/s_params.py:29-353) on seeded data.
 * ``waveforms.npz`` -- ``ModulatedGaussianSource.waveform`` and ``value_at``
   (sources/waveforms.py:97-132) at the configurations' parameters, plus the
   CFL time steps and constants of core/constants.py:21-27,105-175.
@@ -356,6 +363,83 @@ def golden_waveforms():
     np.savez_compressed(os.path.join(HERE, "waveforms.npz"), **out)
 
 
+def golden_brain2d():
+    """``BrainPhantom2D.get_eps_map`` (head_models.py:497-631): Gabriel tissues
+    at 2 GHz on a 200 x 180 grid with a hemorrhage, the fixed-value layers,
+    and the 600^2 default head at 1 GHz; the six tissues' Cole-Cole
+    permittivity and conductivity over 1 MHz - 10 GHz."""
+    from ceep.materials.tissue_database import TissueDatabase
+    from ceep.phantoms.head_models import BrainPhantom2D
+
+    out = {}
+    ph = BrainPhantom2D(nx=200, ny=180, dx=0.5e-3, hemorrhage_location=(2.0, -1.5),
+                        hemorrhage_radius=1.2, head_radius_cm=4.5)
+    out["a_eps"], out["a_sig"] = ph.get_eps_map(2e9)
+    ph = BrainPhantom2D(nx=200, ny=180, dx=0.5e-3, hemorrhage_location=(-1.0, 0.5),
+                        hemorrhage_radius=0.8, head_radius_cm=4.5, use_gabriel_database=False)
+    out["b_eps"], out["b_sig"] = ph.get_eps_map(2e9)
+    ph = BrainPhantom2D()
+    out["c_eps"], out["c_sig"] = ph.get_eps_map(1e9)
+    db = TissueDatabase()
+    freqs = np.logspace(6, 10, 25)
+    out["freqs"] = freqs
+    for key in ("skin_wet", "skull_cortical_bone", "cerebrospinal_fluid", "brain_gray_matter",
+                "brain_white_matter", "hemorrhage"):
+        t = db.get(key)
+        out[f"perm_{key}"] = t.permittivity(freqs)
+        out[f"cond_{key}"] = t.conductivity(freqs)
+    np.savez_compressed(os.path.join(HERE, "brain2d.npz"), **out)
+
+
+def golden_sparams():
+    """S-matrix accessors and exports (s_params.py:70-231), the collector
+    (:234-353; one receiver position missing from a record) and two-port
+    extraction (:29-66; incident values at and below 1e-15 included)."""
+    from ceep.solvers.s_params import (MultistaticDataCollector, MultistaticSParameters,
+                                       extract_s_parameters)
+
+    rng = np.random.default_rng(31)
+    F, N = 7, 5
+    freqs = np.linspace(1e9, 3e9, F)
+    S = rng.standard_normal((F, N, N)) + 1j * rng.standard_normal((F, N, N))
+    Sb = rng.standard_normal((F, N, N)) + 1j * rng.standard_normal((F, N, N))
+    S[2, 1, 3] = 0.0
+    ants = [(10 + 3 * k, 20 - k) for k in range(N)]
+    sp = MultistaticSParameters(frequencies=freqs, s_matrix=S, antenna_positions=ants)
+    bg = MultistaticSParameters(frequencies=freqs, s_matrix=Sb, antenna_positions=ants)
+    out = dict(freqs=freqs, s=S, s_bg=Sb, ants=np.array(ants))
+    out["db"] = sp.to_db().s_matrix
+    out["diff"] = sp.subtract_background(bg).s_matrix
+    out["mono"] = sp.get_monostatic()
+    out["mono3"] = sp.get_monostatic(3)
+    out["elem"] = sp.get_element(2, 4)
+    bi = sp.get_bistatic()
+    out["bi_txrx"] = np.array([(t, r) for t, r, _ in bi])
+    out["bi_s"] = np.array([v for _, _, v in bi])
+    bi_all = sp.get_bistatic(exclude_monostatic=False)
+    out["bi_all_txrx"] = np.array([(t, r) for t, r, _ in bi_all])
+
+    class Array:
+        def get_antenna_positions(self):
+            return ants
+
+    col = MultistaticDataCollector(Array(), freqs, dx=1e-3)
+    rec = rng.standard_normal((N, N, F)) + 1j * rng.standard_normal((N, N, F))
+    for tx in range(N):
+        d = {ants[rx]: rec[tx, rx] for rx in range(N) if (tx, rx) != (1, 2)}
+        col.record_transmission(tx, d)
+    out["rec"] = rec
+    out["collected"] = col.get_s_parameters().s_matrix
+    inc = rng.standard_normal(9) + 1j * rng.standard_normal(9)
+    inc[2] = 0.0
+    inc[5] = 1e-16
+    tot = rng.standard_normal(9) + 1j * rng.standard_normal(9)
+    trn = rng.standard_normal(9) + 1j * rng.standard_normal(9)
+    out["inc"], out["tot"], out["trn"] = inc, tot, trn
+    out["s11"], out["s21"] = extract_s_parameters(inc, tot, trn)
+    np.savez_compressed(os.path.join(HERE, "sparams.npz"), **out)
+
+
 def golden_das():
     """``DelayAndSumBeamformer.reconstruct`` (imaging/beamforming.py:86-313),
     DAS and DMAS, raw and normalised, on a seeded S-matrix."""
@@ -392,4 +476,6 @@ if __name__ == "__main__":
     golden_das()
     golden_run_cpu()
     golden_waveforms()
+    golden_brain2d()
+    golden_sparams()
     print("golden vectors written to", HERE)

@@ -1,5 +1,7 @@
 """Uneven batches: many ``BatchedFDTD2D`` problems (different grids, step
-counts, CPML widths, TX sets) advanced together in one persistent launch.
+counts, CPML widths, TX sets) advanced together: one captured hipGraph of
+per-round launches, each round one launch over every problem still running
+(the single persistent launch is switched off and refused by the library).
 
 Each solver keeps its own API and outputs; ``UnevenBatch.run()`` fills every
 solver's ``probe_data`` as its own ``run()`` would (bit for bit the same as

@@ -53,20 +53,24 @@ def _oracle(s, dt, w, m, debye):
     return fdtd2d.run_cavity(NX, NY, DX, dt, N, SRC, PRB, w, s._eps_r, s._sigma_e, poles=poles)
 
 
-CASES = [(1, 16, 1), (2, 24, 2), (3, 32, 1), (4, 16, 1), (4, 24, 3), (4, 32, 2), (2, 16, 8)]
+# (K, R, TX planes per window, wavefronts per window); the four-wavefront
+# shapes include the default forward (K = 6, R = 32, 2 planes)
+CASES = [(1, 16, 1, 1), (2, 24, 2, 1), (3, 32, 1, 1), (4, 16, 1, 1), (4, 24, 3, 1),
+         (4, 32, 2, 1), (2, 16, 8, 1),
+         (4, 32, 1, 4), (6, 32, 2, 4), (6, 32, 3, 4), (8, 48, 2, 4), (6, 64, 1, 4), (4, 48, 3, 4)]
 
 
 @pytest.mark.parametrize("debye", [False, True])
-@pytest.mark.parametrize("K,R,bpw", CASES)
-def test_blocked_matches_oracle_and_one_step(amd, debye, K, R, bpw):
+@pytest.mark.parametrize("K,R,bpw,waves", CASES)
+def test_blocked_matches_oracle_and_one_step(amd, debye, K, R, bpw, waves):
     s, dt, w, m = _solver(amd, debye, {"temporal_depth": K, "rows_per_wave": R,
-                                       "batch_per_wave": bpw})
+                                       "batch_per_wave": bpw, "forward_waves": waves})
     s.run()
     ref = _oracle(s, dt, w, m, debye)
     err = rel_l2(s.probe_data, ref)
     one, *_ = _solver(amd, debye, {"temporal_depth": 0})
     one.run()
-    print(f"K={K} R={R} bpw={bpw} debye={debye}: rel-L2 vs oracle {err:.2e}, "
+    print(f"K={K} R={R} bpw={bpw} waves={waves} debye={debye}: rel-L2 vs oracle {err:.2e}, "
           f"vs one-step {rel_l2(s.probe_data, one.probe_data):.2e}")
     assert err < TRACE_TOL
     assert rel_l2(s.probe_data, one.probe_data) < 1e-6
@@ -95,16 +99,126 @@ def test_blocked_fields_continuation_and_energy(amd):
     assert s.compute_energy() == pytest.approx(e, rel=1e-5)
 
 
-def test_blocked_run_cpu_wall_sources(amd):
+@pytest.mark.parametrize("tuning", [
+    {"temporal_depth": 3, "rows_per_wave": 16, "batch_per_wave": 1},
+    {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 3, "forward_waves": 4},
+])
+def test_blocked_run_cpu_wall_sources(amd, tuning):
     nx, ny, T2 = 70, 61, 83
     pos = [(30, 30), (0, 10), (40, 60), (69, 20)]
     prb = [(30, 30), (0, 10), (35, 33), (69, 20)]
     s = amd.BatchedFDTD2D(nx=nx, ny=ny, dx=1e-3, total_steps=T2, cpml_thickness=5,
                           source_positions=pos, probe_positions=prb, frequency=3e9)
     s.set_material_circle(34, 30, 9, eps_r=7.0, sigma_e=0.3)
-    s.tuning = {"temporal_depth": 3, "rows_per_wave": 16, "batch_per_wave": 1}
+    s.tuning = tuning
     out = s.run_cpu()
     ref = fdtd2d.run_cpu(nx, ny, 1e-3, s.dt, pos, prb, W.reference_waveform(T2, s.dt, 3e9),
                          s._eps_r, s._sigma_e)
     tr = np.array([[out[t][r] for r in out[t]] for t in out])
     assert rel_l2(tr, ref) < TRACE_TOL
+
+
+@pytest.mark.parametrize("case", ["interior", "walls"])
+@pytest.mark.parametrize("tuning", [
+    {"temporal_depth": 0},
+    {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2, "forward_waves": 4},
+])
+def test_run_cpu_matches_reference_vectors(amd, golden_dir, case, tuning):
+    """run_cpu on the GPU against BatchedFDTD2D.run_cpu() outputs the reference
+    wrote (tests/golden/runcpu_*.npz: sources and probes on the walls, in the
+    corners, duplicated)."""
+    import os
+    z = np.load(os.path.join(golden_dir, f"runcpu_{case}.npz"))
+    src = [tuple(int(v) for v in p) for p in z["sources"]]
+    prb = [tuple(int(v) for v in p) for p in z["probes"]]
+    s = amd.BatchedFDTD2D(nx=int(z["nx"]), ny=int(z["ny"]), dx=float(z["dx"]),
+                          total_steps=z["traces"].shape[-1], cpml_thickness=4,
+                          source_positions=src, probe_positions=prb,
+                          frequency=float(z["frequency"]), delay_factor=float(z["delay_factor"]))
+    s._eps_r[:] = z["eps_r"]
+    s._sigma_e[:] = z["sigma_e"]
+    s.tuning = tuning
+    out = s.run_cpu()
+    tr = np.array([[out[t][r] for r in out[t]] for t in out])
+    err = rel_l2(tr, z["traces"])
+    print(f"run_cpu {case} {tuning}: rel-L2 vs the reference's vectors {err:.2e}")
+    assert s.dt == float(z["dt"])
+    assert err < TRACE_TOL
+
+
+@pytest.fixture(scope="module")
+def edge_autograd(amd):
+    """FP64 autograd of |S - S_target|^2 through the torch loop on the edge
+    geometry (non-dispersive head maps, T = 97)."""
+    torch = pytest.importorskip("torch")
+    from oracle import dft
+    from oracle.torch_loop import run_cavity_torch, s_matrix_torch
+    s, dt, w, m = _solver(amd, False, {"temporal_depth": 0})
+    freqs = np.linspace(4e9, 9e9, 12)
+    eps, sig = s._eps_r.copy(), s._sigma_e.copy()
+    eps_t = eps.copy()
+    eps_t[60:80, 55:75] *= 1.3
+    St = dft.running_dft(fdtd2d.run_cavity(NX, NY, DX, dt, N, SRC, PRB, w, eps_t, sig), freqs, dt)
+    e = torch.tensor(eps, requires_grad=True)
+    sg = torch.tensor(sig, requires_grad=True)
+    S = s_matrix_torch(run_cavity_torch(NX, NY, DX, dt, N, SRC, PRB, w, e, sg), freqs, dt)
+    d = S - torch.tensor(St)
+    L = (d.real ** 2 + d.imag ** 2).sum()
+    L.backward()
+    return dict(freqs=freqs, eps=eps, sig=sig, eps_t=eps_t, St=St,
+                ge=e.grad.numpy().copy(), gs=sg.grad.numpy().copy())
+
+
+@pytest.mark.parametrize("tuning,every", [
+    ({"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2, "forward_waves": 4}, 45),
+    ({"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 3, "forward_waves": 4,
+      "adjoint_batch_per_wave": 3}, 97),
+    ({"temporal_depth": 4, "rows_per_wave": 32, "batch_per_wave": 1, "forward_waves": 4,
+      "adjoint_temporal_depth": 5, "adjoint_batch_per_wave": 2}, 31),
+    ({"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2, "forward_waves": 4,
+      "adjoint_temporal_depth": 4, "adjoint_rows_per_wave": 24, "adjoint_batch_per_wave": 8}, 50),
+])
+def test_four_wavefront_reversible_adjoint_edges(amd, edge_autograd, tuning, every):
+    """The default four-wavefront forward and reversible adjoint on the odd
+    grid with wall-adjacent sources, wall / off-box / duplicated probes and
+    T = 97 (remainder launches in every segment), against FP64 autograd --
+    for both the total-field misfit and the difference form."""
+    from ceep_amd import adjoint
+    r = edge_autograd
+    for mode in ("s_target", "target"):
+        s, *_ = _solver(amd, False, tuning)
+        if mode == "target":
+            s.tuning = {**tuning}
+            _, ge, gs = adjoint.gradient(s, r["eps"], r["sig"], r["freqs"], every=every,
+                                         target=(r["eps_t"], r["sig"]))
+        else:
+            _, ge, gs = adjoint.gradient(s, r["eps"], r["sig"], r["freqs"], r["St"], every=every)
+        ee, es = rel_l2(ge, r["ge"]), rel_l2(gs, r["gs"])
+        print(f"{mode} {tuning} every={every}: dL/deps rel-L2 {ee:.2e}, dL/dsigma {es:.2e}")
+        assert ee < 1e-3 and es < 1e-3
+        assert not ge[:N + 1].any() and not gs[:, :N + 1].any()
+
+
+def test_material_ids_bit_identical(amd):
+    """Piecewise-constant media read as one byte per cell plus a table in LDS
+    (the Debye head has 8 distinct coefficient tuples) give the same bits as
+    the full coefficient maps, in a plain run and in an uneven batch."""
+    from ceep_amd.uneven import UnevenBatch
+    tun = {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2, "forward_waves": 4}
+    out = {}
+    for ids in (0, 1):
+        s, *_ = _solver(amd, True, {**tun, "material_ids": ids})
+        s.run()
+        out[ids] = s.probe_data.copy()
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1])
+    ub_out = {}
+    for ids in (0, 1):
+        s1, *_ = _solver(amd, True, {**tun, "material_ids": ids})
+        s2, *_ = _solver(amd, True, {**tun, "material_ids": ids}, T=61)
+        ub = UnevenBatch([s1, s2], 6, 32, 2, 4)
+        ub.run()
+        ub_out[ids] = (s1.probe_data.copy(), s2.probe_data.copy())
+        ub.close()
+    assert np.array_equal(ub_out[0][0], ub_out[1][0]) and np.array_equal(ub_out[0][1], ub_out[1][1])
+    assert np.array_equal(ub_out[1][0], out[1])

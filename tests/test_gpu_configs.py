@@ -18,7 +18,8 @@ S-matrix rel-L2 over all (f, rx, tx); bars: traces <= 1e-5, S <= 1e-4.
      512, 600; steps U[600, 1200]; CPML 12-15) run together as one uneven batch
   5  3D 200^3, dx 1 mm, 1000 steps, 10-cell CPML, 16 Ez TX = RX on the r = 70
      ring at z = 100, Debye + Lorentz head seed 0, S at 101 frequencies; the
-     oracle steps TX 0 and TX 9 (TX planes are independent) with all 16 RX
+     oracle steps TX 0, 9 and 15 (TX planes are independent; 15 sits in the
+     partial last round of workgroups) with all 16 RX
 """
 import numpy as np
 import pytest
@@ -242,7 +243,7 @@ def test_config5_3d_debye_lorentz_200(amd):
     m = ph.get_debye_maps()
     poles = [ade.debye_pole(m["delta_eps"], m["tau"], dt),
              ade.lorentz_pole(m["lorentz_delta_eps"], m["lorentz_omega_0"], m["lorentz_delta"], dt)]
-    pick = [0, 9]
+    pick = [0, 9, 15]
     ref = c_ref.run_cavity3d((n3, n3, n3), dx3, dt, 10, [pos[b] for b in pick], pos, w,
                              m["eps_inf"], m["sigma"], poles=poles)
     tr = s.probe_data[pick]
@@ -251,3 +252,42 @@ def test_config5_3d_debye_lorentz_200(amd):
     print(f"\nconfig 5: 200^3 B=16 T=1000 Debye+Lorentz head, TX {pick} x 16 RX: traces "
           f"rel-L2 {et:.2e}, S (101 freqs) rel-L2 {es:.2e}", flush=True)
     assert et <= 1e-5 and es <= 1e-4
+
+
+@pytest.mark.timeout(600)
+def test_config5_gradient_fd_2tx(amd, golden_dir):
+    """The configuration-5 gradient at full size on a 2-TX sub-batch (TX 0 and
+    9 of the ring, all 16 probes; 200^3, 1000 steps, checkpoints every 100,
+    Debye + Lorentz head seed 0 against seed 1, 101 frequencies): dL/d eps_r
+    and dL/d sigma at four voxels against FP64 central differences (h = 1e-4)
+    of the oracle's C restatement, stored with the FP64 target S-matrix and
+    loss in tests/golden/fd3d_config5.npz (tests/golden/make_fd3d.py)."""
+    import os
+    from ceep_amd import adjoint
+    from ceep_amd.constants import cfl_timestep_3d
+    from ceep_amd.waveforms import modulated_gaussian
+    z = np.load(os.path.join(golden_dir, "fd3d_config5.npz"))
+    n3, dx3, T = 200, 1e-3, 1000
+    dt = cfl_timestep_3d(dx3, dx3, dx3, 0.5) * 0.99
+    tx = [tuple(int(v) for v in p) for p in z["tx"]]
+    prb = [tuple(int(v) for v in p) for p in z["probes"]]
+    w = modulated_gaussian(T, dt, 2e9, 1.5e9)
+    s = amd.BatchedFDTD3D(nx=n3, ny=n3, nz=n3, dx=dx3, total_steps=T, cpml_thickness=10,
+                          source_positions=tx, probe_positions=prb, frequency=2e9, waveform=w)
+    s.set_phantom(amd.phantoms.DebyeHeadPhantom3D((n3, n3, n3), 0))
+    St = z["s_target"]
+    loss, ge, gs = adjoint.gradient(s, s._eps_r.copy(), s._sigma_e.copy(), z["freqs"], St,
+                                    every=100)
+    worst = 0.0
+    for grad, key in ((ge, "eps"), (gs, "sigma")):
+        for flat, fd in zip(z[f"{key}_idx"], z[f"{key}_fd"]):
+            v = np.unravel_index(int(flat), (n3, n3, n3))
+            dev = abs(grad[v] - fd) / abs(fd)
+            worst = max(worst, dev)
+            print(f"\n  d L / d {key} at {v}: adjoint {grad[v]:.6e}, FD {fd:.6e}, rel dev {dev:.2e}",
+                  end="")
+    print(f"\nconfig 5 gradient: 200^3, TX 0 and 9, 1000 steps, checkpoints every 100: loss "
+          f"{loss:.4e} (FP64 oracle {float(z['loss']):.4e}), residual |S - S_target| / |S_target| "
+          f"= {np.sqrt(float(z['loss'])) / np.linalg.norm(St):.1e}; worst relative deviation "
+          f"from FP64 central differences over 4 voxels {worst:.2e}", flush=True)
+    assert worst < 1e-2

@@ -146,6 +146,7 @@ TUNE_TB3D_DEPTH = 17
 TUNE_TX_GROUP_3D = 18
 TUNE_ADJ3D_DEPTH = 19
 TUNE_ADJ3D_ORDER = 20
+TUNE_MATERIAL_IDS = 22
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
@@ -157,7 +158,7 @@ TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "adjoint_temporal_depth": TUNE_ADJ_DEPTH, "forward_waves": TUNE_FWD_WAVES,
                "temporal_depth_3d": TUNE_TB3D_DEPTH, "tx_group_3d": TUNE_TX_GROUP_3D,
                "adjoint_temporal_depth_3d": TUNE_ADJ3D_DEPTH,
-               "adjoint_order_3d": TUNE_ADJ3D_ORDER}
+               "adjoint_order_3d": TUNE_ADJ3D_ORDER, "material_ids": TUNE_MATERIAL_IDS}
 
 
 class Handle:

@@ -206,7 +206,15 @@ struct TB2DArgs {
   const float* e_ref;
   long long ref_slot;
   const unsigned char* diff_tile; // [tiles_x * tiles_y]
+  // piecewise-constant media: per-cell material id ([Mx][pitch] bytes; id 0 =
+  // all coefficients 0, off the E interior) and the table of (ca, cb, k1,
+  // k2dt) per id; null: the full maps above are read
+  const unsigned char* mat_id;
+  const float4* mat_table;
+  int n_mat;                      // <= kMaxMaterials
 };
+
+constexpr int kMaxMaterials = 256;
 
 // The difference form's `poles` value (non-dispersive media only).
 constexpr int kPolesDiff = 4;
@@ -301,7 +309,7 @@ hipError_t launch_round_tb(const MultiArgs& m, int poles, int R, int K, int seg0
 // The same rounds with four wavefronts per window (R = 32 or 48 window rows,
 // K = 4 or 6): one workgroup per work item.
 hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
-                           long long first, long long count, hipStream_t s);
+                           long long first, long long count, bool mat_ids, hipStream_t s);
 int multi_tb_blocks_per_cu(int poles, int R, int K);
 // dst[i] = sum over k < nparts of src[k * stride + i], i < n
 hipError_t launch_sum_parts(const float* src, int nparts, long long stride, long long n,

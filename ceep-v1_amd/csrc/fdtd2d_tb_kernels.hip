@@ -1149,7 +1149,7 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
 // K = 8.  Per-cell arithmetic is that of step2d_tb_kernel.
 // One task (output tile, TX chunk) advanced p.ksteps steps by the four
 // wavefronts of the calling workgroup; x_e / x_h are its LDS exchange rows.
-template <int RW, int K, int POLES>
+template <int RW, int K, int POLES, bool MID>
 __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, int w,
                                         float (*x_e)[64], float (*x_h)[64]) {
   constexpr int R = 4 * RW;
@@ -1174,7 +1174,24 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
   const bool has_up = w > 0, has_down = w < 3;
 
   float mca[RW], mcb[RW], mk1[RW], mk2[RW];
-  {
+  if (MID) {
+    // piecewise-constant media: one byte per cell (cells off the box read
+    // id 0, the all-zero entry), then the (ca, cb, k1, k2dt) table entry
+    const __amdgpu_buffer_rsrc_t rid = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<unsigned char*>(p.mat_id), 0, (int)(plane_bytes >> 2), 0x00020000);
+    int id[RW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+      id[i] = (int)__builtin_amdgcn_raw_buffer_load_b8(rid, (int)(row_off(i) >> 2), 0, 0);
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+      const float4 m = p.mat_table[id[i]];
+      mca[i] = m.x;
+      mcb[i] = m.y;
+      mk1[i] = m.z;
+      mk2[i] = m.w;
+    }
+  } else {
     const __amdgpu_buffer_rsrc_t rca = plane_rsrc(p.ca, plane_bytes);
     const __amdgpu_buffer_rsrc_t rcb = plane_rsrc(p.cb, plane_bytes);
 #pragma unroll
@@ -1380,19 +1397,19 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
   }
 }
 
-template <int RW, int K, int POLES>
+template <int RW, int K, int POLES, bool MID>
 __global__ __launch_bounds__(256) void step2d_mw_kernel(TB2DArgs p) {
   __shared__ float x_e[4][64], x_h[4][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int task = xcd_block(p.xcd_order);
   if (task >= p.ntasks) return;                       // uniform per workgroup
-  mw_task<RW, K, POLES>(p, task, lane, w, x_e, x_h);
+  mw_task<RW, K, POLES, MID>(p, task, lane, w, x_e, x_h);
 }
 
 // Uneven batch, one launch per round, four wavefronts per window: each
 // workgroup is one work item (problem, tile, TX chunk) of the round.
-template <int RW, int K, int POLES>
+template <int RW, int K, int POLES, bool MID>
 __global__ __launch_bounds__(256) void round_mw_kernel(MultiArgs m, int seg0, int seg1,
                                                        long long first, long long count) {
   __shared__ float x_e[4][64], x_h[4][64];
@@ -1418,27 +1435,31 @@ __global__ __launch_bounds__(256) void round_mw_kernel(MultiArgs m, int seg0, in
   a.step0 = round * K;
   a.ksteps = min(K, a.T - a.step0);
   a.e_save = nullptr;
-  mw_task<RW, K, POLES>(a, task, lane, w, x_e, x_h);
+  // every problem of the batch is or is not given material ids (the host
+  // sets them for all or none)
+  mw_task<RW, K, POLES, MID>(a, task, lane, w, x_e, x_h);
 }
 
-template <int RW, int K>
+template <int RW, int K, bool MID>
 static hipError_t launch_round_mw_rk(const MultiArgs& m, int poles, int seg0, int seg1,
                                      long long first, long long count, hipStream_t s) {
   const dim3 blk(256, 1, 1), grd((unsigned)count, 1, 1);
-  if (poles == 0) hipLaunchKernelGGL((round_mw_kernel<RW, K, 0>), grd, blk, 0, s, m, seg0, seg1, first, count);
-  else if (poles == 1) hipLaunchKernelGGL((round_mw_kernel<RW, K, 1>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  if (poles == 0) hipLaunchKernelGGL((round_mw_kernel<RW, K, 0, MID>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  else if (poles == 1) hipLaunchKernelGGL((round_mw_kernel<RW, K, 1, MID>), grd, blk, 0, s, m, seg0, seg1, first, count);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
-                           long long first, long long count, hipStream_t s) {
+                           long long first, long long count, bool mat_ids, hipStream_t s) {
   if (count < 1) return hipSuccess;
   if (count > 0x7fffffffLL) return hipErrorInvalidValue;
 #define CEEP_RMW(RW)                                                                         \
   switch (K) {                                                                               \
-    case 4: return launch_round_mw_rk<RW, 4>(m, poles, seg0, seg1, first, count, s);          \
-    case 6: return launch_round_mw_rk<RW, 6>(m, poles, seg0, seg1, first, count, s);          \
+    case 4: return mat_ids ? launch_round_mw_rk<RW, 4, true>(m, poles, seg0, seg1, first, count, s)   \
+                           : launch_round_mw_rk<RW, 4, false>(m, poles, seg0, seg1, first, count, s); \
+    case 6: return mat_ids ? launch_round_mw_rk<RW, 6, true>(m, poles, seg0, seg1, first, count, s)   \
+                           : launch_round_mw_rk<RW, 6, false>(m, poles, seg0, seg1, first, count, s); \
   }                                                                                          \
   return hipErrorInvalidValue;
   if (R == 32) { CEEP_RMW(8) }
@@ -1450,9 +1471,12 @@ hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0
 template <int RW, int K>
 static hipError_t launch_mw_rk(const TB2DArgs& a, int poles, hipStream_t s) {
   const dim3 blk(256, 1, 1), grd((unsigned)a.ntasks, 1, 1);
-  if (poles == 0) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0>), grd, blk, 0, s, a);
-  else if (poles == 1) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1>), grd, blk, 0, s, a);
-  else if (poles == kPolesDiff) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, kPolesDiff>), grd, blk, 0, s, a);
+  const bool ids = a.mat_id != nullptr;
+  if (poles == 0 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, true>), grd, blk, 0, s, a);
+  else if (poles == 0) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, false>), grd, blk, 0, s, a);
+  else if (poles == 1 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, true>), grd, blk, 0, s, a);
+  else if (poles == 1) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, false>), grd, blk, 0, s, a);
+  else if (poles == kPolesDiff) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, kPolesDiff, false>), grd, blk, 0, s, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

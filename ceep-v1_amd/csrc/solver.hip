@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -210,6 +211,13 @@ struct ceep_fdtd {
   std::vector<float> ref_traces;   // [B][P][T]
   bool diff_active = false;    // launch_range steps the difference form
 
+  // piecewise-constant 2D media: per-cell material id and the (ca, cb, k1,
+  // k2dt) table the blocked four-wavefront kernels read instead of the maps
+  unsigned char* mat_id = nullptr;
+  float4* mat_table = nullptr;
+  int n_mat = 0;               // 0: no id map (too many distinct cells, or switched off)
+  int mat_ids_on = 1;          // tuning key 22
+
   bool mats_dirty = true;
   bool has_run = false;
   int bb = 4;
@@ -337,6 +345,60 @@ int upload_f32(float* dst, const std::vector<float>& v, hipStream_t s) {
 
 // FP64 coefficient maps (fdtd_2d_batched.py:152-165, fdtd_3d_batched.py:282-296,
 // dispersive.py:253-299, core/grid.py:178-184) over the box, stored as FP32.
+// Material ids of piecewise-constant 2D media (host-formed maps): the
+// distinct (ca, cb, k1, k2dt) tuples of the box, id 0 the all-zero tuple of
+// cells off the E interior; none when there are more than kMaxMaterials, the
+// maps were formed on the device, or the feature is off.  The tables hold
+// the exact FP32 map values, so results are bit-identical to the maps'.
+int build_material_ids(ceep_fdtd* h, const std::vector<float>& ca, const std::vector<float>& cb,
+                       const std::vector<float>& k1, const std::vector<float>& k2) {
+  dfree(h->mat_id);
+  dfree(h->mat_table);
+  h->n_mat = 0;
+  if (h->ndim != 2 || h->cpml() || h->has_lorentz || !h->mat_ids_on || ca.empty() ||
+      h->plane >= (1LL << 29))
+    return CEEP_OK;
+  struct Key {
+    uint32_t v[4];
+    bool operator==(const Key& o) const {
+      return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2] && v[3] == o.v[3];
+    }
+  };
+  struct Hash {
+    size_t operator()(const Key& k) const {
+      uint64_t x = 1469598103934665603ULL;
+      for (uint32_t u : k.v) x = (x ^ u) * 1099511628211ULL;
+      return (size_t)x;
+    }
+  };
+  auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
+  std::unordered_map<Key, int, Hash> ids;
+  std::vector<float4> table;
+  ids.emplace(Key{{0u, 0u, 0u, 0u}}, 0);
+  table.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+  std::vector<unsigned char> id((size_t)h->plane, 0);
+  const bool pole = !k1.empty();
+  for (long long o = 0; o < h->plane; ++o) {
+    const float a = ca[o], b = cb[o], c = pole ? k1[o] : 0.f, d = pole ? k2[o] : 0.f;
+    const Key k{{bits(a), bits(b), bits(c), bits(d)}};
+    auto it = ids.find(k);
+    if (it == ids.end()) {
+      if ((int)table.size() == ceep::kMaxMaterials) return CEEP_OK;   // not piecewise constant
+      it = ids.emplace(k, (int)table.size()).first;
+      table.push_back(make_float4(a, b, c, d));
+    }
+    id[(size_t)o] = (unsigned char)it->second;
+  }
+  int rc;
+  if ((rc = dalloc(&h->mat_id, id.size())) || (rc = dalloc(&h->mat_table, table.size()))) return rc;
+  CEEP_HIP(hipMemcpyAsync(h->mat_id, id.data(), id.size(), hipMemcpyHostToDevice, h->stream));
+  CEEP_HIP(hipMemcpyAsync(h->mat_table, table.data(), table.size() * sizeof(float4),
+                          hipMemcpyHostToDevice, h->stream));
+  CEEP_HIP(hipStreamSynchronize(h->stream));
+  h->n_mat = (int)table.size();
+  return CEEP_OK;
+}
+
 // 2D without poles: ca / cb (and, with a reference medium, its maps and the
 // difference-form maps) formed on the device from the FP64 maps
 int commit_materials_2d_device(ceep_fdtd* h) {
@@ -436,6 +498,7 @@ int commit_materials(ceep_fdtd* h) {
   } else if ((rc = upload_f32(h->ca, ca, s)) || (rc = upload_f32(h->cb, cb, s))) {
     return rc;
   }
+  if ((rc = build_material_ids(h, ca, cb, dk1, dk2))) return rc;
   dfree(h->da); dfree(h->dbi); dfree(h->dk1); dfree(h->dk2);
   dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
   if (!da.empty()) {
@@ -572,6 +635,7 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     a.xcd_order = h->xcd_order;
     a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
     a.ca = h->ca; a.cb = h->cb; a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
+    if (h->n_mat > 0) { a.mat_id = h->mat_id; a.mat_table = h->mat_table; a.n_mat = h->n_mat; }
     a.ch = h->ch; a.idx = h->idx;
     a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
     a.T = h->T; a.P = h->P; a.probe_cell = h->probe_cell; a.traces = h->traces;
@@ -1034,6 +1098,7 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   dfree(h->ca64); dfree(h->cb64); dfree(h->wave64); dfree(h->traces64);
   h->free_reference();
   dfree(h->eps64); dfree(h->sig64); dfree(h->grad64);
+  dfree(h->mat_id); dfree(h->mat_table);
   dfree(h->ref_eps64); dfree(h->ref_sig64);
   for (int k = 0; k < 12; ++k) dfree(h->psi[k]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -2126,6 +2191,9 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 0 || value > 2)
       return fail(CEEP_EINVAL, "3D adjoint steps per launch must be 0 (three-kernel form), 1 or 2");
     h->adj3d_k = value;
+  } else if (key == 22) {
+    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "material ids must be 0 or 1");
+    if (h->mat_ids_on != value) { h->mat_ids_on = value; h->mats_dirty = true; }
   } else if (key == 20) {
     if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D adjoint block order must be 0, 1 or 2");
     h->adj3d_order = value;
@@ -2264,6 +2332,7 @@ int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out) {
 struct ceep_fdtd_multi {
   std::vector<ceep_fdtd*> hs;
   int K = 4, R = 16, bpw = 1, poles = 0, blocks = 0, waves = 1;
+  bool mat_ids = false;        // every problem reads material ids (four-wavefront rounds)
   ceep::MultiDesc* d_desc = nullptr;
   long long* d_seg_start = nullptr;
   int* d_seg_prob = nullptr;
@@ -2324,7 +2393,8 @@ int multi_enqueue(ceep_fdtd_multi* m, int mode) {
     const long long first = m->seg_start_h[s0];
     const long long last = s1 < m->nseg ? m->seg_start_h[s1] : (long long)m->total;
     if (m->waves == 4)
-      CEEP_HIP(ceep::launch_round_mw(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
+      CEEP_HIP(ceep::launch_round_mw(a, m->poles, m->R, m->K, s0, s1, first, last - first,
+                                     m->mat_ids, s));
     else
       CEEP_HIP(ceep::launch_round_tb(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
   }
@@ -2363,6 +2433,9 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   ceep_fdtd_multi* m = new ceep_fdtd_multi();
   m->hs.assign(handles, handles + n);
   m->K = K; m->R = R; m->bpw = bpw; m->poles = poles; m->waves = waves;
+  // material ids only when every problem has them (one kernel form per round)
+  m->mat_ids = waves == 4;
+  for (int i = 0; i < n; ++i) m->mat_ids = m->mat_ids && handles[i]->n_mat > 0;
   std::vector<ceep::MultiDesc> desc(n);
   std::vector<int> ntasks(n);
   int max_rounds = 0;
@@ -2371,6 +2444,7 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
     ceep::TB2DArgs& a = desc[i].a;
     a.Mx = h->M[0]; a.My = h->M[1]; a.pitch = h->pitch; a.B = h->B; a.bstride = h->plane;
     a.ca = h->ca; a.cb = h->cb; a.d_k1 = h->dk1; a.d_k2dt = h->dk2;
+    if (m->mat_ids) { a.mat_id = h->mat_id; a.mat_table = h->mat_table; a.n_mat = h->n_mat; }
     a.ch = h->ch; a.idx = h->idx;
     a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
     a.T = h->T; a.P = h->P; a.probe_cell = h->probe_cell; a.traces = h->traces;

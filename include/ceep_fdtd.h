@@ -280,10 +280,10 @@ int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out);
 /* ---- uneven batches (no counterpart in the reference) ----------------------
  * Many 2D problems of different grid sizes and step counts, each a fully set
  * up handle (materials, sources, probes; no Lorentz poles; one pole set for
- * all), advanced together in ONE persistent kernel launch on a work queue of
- * (K-step round, problem, window) items; the launch sequence is captured as a
- * hipGraph on the first run.  Traces land in each handle (ceep_fdtd_get_traces,
- * ceep_fdtd_dft). */
+ * all), advanced together round by round: each round (K steps of every
+ * problem still running) is one launch over all problems' windows, and the
+ * clears and rounds are captured as one hipGraph on the first run.  Traces
+ * land in each handle (ceep_fdtd_get_traces, ceep_fdtd_dft). */
 typedef struct ceep_fdtd_multi ceep_fdtd_multi;
 
 /* waves = 1: one wavefront per window, K = steps per round (1..4), R =

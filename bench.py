@@ -347,7 +347,7 @@ def _bench_3d(steps_timed=2, T3=1000, tuning=None):
                          f"1000 steps, Debye+Lorentz head seed 0 ({kernel}, {k3} steps per launch)")}
 
 
-def _bench_uneven(n_problems=64, seed=4, autotune=True):
+def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False):
     """64 seeded Debye heads (the head generator above scaled to each grid),
     grids drawn from {256, 384, 512, 600}^2, steps U[600, 1200], 16 TX = RX on a
     ring of radius 200/600 of the grid, CPML 12-15 cells, each followed by its
@@ -380,6 +380,11 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True):
     if autotune:
         shape, log = autotune_uneven(solvers)
     ub = UnevenBatch(solvers, *shape)
+    if single:                       # one batch, nothing timed (for rocprofv3 --pmc)
+        ub._multi.run(UnevenBatch.PER_ROUND)
+        _device_sync()
+        ub.close()
+        return None
     cell_steps = ub.cell_steps
     full_cells = sum(g * g * B * s.total_steps for g, s in zip(grids, solvers))
 
@@ -571,6 +576,9 @@ def main():
     ap.add_argument("--single-gradient", action="store_true",
                     help="one 2D gradient call on the gradient workload (600^2 disk, 16 TX, 800 steps, "
                          "checkpoints every 50), no output (for rocprofv3 --pmc); --tuning applies")
+    ap.add_argument("--single-uneven", action="store_true",
+                    help="one uneven batch of the 64 heads with the tuner's usual shape (K 6, 32 "
+                         "rows, 4 TX per window, four wavefronts), no output (for rocprofv3 --pmc)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of 256 MiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -586,6 +594,9 @@ def main():
         return
     if args.single_gradient_3d:
         _bench_3d_gradient(T3=100, every=20, single=True)
+        return
+    if args.single_uneven:
+        _bench_uneven(autotune=False, single=True)
         return
     if args.single_gradient:
         from ceep_amd import adjoint
@@ -625,7 +636,9 @@ def main():
     cells = int(box[0] * box[1])
     K = (tuning or {}).get("temporal_depth", 0)
     if K and (tuning or {}).get("forward_waves", 1) == 4:
-        kernel = f"step2d_mw_kernel<{tuning['rows_per_wave'] // 4}, {K}, 1>"
+        # the head phantom is piecewise constant: material ids unless switched off
+        ids = "true" if (tuning or {}).get("material_ids", 1) else "false"
+        kernel = f"step2d_mw_kernel<{tuning['rows_per_wave'] // 4}, {K}, 1, {ids}>"
         launches = -(-T // K)
         steps_per_launch = K
     elif K:

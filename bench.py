@@ -337,7 +337,7 @@ def _bench_3d(steps_timed=2, T3=1000, tuning=None):
     # and J one step older) read and written once per cell and TX, the 7 maps
     # once per cell; a K-step launch is credited K steps
     bytes_launch = cells * (16 * BYTES3D_PER_CELL_PER_TX + BYTES3D_SHARED) * k3
-    kernel = f"step3d_tb_kernel<{k3}, 3, false>" if form == 4 else "h3d_kernel + e3d_kernel"
+    kernel = f"step3d_tb_kernel<{k3}, 3, false, false>" if form == 4 else "h3d_kernel + e3d_kernel"
     return {"value": round(cells * 16 * T3 / (ms * 1e-3) / 1e9, 3), "unit": "GCell-steps/s",
             "ms_per_run": round(ms, 2), "cells_counted": f"stepped box {box[0]}x{box[1]}x{box[2]}",
             "value_full_grid_convention": round(n3 ** 3 * 16 * T3 / (ms * 1e-3) / 1e9, 3),
@@ -434,13 +434,14 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False):
                          "run one after another with the one-step kernel")}
 
 
-def _bench_3d_gradient(T3=1000, every=100, single=False):
+def _bench_3d_gradient(T3=1000, every=100, single=False, tuning=None):
     """dL/d eps_r and dL/d sigma over the whole 200^3 volume of the 3D
     workload (Debye + Lorentz head seed 0, 16 TX, 1000 steps), loss = sum
     |S - S_target|^2 over the 101-frequency S-matrix against head seed 1,
     checkpoints every 100 steps with recomputation.  Ratio = gradient wall
     time over the forward's (reset, run, DFT).  With `single`: one gradient
-    call and nothing timed (for rocprofv3 --pmc)."""
+    call and nothing timed (for rocprofv3 --pmc).  `tuning`: launch settings
+    on top of the defaults, for both solvers."""
     import ceep_amd
     from ceep_amd import adjoint
     from ceep_amd.constants import cfl_timestep_3d
@@ -458,6 +459,8 @@ def _bench_3d_gradient(T3=1000, every=100, single=False):
                                    cpml_thickness=10, source_positions=pos, probe_positions=pos,
                                    frequency=2e9, waveform=w)
         s.set_phantom(ceep_amd.phantoms.DebyeHeadPhantom3D((n3, n3, n3), seed))
+        if tuning is not None:
+            s.tuning = tuning
         return s
 
     tgt = solver(1)
@@ -567,7 +570,8 @@ def main():
     ap.add_argument("--single-run", action="store_true",
                     help="one simulation + DFT with the default tuning, no output (for rocprofv3 --pmc)")
     ap.add_argument("--tuning", default=None,
-                    help="with --single-run: JSON dict of launch settings applied on top of the defaults")
+                    help="with --single-run, --single-gradient, --single-run-3d, --single-gradient-3d: "
+                         "JSON dict of launch settings applied on top of the defaults")
     ap.add_argument("--single-run-3d", action="store_true",
                     help="one 3D simulation (200^3, Debye + Lorentz, 100 steps), no output (for rocprofv3 --pmc)")
     ap.add_argument("--single-gradient-3d", action="store_true",
@@ -589,11 +593,12 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         return
+    tun = json.loads(args.tuning) if args.tuning else None
     if args.single_run_3d:
-        _bench_3d(steps_timed=1, T3=100)
+        _bench_3d(steps_timed=1, T3=100, tuning=tun)
         return
     if args.single_gradient_3d:
-        _bench_3d_gradient(T3=100, every=20, single=True)
+        _bench_3d_gradient(T3=100, every=20, single=True, tuning=tun)
         return
     if args.single_uneven:
         _bench_uneven(autotune=False, single=True)

@@ -56,13 +56,36 @@ struct EGSlab {
 struct AMaps {
   float ca, cb, dk1, dk2, lk1, lk2, lk3, da, dbi;
   int ph;
+  int id;   // material-id form: the cell's table row (the map values come from LDS at use)
 };
+
+// The maps of table row m.id (the rest of m kept) from the LDS copy of the
+// material table, two float4 per row in the packed maps' order (map_pack)
+template <int POLES>
+__device__ __forceinline__ AMaps resolve_amaps(const float4* mtab, AMaps m) {
+  constexpr MapPack MPK = map_pack(POLES, false);
+  const float4 t0 = mtab[2 * m.id], t1 = mtab[2 * m.id + 1];
+  const float v[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+  m.ca = v[0];
+  m.cb = v[1];
+  if (POLES & 1) {
+    m.dk1 = v[MPK.dk];
+    m.dk2 = v[MPK.dk + 1];
+  }
+  if (POLES & 2) {
+    m.lk1 = v[MPK.lk];
+    m.lk2 = v[MPK.lk + 1];
+    m.lk3 = v[MPK.lk + 2];
+  }
+  return m;
+}
 
 }  // namespace
 
-template <int K, int POLES, bool MAG>
+template <int K, int POLES, bool MAG, bool MID>
 __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
 #pragma clang fp contract(off)
+  static_assert(!(MID && MAG), "the material-id form has no mu_r / sigma_m maps");
   constexpr int R = kRowsA;
   constexpr int NJ = ((POLES & 1) ? 3 : 0) + ((POLES & 2) ? 6 : 0);
   constexpr int JL = (POLES & 1) ? 3 : 0;
@@ -73,6 +96,7 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   // later (two buffers by slab parity; each thread reads only its own slots)
   constexpr int NP = K == 2 ? NJ + 3 : 1;
   __shared__ float park[K == 2 ? 2 : 1][NP][1024];
+  __shared__ float4 mtab[MID ? 2 * kMaxMat3 : 1];   // material table (MID)
   const int tid = threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -130,6 +154,10 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
       plane_rsrc(NJ ? p.jp + ((long long)(p.in_set ^ 1) * p.B + bt) * p.bstride * NJ : p.mp,
                  jbytes);
   const __amdgpu_buffer_rsrc_t rm = plane_rsrc(p.mp, (unsigned)(p.bstride * 4 * MPK.n));
+  // MID: one byte per cell; lane offset yz, slab offset a * sx
+  const __amdgpu_buffer_rsrc_t rid = plane_rsrc(
+      MID ? reinterpret_cast<const float*>(p.mat_id) : p.mp, (unsigned)p.bstride);
+  const unsigned yzi = yz_in ? yz : kOffNone;
   // E-store slots of steps S - K + 1 .. S + 1, one resource each (component
   // c at + c * nbox; the host keeps (2 nbox + bstride) * 4 below 2^31)
   __amdgpu_buffer_rsrc_t re[K + 1];
@@ -145,6 +173,11 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   const __amdgpu_buffer_rsrc_t rp = plane_rsrc(
       p.probe_head ? reinterpret_cast<const float*>(p.probe_head) : p.mp, (unsigned)(p.bstride * 2));
   const long long sx = p.sx;
+  if (MID) {   // the table, once per workgroup (read after the barrier below)
+    const float4* mt = reinterpret_cast<const float4*>(p.mat_table);
+    for (int t = tid; t < 2 * p.n_mat; t += 1024) mtab[t] = mt[t];
+    lds_barrier();
+  }
 
   const long long sc = p.src_cell[bt];
   const int src_a = (int)(sc / sx);
@@ -172,6 +205,16 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   };
   auto load_maps = [&](AMaps& m, int a) __attribute__((always_inline)) {
     const bool ok = slab_ok(a);
+    if (MID) {   // out-of-range slabs and cells read id 0, the all-zero row
+      m.id = (int)__builtin_amdgcn_raw_buffer_load_b8(rid, (int)(ok ? yzi : kOffNone),
+                                                      (int)((unsigned)max(a, 0) * (unsigned)sx), 0);
+      m.dk1 = m.dk2 = m.lk1 = m.lk2 = m.lk3 = 0.f;
+      m.da = 1.f;
+      m.dbi = p.ch;
+      m.ph = probes ? rload_short(rp, ok ? yzp : kOffNone, (unsigned)max(a, 0) * (unsigned)(sx * 2))
+                    : -1;
+      return;
+    }
     const unsigned v = ok ? yzm : kOffNone;
     const unsigned so = (unsigned)max(a, 0) * sxm_b;
     m.dk1 = m.dk2 = m.lk1 = m.lk2 = m.lk3 = 0.f;
@@ -324,8 +367,14 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
     load_eg(eg[0], 0);
     load_eg(eg[1], 1);
   }
-  step_a(ring[0], mring[0], 0, 0);
-  if (K == 1) finish_a(ring[0], mring[0], 0, eg[0]);
+  auto maps_of = [&](const AMaps& m) __attribute__((always_inline)) {
+    return MID ? resolve_amaps<POLES>(mtab, m) : m;
+  };
+  {
+    const AMaps m0 = maps_of(mring[0]);
+    step_a(ring[0], m0, 0, 0);
+    if (K == 1) finish_a(ring[0], m0, 0, eg[0]);
+  }
 
   auto iteration = [&](auto phase, int i) __attribute__((always_inline)) {
     constexpr int U = decltype(phase)::value;   // i % 4: slab x lives in ring[x % 4]
@@ -336,9 +385,10 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
       const int xb = i - k;             // B's and C's slab
       ASlab<POLES, K>& sA = ring[(U + 5 - k) % 4];
       ASlab<POLES, K>& sB = ring[(U + 4 - k) % 4];
-      step_a(sA, mring[(U + 5 - k) % 4], xa, k);
+      const AMaps mA = maps_of(mring[(U + 5 - k) % 4]);
+      step_a(sA, mA, xa, k);
       if (k == K - 1) {
-        finish_a(sA, mring[(U + 5 - k) % 4], xa, eg[(U + 1 - k) & 1]);
+        finish_a(sA, mA, xa, eg[(U + 1 - k) & 1]);
         // the next slab's prefetch, issued after every load the rest of
         // this iteration waits for
         load_slab(ring[(U + 2) % 4], i + 2);
@@ -428,10 +478,13 @@ hipError_t launch_adj3d_tb(const AdjTB3DArgs& a, int K, int poles, hipStream_t s
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
   const dim3 grd((unsigned)nblk, 1, 1), blk(1024, 1, 1);
   const bool mag = a.magnetic != 0;
-#define CEEP_ATB(KK, PP)                                                                 \
-  do {                                                                                   \
-    if (mag) hipLaunchKernelGGL((adj3d_tb_kernel<KK, PP, true>), grd, blk, 0, s, a);     \
-    else hipLaunchKernelGGL((adj3d_tb_kernel<KK, PP, false>), grd, blk, 0, s, a);        \
+  const bool mid = a.mat_id != nullptr;
+  if (mid && (mag || !a.mat_table || a.n_mat < 1 || a.n_mat > kMaxMat3)) return hipErrorInvalidValue;
+#define CEEP_ATB(KK, PP)                                                                    \
+  do {                                                                                      \
+    if (mag) hipLaunchKernelGGL((adj3d_tb_kernel<KK, PP, true, false>), grd, blk, 0, s, a); \
+    else if (mid) hipLaunchKernelGGL((adj3d_tb_kernel<KK, PP, false, true>), grd, blk, 0, s, a); \
+    else hipLaunchKernelGGL((adj3d_tb_kernel<KK, PP, false, false>), grd, blk, 0, s, a);    \
   } while (0)
 #define CEEP_ATBK(PP)            \
   do {                           \

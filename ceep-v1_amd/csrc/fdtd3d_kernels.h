@@ -176,6 +176,13 @@ struct TB3DArgs {
   int order;                      // workgroup order: 0 dispatch (y tiles fastest), 1 per XCD, 2 per XCD TX
                                   // fastest, 3 per XCD in groups of tx_group TX, then y tiles
   int tx_group;
+  // piecewise-constant media without mu_r / sigma_m maps: one byte per cell
+  // (box layout, no batch axis) indexing a table of kMat3Floats floats per
+  // material, in the order of a packed record (ca, cb, Debye k1, k2/dt,
+  // Lorentz k1, k2/dt, k3, absent groups taking no room); null: packed maps
+  const unsigned char* mat_id;
+  const float* mat_table;
+  int n_mat;
 };
 
 hipError_t launch_step3d_tb(const TB3DArgs& a, int K, int poles, hipStream_t s);
@@ -186,6 +193,10 @@ hipError_t launch_step3d_tb(const TB3DArgs& a, int K, int poles, hipStream_t s);
 struct MapPack {
   int dk, lk, mag, n;
 };
+// Material-id form: at most kMaxMat3 materials of kMat3Floats table floats
+// (two float4 per material; a record of at most 7 maps, the rest padding)
+constexpr int kMaxMat3 = 256;
+constexpr int kMat3Floats = 8;
 constexpr MapPack map_pack(int poles, bool magnetic) {
   return MapPack{2, 2 + ((poles & 1) ? 2 : 0), 2 + ((poles & 1) ? 2 : 0) + ((poles & 2) ? 3 : 0),
                  2 + ((poles & 1) ? 2 : 0) + ((poles & 2) ? 3 : 0) + (magnetic ? 2 : 0)};
@@ -220,6 +231,9 @@ struct AdjTB3DArgs {
   int tiles_y, tiles_z;           // ceil((My-1)/(16-2K)), ceil((Mz-1)/(64-2K))
   int order;                      // 0: dispatch order (y tiles fastest), 1: the same with
                                   // contiguous ranges per XCD, 2: per XCD with the TX fastest
+  const unsigned char* mat_id;    // material ids and table as TB3DArgs (null: packed maps)
+  const float* mat_table;
+  int n_mat;
 };
 
 hipError_t launch_adj3d_tb(const AdjTB3DArgs& a, int K, int poles, hipStream_t s);

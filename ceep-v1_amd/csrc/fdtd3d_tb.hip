@@ -43,18 +43,55 @@ struct Cell3 {
 struct Maps3 {
   float ca, cb, dk1, dk2, lk1, lk2, lk3, da, dbi;
   int ph;
+  int id;   // material-id form: the cell's table row (the map values come from LDS at use)
 };
+
+// Float j of a material's two-float4 table row
+__device__ __forceinline__ float tab_float(const float4& t0, const float4& t1, int j) {
+  switch (j) {
+    case 0: return t0.x;
+    case 1: return t0.y;
+    case 2: return t0.z;
+    case 3: return t0.w;
+    case 4: return t1.x;
+    case 5: return t1.y;
+    case 6: return t1.z;
+    default: return t1.w;
+  }
+}
+
+// The maps of table row m.id (the rest of m kept), read from the LDS copy of
+// the material table; the record order is the packed maps' (map_pack)
+template <int POLES>
+__device__ __forceinline__ Maps3 resolve_maps(const float4* mtab, Maps3 m) {
+  constexpr MapPack MPK = map_pack(POLES, false);
+  const float4 t0 = mtab[2 * m.id], t1 = mtab[2 * m.id + 1];
+  m.ca = t0.x;
+  m.cb = t0.y;
+  if (POLES & 1) {
+    m.dk1 = tab_float(t0, t1, MPK.dk);
+    m.dk2 = tab_float(t0, t1, MPK.dk + 1);
+  }
+  if (POLES & 2) {
+    m.lk1 = tab_float(t0, t1, MPK.lk);
+    m.lk2 = tab_float(t0, t1, MPK.lk + 1);
+    m.lk3 = tab_float(t0, t1, MPK.lk + 2);
+  }
+  return m;
+}
 
 }  // namespace
 
-template <int K, int POLES, bool MAG>
+template <int K, int POLES, bool MAG, bool MID>
 __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
 #pragma clang fp contract(off)
+  static_assert(!(MID && MAG), "the material-id form has no mu_r / sigma_m maps");
   constexpr int R = kRows;
   constexpr int NJ = ((POLES & 1) ? 3 : 0) + ((POLES & 2) ? 6 : 0);
   constexpr int JL = (POLES & 1) ? 3 : 0;   // packed index of Lorentz J x
   __shared__ float xe[K][2][R][64];   // E (x, z) each level starts from: +y neighbours
   __shared__ float xh[K][2][R][64];   // H (z, x) after each level's H update: -y neighbours
+  __shared__ float4 mtab[MID ? 2 * kMaxMat3 : 1];   // material table (MID)
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int L = xcd_block(p.order != 0);
@@ -117,10 +154,19 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
       plane_rsrc(NJ ? p.jp + ((long long)(p.in_set ^ 1) * p.B + bt) * p.bstride * NJ : p.mp,
                  jbytes);
   const __amdgpu_buffer_rsrc_t rm = plane_rsrc(p.mp, (unsigned)(p.bstride * 4 * MPK.n));
+  // MID: one byte per cell; lane offset yz, slab offset a * sx
+  const __amdgpu_buffer_rsrc_t rid = plane_rsrc(
+      MID ? reinterpret_cast<const float*>(p.mat_id) : p.mp, (unsigned)p.bstride);
+  const unsigned yzi = yz_in ? yz : kOffNone;
   const bool probes = p.probe_head != nullptr;
   const __amdgpu_buffer_rsrc_t rp = plane_rsrc(
       probes ? reinterpret_cast<const float*>(p.probe_head) : p.mp, (unsigned)(p.bstride * 2));
   const long long sx = p.sx;
+  if (MID) {   // the table, once per workgroup (read after the barrier below)
+    const float4* mt = reinterpret_cast<const float4*>(p.mat_table);
+    for (int t = threadIdx.x; t < 2 * p.n_mat; t += 1024) mtab[t] = mt[t];
+    lds_barrier();
+  }
 
   const long long sc = p.src_cell[bt];
   const int src_a = (int)(sc / sx);
@@ -157,6 +203,13 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
   };
   auto load_maps = [&](Maps3& m, int a) {
     const bool ok = slab_ok(a);
+    if (MID) {   // out-of-range slabs and cells read id 0, the all-zero row
+      m.id = (int)__builtin_amdgcn_raw_buffer_load_b8(rid, (int)(ok ? yzi : kOffNone),
+                                                      (int)((unsigned)max(a, 0) * (unsigned)sx), 0);
+      m.ph = probes ? rload_short(rp, ok ? yzp : kOffNone, (unsigned)max(a, 0) * (unsigned)(p.sx * 2))
+                    : -1;
+      return;
+    }
     const unsigned v = ok ? yzm : kOffNone;
     const unsigned so = (unsigned)max(a, 0) * sxm_b;
     m.ca = rload(rm, v, so);
@@ -222,7 +275,8 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
       const int a = i - k + 1;
       Cell3<POLES>& c = ring[(U + 5 - k) % 4];
       const Cell3<POLES>& cn = ring[(U + 6 - k) % 4];   // slab a + 1, at level k - 1
-      const Maps3& m = mring[(U + 5 - k) % 4];
+      const Maps3 m = MID ? resolve_maps<POLES>(mtab, mring[(U + 5 - k) % 4])
+                          : mring[(U + 5 - k) % 4];
       const bool xa = a < p.Mx - 1;
       // H (h3d_kernel): forward differences
       {
@@ -336,10 +390,13 @@ hipError_t launch_step3d_tb(const TB3DArgs& a, int K, int poles, hipStream_t s) 
   const dim3 grd((unsigned)nblk, 1, 1), blk(1024, 1, 1);
   if (!a.mp || (poles != 0 && !a.jp)) return hipErrorInvalidValue;
   const bool mag = a.magnetic != 0;
-#define CEEP_TB3(KK, PP)                                                                  \
-  do {                                                                                    \
-    if (mag) hipLaunchKernelGGL((step3d_tb_kernel<KK, PP, true>), grd, blk, 0, s, a);     \
-    else hipLaunchKernelGGL((step3d_tb_kernel<KK, PP, false>), grd, blk, 0, s, a);        \
+  const bool mid = a.mat_id != nullptr;
+  if (mid && (mag || !a.mat_table || a.n_mat < 1 || a.n_mat > kMaxMat3)) return hipErrorInvalidValue;
+#define CEEP_TB3(KK, PP)                                                                     \
+  do {                                                                                       \
+    if (mag) hipLaunchKernelGGL((step3d_tb_kernel<KK, PP, true, false>), grd, blk, 0, s, a); \
+    else if (mid) hipLaunchKernelGGL((step3d_tb_kernel<KK, PP, false, true>), grd, blk, 0, s, a); \
+    else hipLaunchKernelGGL((step3d_tb_kernel<KK, PP, false, false>), grd, blk, 0, s, a);    \
   } while (0)
 #define CEEP_TB3K(PP)            \
   do {                           \

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -216,7 +217,7 @@ struct ceep_fdtd {
   unsigned char* mat_id = nullptr;
   float4* mat_table = nullptr;
   int n_mat = 0;               // 0: no id map (too many distinct cells, or switched off)
-  int mat_ids_on = 1;          // tuning key 22
+  int mat_ids_on = 1;          // tuning key 22: 0 off, 1 2D kernels + 3D adjoint, 2 also the 3D forward
 
   bool mats_dirty = true;
   bool has_run = false;
@@ -345,57 +346,67 @@ int upload_f32(float* dst, const std::vector<float>& v, hipStream_t s) {
 
 // FP64 coefficient maps (fdtd_2d_batched.py:152-165, fdtd_3d_batched.py:282-296,
 // dispersive.py:253-299, core/grid.py:178-184) over the box, stored as FP32.
-// Material ids of piecewise-constant 2D media (host-formed maps): the
-// distinct (ca, cb, k1, k2dt) tuples of the box, id 0 the all-zero tuple of
-// cells off the E interior; none when there are more than kMaxMaterials, the
-// maps were formed on the device, or the feature is off.  The tables hold
-// the exact FP32 map values, so results are bit-identical to the maps'.
+// Material ids of piecewise-constant media: the distinct map records of the
+// box, id 0 the all-zero record of cells off the E interior; none when there
+// are more than the kernels' table holds, or the feature is off.  2D (host-
+// formed maps, no Lorentz pole): records (ca, cb, k1, k2dt), one float4 per
+// material.  3D blocked form without mu_r / sigma_m maps: the packed record
+// (ca, cb, Debye k1, k2dt, Lorentz k1, k2dt, k3; absent poles take no room)
+// padded to kMat3Floats.  The tables hold the exact FP32 map values, so
+// results are bit-identical to the maps'.
 int build_material_ids(ceep_fdtd* h, const std::vector<float>& ca, const std::vector<float>& cb,
-                       const std::vector<float>& k1, const std::vector<float>& k2) {
+                       const std::vector<float>& k1, const std::vector<float>& k2,
+                       const std::vector<float>& l1, const std::vector<float>& l2,
+                       const std::vector<float>& l3, bool magnetic) {
   dfree(h->mat_id);
   dfree(h->mat_table);
   h->n_mat = 0;
-  if (h->ndim != 2 || h->cpml() || h->has_lorentz || !h->mat_ids_on || ca.empty() ||
-      h->plane >= (1LL << 29))
-    return CEEP_OK;
-  struct Key {
-    uint32_t v[4];
-    bool operator==(const Key& o) const {
-      return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2] && v[3] == o.v[3];
-    }
-  };
+  if (!h->mat_ids_on || ca.empty() || h->cpml() || h->plane >= (1LL << 29)) return CEEP_OK;
+  const bool d2 = h->ndim == 2 && !h->has_lorentz;
+  const bool d3 = h->tb3d() && !magnetic;
+  if (!d2 && !d3) return CEEP_OK;
+  const int stride = d2 ? 4 : ceep::kMat3Floats;
+  const int max_mat = d2 ? ceep::kMaxMaterials : ceep::kMaxMat3;
+  using Key = std::array<uint32_t, 8>;
   struct Hash {
     size_t operator()(const Key& k) const {
       uint64_t x = 1469598103934665603ULL;
-      for (uint32_t u : k.v) x = (x ^ u) * 1099511628211ULL;
+      for (uint32_t u : k) x = (x ^ u) * 1099511628211ULL;
       return (size_t)x;
     }
   };
   auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
   std::unordered_map<Key, int, Hash> ids;
-  std::vector<float4> table;
-  ids.emplace(Key{{0u, 0u, 0u, 0u}}, 0);
-  table.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+  std::vector<float> table((size_t)stride, 0.f);
+  ids.emplace(Key{}, 0);
   std::vector<unsigned char> id((size_t)h->plane, 0);
-  const bool pole = !k1.empty();
+  std::vector<const std::vector<float>*> cols = {&ca, &cb};
+  for (const auto* c : {&k1, &k2, &l1, &l2, &l3})
+    if (!c->empty()) cols.push_back(c);
+  float rec[8];
   for (long long o = 0; o < h->plane; ++o) {
-    const float a = ca[o], b = cb[o], c = pole ? k1[o] : 0.f, d = pole ? k2[o] : 0.f;
-    const Key k{{bits(a), bits(b), bits(c), bits(d)}};
+    Key k{};
+    for (size_t j = 0; j < cols.size(); ++j) {
+      rec[j] = (*cols[j])[o];
+      k[j] = bits(rec[j]);
+    }
     auto it = ids.find(k);
     if (it == ids.end()) {
-      if ((int)table.size() == ceep::kMaxMaterials) return CEEP_OK;   // not piecewise constant
-      it = ids.emplace(k, (int)table.size()).first;
-      table.push_back(make_float4(a, b, c, d));
+      if ((int)ids.size() == max_mat) return CEEP_OK;   // not piecewise constant
+      it = ids.emplace(k, (int)ids.size()).first;
+      table.resize(table.size() + stride, 0.f);
+      std::memcpy(&table[table.size() - stride], rec, cols.size() * sizeof(float));
     }
     id[(size_t)o] = (unsigned char)it->second;
   }
   int rc;
-  if ((rc = dalloc(&h->mat_id, id.size())) || (rc = dalloc(&h->mat_table, table.size()))) return rc;
+  if ((rc = dalloc(&h->mat_id, id.size())) || (rc = dalloc(&h->mat_table, table.size() / 4)))
+    return rc;
   CEEP_HIP(hipMemcpyAsync(h->mat_id, id.data(), id.size(), hipMemcpyHostToDevice, h->stream));
-  CEEP_HIP(hipMemcpyAsync(h->mat_table, table.data(), table.size() * sizeof(float4),
+  CEEP_HIP(hipMemcpyAsync(h->mat_table, table.data(), table.size() * sizeof(float),
                           hipMemcpyHostToDevice, h->stream));
   CEEP_HIP(hipStreamSynchronize(h->stream));
-  h->n_mat = (int)table.size();
+  h->n_mat = (int)ids.size();
   return CEEP_OK;
 }
 
@@ -498,7 +509,7 @@ int commit_materials(ceep_fdtd* h) {
   } else if ((rc = upload_f32(h->ca, ca, s)) || (rc = upload_f32(h->cb, cb, s))) {
     return rc;
   }
-  if ((rc = build_material_ids(h, ca, cb, dk1, dk2))) return rc;
+  if ((rc = build_material_ids(h, ca, cb, dk1, dk2, lk1, lk2, lk3, !da.empty()))) return rc;
   dfree(h->da); dfree(h->dbi); dfree(h->dk1); dfree(h->dk2);
   dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
   if (!da.empty()) {
@@ -752,6 +763,10 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     t.jp = h->jpack;
     t.mp = h->mpack;
     t.magnetic = h->da != nullptr;
+    if (h->n_mat > 0 && h->mat_ids_on == 2) {   // measured slower than the packed maps: opt-in
+      t.mat_id = h->mat_id; t.mat_table = reinterpret_cast<const float*>(h->mat_table);
+      t.n_mat = h->n_mat;
+    }
     t.ch = h->ch; t.idx = h->idx;
     t.src_cell = h->src3; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
     t.T = h->T; t.P = h->P;
@@ -1715,6 +1730,10 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
   t.jp = nj ? h->adjtb_j : nullptr;
   t.mp = h->mpack;
   t.magnetic = h->da != nullptr;
+  if (h->n_mat > 0) {
+    t.mat_id = h->mat_id; t.mat_table = reinterpret_cast<const float*>(h->mat_table);
+    t.n_mat = h->n_mat;
+  }
   t.ch = h->ch; t.idx = h->idx;
   t.src_cell = h->src3; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
   t.T = T; t.P = P; t.dtr = h->d_dtr;
@@ -2192,7 +2211,7 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
       return fail(CEEP_EINVAL, "3D adjoint steps per launch must be 0 (three-kernel form), 1 or 2");
     h->adj3d_k = value;
   } else if (key == 22) {
-    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "material ids must be 0 or 1");
+    if (value < 0 || value > 2) return fail(CEEP_EINVAL, "material ids must be 0, 1 or 2");
     if (h->mat_ids_on != value) { h->mat_ids_on = value; h->mats_dirty = true; }
   } else if (key == 20) {
     if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D adjoint block order must be 0, 1 or 2");

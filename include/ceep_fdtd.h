@@ -245,7 +245,14 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
  *   key 20: 3D blocked adjoint: workgroup order, 0 = dispatch order (y tiles
  *          fastest), 1 = the same renumbered so each XCD runs a contiguous
  *          range (default; y-neighbour tiles share halo rows in an L2),
- *          2 = per XCD with the TX fastest.  Results are identical. */
+ *          2 = per XCD with the TX fastest.  Results are identical.
+ *   key 22: material ids for piecewise-constant media (at most 256 distinct
+ *          coefficient records in the box): one byte per cell plus a table
+ *          instead of the per-cell coefficient maps.  0 = off, 1 (default) =
+ *          the 2D blocked forward, residual and uneven-round kernels and the
+ *          3D blocked adjoint, 2 = also the 3D blocked forward.  Media with
+ *          more records, Lorentz poles in 2D, mu_r / sigma_m maps in 3D and
+ *          the physical-CPML mode use the maps.  Results are bit-identical. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,

@@ -258,3 +258,59 @@ def test_blocked_3d_workgroup_orders_agree(amd, tuning):
     assert np.abs(out[0][0]).max() > 0
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kinds", [(), ("debye",), ("lorentz",), ("debye", "lorentz")],
+                         ids=["plain", "debye", "lorentz", "debye+lorentz"])
+def test_blocked_3d_material_ids_bit_identical(amd, kinds):
+    """Piecewise-constant media read as one byte per cell plus a material
+    table in LDS (the blocked adjoint by default, the blocked forward too
+    with key 22 = 2) give the same bits as the packed per-cell maps: traces,
+    fields, energy and both gradients."""
+    shape, n, T, dx = (26, 44, 70), 3, 41, 1e-3
+    dt = W.dt_3d_courant(dx, 0.5)
+    w = W.modulated_gaussian(T, dt, 20e9, 30e9, delay_factor=3.0)
+    eps, sig, m = _maps(shape, 11)
+    src = [(13, 22, 35), (10, 18, 30)]
+    prb = src + [(12, 30, 20), (n, 22, 35)]
+    out = {}
+    for ids in (2, 0):
+        s = _solver(amd, shape, n, T, dx, src, prb, w, eps, sig, m, kinds)
+        s.tuning = {"material_ids": ids}
+        print(f"3D material ids: {kinds} ids={ids}", flush=True)
+        s.run()
+        out[ids] = (s.probe_data.copy(), s.ez, s.hx, s.compute_energy())
+    assert np.abs(out[2][0]).max() > 0
+    for a, b in zip(out[2], out[0]):
+        assert np.array_equal(a, b)
+    g2, g1, g0 = _gradients(amd, (26, 30, 70), 3, 39, 20, kinds, False,
+                            [{"material_ids": 2}, {"material_ids": 1}, {"material_ids": 0}], seed=4)
+    for g in (g2, g1):
+        assert g[0] == g0[0]
+        assert np.array_equal(g[1], g0[1]) and np.array_equal(g[2], g0[2])
+    assert np.abs(g1[1]).max() > 0
+
+
+def test_blocked_3d_many_materials_use_the_maps(amd):
+    """A medium with more distinct cells than the material table holds (a
+    random eps_r per cell) runs on the packed maps: the result does not
+    depend on the material-id setting and matches the one-step kernels."""
+    shape, n, T, dx = (24, 22, 40), 3, 31, 1e-3
+    dt = W.dt_3d_courant(dx, 0.5)
+    w = W.modulated_gaussian(T, dt, 20e9, 30e9, delay_factor=3.0)
+    rng = np.random.default_rng(12)
+    eps = rng.uniform(1.0, 10.0, shape)
+    sig = np.zeros(shape)
+    m = np.zeros(shape, bool)
+    src = [(12, 11, 20)]
+    prb = src + [(10, 9, 25)]
+    out = []
+    for tun in ({"material_ids": 2}, {"material_ids": 0}, {"fused_3d": 0}):
+        s = _solver(amd, shape, n, T, dx, src, prb, w, eps, sig, m, ())
+        s.tuning = tun
+        s.run()
+        out.append((s.probe_data.copy(), s.ez))
+    assert np.abs(out[0][0]).max() > 0
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)

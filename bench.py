@@ -347,17 +347,10 @@ def _bench_3d(steps_timed=2, T3=1000, tuning=None):
                          f"1000 steps, Debye+Lorentz head seed 0 ({kernel}, {k3} steps per launch)")}
 
 
-def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False):
-    """64 seeded Debye heads (the head generator above scaled to each grid),
-    grids drawn from {256, 384, 512, 600}^2, steps U[600, 1200], 16 TX = RX on a
-    ring of radius 200/600 of the grid, CPML 12-15 cells, each followed by its
-    301-frequency DFT.  The batch runs as one captured graph of per-round
-    launches (launch shape from the coordinate autotuner); the comparison runs
-    the 64 problems one after another with the one-step kernel."""
+def _uneven_solvers(n_problems=64, seed=4):
+    """The uneven batch's 64 seeded Debye heads (built solvers) and their grid sizes."""
     import ceep_amd
-    from ceep_amd.autotune import autotune_uneven
     from ceep_amd.constants import cfl_timestep_2d
-    from ceep_amd.uneven import UnevenBatch
     from ceep_amd.waveforms import modulated_gaussian
 
     rng = np.random.default_rng(seed)
@@ -376,20 +369,40 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False):
         s._build()
         solvers.append(s)
         grids.append(g)
+    return solvers, grids
+
+
+def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False, persistent=False):
+    """64 seeded Debye heads (the head generator above scaled to each grid),
+    grids drawn from {256, 384, 512, 600}^2, steps U[600, 1200], 16 TX = RX on a
+    ring of radius 200/600 of the grid, CPML 12-15 cells, each followed by its
+    301-frequency DFT.  The batch runs as one captured graph, timed in both
+    forms -- one persistent launch (four-wavefront shapes) and one launch per
+    round -- with the launch shape from the coordinate autotuner; `value` is
+    the faster.  The comparison runs the 64 problems one after another with
+    the one-step kernel.  `single`: one batch in the persistent form (or per
+    round with `persistent` False), nothing timed (for rocprofv3 --pmc)."""
+    from ceep_amd.autotune import autotune_uneven
+    from ceep_amd.uneven import UnevenBatch
+
+    solvers, grids = _uneven_solvers(n_problems, seed)
     shape, log = (6, 32, 4, 4), []    # K, window rows, TX planes per window, wavefronts (the tuner's pick)
     if autotune:
         shape, log = autotune_uneven(solvers)
     ub = UnevenBatch(solvers, *shape)
     if single:                       # one batch, nothing timed (for rocprofv3 --pmc)
-        ub._multi.run(UnevenBatch.PER_ROUND)
+        ub._multi.run(UnevenBatch.PERSISTENT if persistent else UnevenBatch.PER_ROUND)
         _device_sync()
         ub.close()
         return None
     cell_steps = ub.cell_steps
     full_cells = sum(g * g * B * s.total_steps for g, s in zip(grids, solvers))
+    modes = {"per_round": UnevenBatch.PER_ROUND}
+    if shape[3] == 4:
+        modes["persistent"] = UnevenBatch.PERSISTENT
 
-    def batched():
-        ub._multi.run(UnevenBatch.PER_ROUND)
+    def batched(mode):
+        ub._multi.run(mode)
         for s in solvers:
             s._handle.dft(FREQS)
 
@@ -399,13 +412,17 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False):
             s._handle.run(copy=False)
             s._handle.dft(FREQS)
 
-    batched()
-    _device_sync()
-    t0 = time.perf_counter()
-    batched()
-    _device_sync()
-    t_b = time.perf_counter() - t0
-    ms_graph = ub.time(2)
+    t_mode, graph_mode = {}, {}
+    for name, mode in modes.items():
+        batched(mode)
+        _device_sync()
+        t0 = time.perf_counter()
+        batched(mode)
+        _device_sync()
+        t_mode[name] = time.perf_counter() - t0
+        graph_mode[name] = ub.time(2, mode)
+    best = min(t_mode, key=t_mode.get)
+    t_b, ms_graph = t_mode[best], graph_mode[best]
     info = ub.info()
     ub.close()
     for s in solvers:
@@ -418,6 +435,9 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False):
     t_s = time.perf_counter() - t0
     return {"value": round(cell_steps / t_b / 1e9, 3), "unit": "GCell-steps/s",
             "ms_per_batch": round(t_b * 1e3, 2), "ms_graph_only": round(ms_graph, 2),
+            "form": best,
+            "ms_per_batch_by_form": {k: round(v * 1e3, 2) for k, v in t_mode.items()},
+            "ms_graph_only_by_form": {k: round(v, 2) for k, v in graph_mode.items()},
             "value_full_grid_convention": round(full_cells / t_b / 1e9, 3),
             "sequential_one_step_value": round(cell_steps / t_s / 1e9, 3),
             "sequential_one_step_ms": round(t_s * 1e3, 2),
@@ -426,12 +446,14 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False):
                              "batch_per_wave": shape[2], "wavefronts_per_window": shape[3]},
             "autotune_log_ms": [[round(ms, 2), list(sh)] for ms, sh in sorted(log)],
             "work_items": info["items"], "segments": info["segments"],
+            "persistent_items": info["persistent_items"],
+            "persistent_workgroups": info["persistent_workgroups"],
             "cell_steps": cell_steps,
             "grids": {str(g): grids.count(g) for g in sorted(set(grids))},
             "workload": (f"{n_problems} seeded Debye heads, grids from 256/384/512/600 squared, "
                          "steps U[600,1200], 16 TX/RX, CPML 12-15, 301-frequency DFT each; "
-                         "one hipGraph of per-round launches over all problems vs the problems "
-                         "run one after another with the one-step kernel")}
+                         "one hipGraph (one persistent launch, or one launch per round) over all "
+                         "problems vs the problems run one after another with the one-step kernel")}
 
 
 def _bench_3d_gradient(T3=1000, every=100, single=False, tuning=None):
@@ -583,6 +605,8 @@ def main():
     ap.add_argument("--single-uneven", action="store_true",
                     help="one uneven batch of the 64 heads with the tuner's usual shape (K 6, 32 "
                          "rows, 4 TX per window, four wavefronts), no output (for rocprofv3 --pmc)")
+    ap.add_argument("--uneven-persistent", action="store_true",
+                    help="with --single-uneven: the persistent launch instead of per-round launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of 256 MiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -601,7 +625,7 @@ def main():
         _bench_3d_gradient(T3=100, every=20, single=True, tuning=tun)
         return
     if args.single_uneven:
-        _bench_uneven(autotune=False, single=True)
+        _bench_uneven(autotune=False, single=True, persistent=args.uneven_persistent)
         return
     if args.single_gradient:
         from ceep_amd import adjoint

@@ -352,9 +352,10 @@ class MultiHandle:
         return v.value
 
     def info(self):
-        out = (ctypes.c_longlong * 3)()
+        out = (ctypes.c_longlong * 4)()
         check(self._lib.ceep_fdtd_multi_info(self._m, out))
-        return {"items": out[0], "segments": out[1], "workgroups": out[2]}
+        return {"items": out[0], "segments": out[1], "persistent_workgroups": out[2],
+                "persistent_items": out[3]}
 
     def close(self):
         if getattr(self, "_m", None) and self._m.value:

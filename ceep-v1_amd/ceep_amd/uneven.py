@@ -1,7 +1,8 @@
 """Uneven batches: many ``BatchedFDTD2D`` problems (different grids, step
-counts, CPML widths, TX sets) advanced together: one captured hipGraph of
-per-round launches, each round one launch over every problem still running
-(the single persistent launch is switched off and refused by the library).
+counts, CPML widths, TX sets) advanced together in one captured hipGraph:
+either one persistent launch, whose workgroups take whole (problem, TX chunk)
+runs from a largest-first work list, or one launch per round over every
+problem still running.
 
 Each solver keeps its own API and outputs; ``UnevenBatch.run()`` fills every
 solver's ``probe_data`` as its own ``run()`` would (bit for bit the same as
@@ -45,9 +46,10 @@ class UnevenBatch:
     PER_ROUND = _native.MultiHandle.PER_ROUND
 
     def run(self, mode: int = PER_ROUND) -> List[Dict[int, Dict[int, object]]]:
-        """PER_ROUND: one launch per K-step round over every problem, the whole
-        run replayed as one hipGraph.  PERSISTENT (one launch) is switched off
-        in this build and raises NotImplementedError."""
+        """PERSISTENT (four-wavefront shapes; NotImplementedError otherwise):
+        one launch in which each workgroup steps whole (problem, TX chunk)
+        runs, every round of one alone.  PER_ROUND: one launch per K-step
+        round over every problem.  Either is replayed as one hipGraph."""
         self._multi.run(mode)
         out = []
         for s in self.solvers:

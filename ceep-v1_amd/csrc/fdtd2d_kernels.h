@@ -285,8 +285,8 @@ struct MultiDesc {
   int nrounds;
 };
 
-// The persistent uneven-batch launch: items are (round, problem, task),
-// enumerated as segments of one problem's tasks in one round.
+// Uneven batches: items are (round, problem, task), enumerated as segments
+// of one problem's tasks in one round; one launch per round.
 struct MultiArgs {
   const MultiDesc* probs;
   int nprob;
@@ -295,14 +295,29 @@ struct MultiArgs {
   const int* seg_round;
   int nseg;
   unsigned long long total;       // number of items
-  unsigned long long* queue;      // next item (zeroed before the launch)
-  unsigned* done;                 // [nprob] finished items (zeroed before the launch)
-  unsigned* status;               // set to 1 if a dependency wait ran out
-  unsigned long long wait_ticks;  // real-time clock ticks before a wait gives up
-  int xcd_order;                  // per-round launches: contiguous item ranges per XCD
+  int xcd_order;                  // contiguous item ranges per XCD
 };
 
-hipError_t launch_multi_tb(const MultiArgs& m, int poles, int R, int K, int blocks, hipStream_t s);
+// One persistent launch of an uneven batch: items are whole (problem, TX
+// chunk) runs, taken by workgroups of four-wavefront groups from a counter
+// over `items` (largest first); see persist_mw_kernel.
+struct PersistItem {
+  int prob;
+  int chunk;                      // TX planes [chunk * bpw, min((chunk + 1) * bpw, B))
+};
+struct PersistArgs {
+  const MultiDesc* probs;
+  const PersistItem* items;
+  int nitems;
+  unsigned* queue;                // next item (zeroed before the launch)
+};
+// Four-wavefront tasks (R = 32 or 48, K = 4 or 6), `blocks` workgroups of
+// persist_threads(poles, R) threads (groups of four wavefronts).
+hipError_t launch_persist_mw(const PersistArgs& m, int poles, int R, int K, bool mat_ids,
+                             int blocks, hipStream_t s);
+int persist_threads(int poles, int R);
+int persist_mw_blocks_per_cu(int poles, int R, int K, bool mat_ids);
+
 // One round's items [first, first + count) (segments [seg0, seg1)) as one launch.
 hipError_t launch_round_tb(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
                            long long first, long long count, hipStream_t s);
@@ -310,7 +325,6 @@ hipError_t launch_round_tb(const MultiArgs& m, int poles, int R, int K, int seg0
 // K = 4 or 6): one workgroup per work item.
 hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
                            long long first, long long count, bool mat_ids, hipStream_t s);
-int multi_tb_blocks_per_cu(int poles, int R, int K);
 // dst[i] = sum over k < nparts of src[k * stride + i], i < n
 hipError_t launch_sum_parts(const float* src, int nparts, long long stride, long long n,
                             float* dst, hipStream_t s);

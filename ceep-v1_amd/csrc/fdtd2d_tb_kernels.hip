@@ -633,83 +633,8 @@ __global__ __launch_bounds__(256) void step2d_tb_kernel(TB2DArgs p) {
   tb_task<R, K, POLES>(p, task, lane);
 }
 
-// NOTE: switched off (ceep_fdtd_multi_run refuses mode 0).  On MI355X the
-// first version (acquire load per poll) and this one (relaxed polls, one
-// acquire fence, real-time-bounded waits, global abort flag) both ran without
-// terminating, although a CPU model of the queue drains for every worker
-// count; the cause is not established -- the leading suspects are the
-// visibility of the done counters across XCDs to the polling loads and the
-// real-time bound -- so the path stays disabled until it is.
-//
-// Uneven batch in one persistent launch.  Every wavefront is a worker that
-// takes items from one queue ordered round by round (a round = K time steps
-// of every problem still running), so all items an item depends on -- the
-// previous round of its own problem, which read the buffer it will write and
-// wrote the buffer it will read -- were taken earlier and are running or
-// done; the oldest unfinished item never waits, so the queue always drains.
-// A worker publishes a finished item with an agent-scope release (its stores
-// reach the memory shared by all XCDs); a waiting worker polls the counter
-// with relaxed loads (an acquire per poll would invalidate the L2 each time)
-// and issues one agent-scope acquire fence once the count is reached.
-// Every wait is bounded in real time (m.wait_ticks of the 100 MHz constant
-// clock); a wait that runs out sets *status, after which every wait and every
-// queue take gives up at once, so the launch always drains.
-template <int R, int K, int POLES>
-__global__ __launch_bounds__(256) void multi_tb_kernel(MultiArgs m) {
-  const int lane = threadIdx.x & 63;
-  for (;;) {
-    unsigned long long i = 0;
-    if (lane == 0) {
-      i = __hip_atomic_load(m.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-              ? m.total : atomicAdd(m.queue, 1ull);
-    }
-    i = __shfl(i, 0);
-    if (i >= m.total) break;
-    int lo = 0, hi = m.nseg - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if ((unsigned long long)m.seg_start[mid] <= i) lo = mid; else hi = mid - 1;
-    }
-    const int pidx = m.seg_prob[lo];
-    const int round = m.seg_round[lo];
-    const int task = (int)(i - (unsigned long long)m.seg_start[lo]);
-    const MultiDesc& d = m.probs[pidx];
-    if (round > 0) {
-      const unsigned need = (unsigned)round * (unsigned)d.a.ntasks;
-      int ok = 1;
-      if (lane == 0) {
-        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-        while (__hip_atomic_load(m.done + pidx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-          if (__hip_atomic_load(m.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-              __builtin_amdgcn_s_memrealtime() - t0 > m.wait_ticks) {
-            ok = 0;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(32);
-        }
-        if (!ok) atomicOr(m.status, 1u);
-      }
-      ok = __shfl(ok, 0);
-      if (!ok) continue;                      // abandoned: the host reports the failure
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    TB2DArgs a = d.a;
-    const int cur = round & 1, nxt = cur ^ 1;
-    a.ez_in = d.f[cur][0]; a.hx_in = d.f[cur][1]; a.hy_in = d.f[cur][2];
-    a.ez_out = d.f[nxt][0]; a.hx_out = d.f[nxt][1]; a.hy_out = d.f[nxt][2];
-    a.jd_in = d.jd[cur]; a.jd_out = d.jd[nxt];
-    a.step0 = round * K;
-    a.ksteps = min(K, a.T - a.step0);
-    a.e_save = nullptr;
-    tb_task<R, K, POLES>(a, task, lane);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0)
-      __hip_atomic_fetch_add(m.done + pidx, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// The same work items without in-kernel waits: one launch per round covers
-// that round's segments [seg0, seg1) and the launch boundary orders rounds.
+// Uneven batch, one launch per round: the launch covers that round's
+// segments [seg0, seg1), and the launch boundary orders rounds.
 template <int R, int K, int POLES>
 __global__ __launch_bounds__(256) void round_tb_kernel(MultiArgs m, int seg0, int seg1,
                                                        long long first, long long count) {
@@ -765,48 +690,6 @@ hipError_t launch_round_tb(const MultiArgs& m, int poles, int R, int K, int seg0
   if (R == 32) { CEEP_RR(32) }
 #undef CEEP_RR
   return hipErrorInvalidValue;
-}
-
-template <int R, int K>
-static hipError_t launch_multi_rk(const MultiArgs& m, int poles, int blocks, hipStream_t s) {
-  const dim3 blk(256, 1, 1), grd(blocks, 1, 1);
-  if (poles == 0) hipLaunchKernelGGL((multi_tb_kernel<R, K, 0>), grd, blk, 0, s, m);
-  else if (poles == 1) hipLaunchKernelGGL((multi_tb_kernel<R, K, 1>), grd, blk, 0, s, m);
-  else return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-
-template <int R>
-static hipError_t launch_multi_r(const MultiArgs& m, int poles, int K, int blocks, hipStream_t s) {
-  switch (K) {
-    case 1: return launch_multi_rk<R, 1>(m, poles, blocks, s);
-    case 2: return launch_multi_rk<R, 2>(m, poles, blocks, s);
-    case 3: return launch_multi_rk<R, 3>(m, poles, blocks, s);
-    case 4: return launch_multi_rk<R, 4>(m, poles, blocks, s);
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_multi_tb(const MultiArgs& m, int poles, int R, int K, int blocks, hipStream_t s) {
-  switch (R) {
-    case 16: return launch_multi_r<16>(m, poles, K, blocks, s);
-    case 24: return launch_multi_r<24>(m, poles, K, blocks, s);
-    case 32: return launch_multi_r<32>(m, poles, K, blocks, s);
-  }
-  return hipErrorInvalidValue;
-}
-
-int multi_tb_blocks_per_cu(int poles, int R, int K) {
-  int n = 0;
-  const void* f = nullptr;
-#define CEEP_MF(RR, KK) \
-  f = poles ? (const void*)multi_tb_kernel<RR, KK, 1> : (const void*)multi_tb_kernel<RR, KK, 0>
-  if (R == 16) { if (K == 1) CEEP_MF(16, 1); else if (K == 2) CEEP_MF(16, 2); else if (K == 3) CEEP_MF(16, 3); else CEEP_MF(16, 4); }
-  else if (R == 24) { if (K == 1) CEEP_MF(24, 1); else if (K == 2) CEEP_MF(24, 2); else if (K == 3) CEEP_MF(24, 3); else CEEP_MF(24, 4); }
-  else { if (K == 1) CEEP_MF(32, 1); else if (K == 2) CEEP_MF(32, 2); else if (K == 3) CEEP_MF(32, 3); else CEEP_MF(32, 4); }
-#undef CEEP_MF
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, 0) != hipSuccess) return 0;
-  return n;
 }
 
 template <int R, int K>
@@ -1148,7 +1031,9 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
 // (R/(R-2K) (64/(64-2K)) + 1)/K = 0.82 at R = 16, K = 4 to 0.38 at R = 48,
 // K = 8.  Per-cell arithmetic is that of step2d_tb_kernel.
 // One task (output tile, TX chunk) advanced p.ksteps steps by the four
-// wavefronts of the calling workgroup; x_e / x_h are its LDS exchange rows.
+// wavefronts of the calling workgroup (or of its group: the task's two
+// workgroup barriers per step then hold every group); x_e / x_h are its LDS
+// exchange rows.
 template <int RW, int K, int POLES, bool MID>
 __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, int w,
                                         float (*x_e)[64], float (*x_h)[64]) {
@@ -1466,6 +1351,113 @@ hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0
   if (R == 48) { CEEP_RMW(12) }
 #undef CEEP_RMW
   return hipErrorInvalidValue;
+}
+
+// Uneven batch in one persistent launch.  A workgroup is G groups of four
+// wavefronts (G = 2..4, as many as the task's registers allow at 512 VGPRs
+// per SIMD lane).  It takes whole items -- one problem's TX
+// chunk, every round of it -- from an atomic counter over the item list
+// (ordered largest first) and steps each alone: in a round the groups take
+// the problem's tiles in turn, each tile being the four-wavefront task of
+// step2d_mw_kernel (the same code, so every problem is bit-identical to its
+// own run with that launch shape), and one workgroup barrier separates
+// rounds (the ping-pong sets: a round reads what the last one wrote).  No
+// workgroup ever waits for another, so the launch drains whatever the
+// order in which workgroups become resident.  A group with no tile left in
+// a round takes the task's barriers alone (two per step and TX plane), so
+// every wavefront meets the same barriers.
+template <int RW, int K, int POLES, bool MID, int G>
+__global__ __launch_bounds__(256 * G) void persist_mw_kernel(PersistArgs m) {
+  __shared__ float x_e[G][4][64], x_h[G][4][64];   // per group: the task's exchange rows
+  __shared__ int item_s;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = wave >> 2, w = wave & 3;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = (int)atomicAdd(m.queue, 1u);
+    __syncthreads();
+    const int item = item_s;
+    __syncthreads();                                  // read before the next take overwrites it
+    if (item >= m.nitems) break;                      // uniform per workgroup
+    const PersistItem it = m.items[item];
+    const MultiDesc* d = m.probs + it.prob;
+    const int tiles = d->a.tiles_x * d->a.tiles_y;
+    const int nrounds = d->nrounds;
+    for (int round = 0; round < nrounds; ++round) {
+      for (int t0 = 0; t0 < tiles; t0 += G) {
+        // the task's arguments are read afresh for each task (scalar loads)
+        // rather than kept live across the loops
+        TB2DArgs a = d->a;
+        const int cur = round & 1, nxt = cur ^ 1;
+        a.ez_in = d->f[cur][0]; a.hx_in = d->f[cur][1]; a.hy_in = d->f[cur][2];
+        a.ez_out = d->f[nxt][0]; a.hx_out = d->f[nxt][1]; a.hy_out = d->f[nxt][2];
+        a.jd_in = d->jd[cur]; a.jd_out = d->jd[nxt];
+        a.step0 = round * K;
+        a.ksteps = min(K, a.T - a.step0);
+        a.e_save = nullptr;
+        const int t = t0 + g;
+        if (t < tiles) {
+          mw_task<RW, K, POLES, MID>(a, it.chunk * tiles + t, lane, w, x_e[g], x_h[g]);
+        } else {   // no tile left for this group: the task's barriers alone
+          const int b_begin = it.chunk * a.bpw;
+          const int nb = min(b_begin + a.bpw, a.B) - b_begin;
+          for (int b = 0; b < nb; ++b)
+            for (int st = 0; st < K && st < a.ksteps; ++st) {
+              __syncthreads();
+              __syncthreads();
+            }
+        }
+      }
+      __syncthreads();                                // this round's set is complete
+    }
+  }
+}
+
+// groups per workgroup: 32-row windows take 82 VGPRs (plain) and 131
+// (Debye) per task, 48-row windows 110 and 182
+constexpr int persist_groups(int RW, int poles) {
+  return RW == 8 ? (poles ? 3 : 4) : (poles ? 2 : 3);
+}
+
+template <int RW, int K, bool MID>
+static const void* persist_fn(int poles) {
+  return poles == 0 ? (const void*)persist_mw_kernel<RW, K, 0, MID, persist_groups(RW, 0)>
+                    : (const void*)persist_mw_kernel<RW, K, 1, MID, persist_groups(RW, 1)>;
+}
+
+static const void* persist_kernel(int poles, int R, int K, bool mat_ids) {
+  if (poles != 0 && poles != 1) return nullptr;
+#define CEEP_PMW(RW)                                                              \
+  switch (K) {                                                                    \
+    case 4: return mat_ids ? persist_fn<RW, 4, true>(poles) : persist_fn<RW, 4, false>(poles); \
+    case 6: return mat_ids ? persist_fn<RW, 6, true>(poles) : persist_fn<RW, 6, false>(poles); \
+  }                                                                               \
+  return nullptr;
+  if (R == 32) { CEEP_PMW(8) }
+  if (R == 48) { CEEP_PMW(12) }
+#undef CEEP_PMW
+  return nullptr;
+}
+
+hipError_t launch_persist_mw(const PersistArgs& m, int poles, int R, int K, bool mat_ids,
+                             int blocks, hipStream_t s) {
+  const void* f = persist_kernel(poles, R, K, mat_ids);
+  if (!f || blocks < 1 || m.nitems < 0) return hipErrorInvalidValue;
+  void* args[] = {const_cast<PersistArgs*>(&m)};
+  return hipLaunchKernel(f, dim3((unsigned)blocks), dim3(persist_threads(poles, R)), args, 0, s);
+}
+
+int persist_threads(int poles, int R) {
+  return 256 * persist_groups(R == 32 ? 8 : 12, poles);
+}
+
+int persist_mw_blocks_per_cu(int poles, int R, int K, bool mat_ids) {
+  const void* f = persist_kernel(poles, R, K, mat_ids);
+  int n = 0;
+  if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, persist_threads(poles, R), 0) !=
+                hipSuccess)
+    return 0;
+  return n;
 }
 
 template <int RW, int K>

@@ -2358,9 +2358,9 @@ struct ceep_fdtd_multi {
   int* d_seg_round = nullptr;
   int nseg = 0;
   unsigned long long total = 0;
-  unsigned long long* d_queue = nullptr;
-  unsigned* d_done = nullptr;
-  unsigned* d_status = nullptr;
+  ceep::PersistItem* d_items = nullptr;   // persistent launch: (problem, TX chunk), largest first
+  int nitems = 0;
+  unsigned* d_queue = nullptr;
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // per mode
   std::vector<int> rounds;
@@ -2375,7 +2375,7 @@ void multi_free(ceep_fdtd_multi* m) {
   for (auto& g : m->gexec)
     if (g) (void)hipGraphExecDestroy(g);
   dfree(m->d_desc); dfree(m->d_seg_start); dfree(m->d_seg_prob); dfree(m->d_seg_round);
-  dfree(m->d_queue); dfree(m->d_done); dfree(m->d_status);
+  dfree(m->d_items); dfree(m->d_queue);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -2392,20 +2392,18 @@ int multi_enqueue(ceep_fdtd_multi* m, int mode) {
       if (h->jd[i]) CEEP_HIP(ceep::launch_zero(h->jd[i], bytes, s));
     CEEP_HIP(ceep::launch_zero(h->traces, (size_t)h->B * h->P * h->T * sizeof(float), s));
   }
-  CEEP_HIP(ceep::launch_zero(m->d_queue, sizeof(unsigned long long), s));
-  CEEP_HIP(ceep::launch_zero(m->d_done, m->hs.size() * sizeof(unsigned), s));
-  CEEP_HIP(ceep::launch_zero(m->d_status, sizeof(unsigned), s));
+  if (mode == 0) {
+    CEEP_HIP(ceep::launch_zero(m->d_queue, sizeof(unsigned), s));
+    ceep::PersistArgs p{};
+    p.probs = m->d_desc; p.items = m->d_items; p.nitems = m->nitems; p.queue = m->d_queue;
+    CEEP_HIP(ceep::launch_persist_mw(p, m->poles, m->R, m->K, m->mat_ids, m->blocks, s));
+    return CEEP_OK;
+  }
   ceep::MultiArgs a{};
   a.probs = m->d_desc; a.nprob = (int)m->hs.size();
   a.seg_start = m->d_seg_start; a.seg_prob = m->d_seg_prob; a.seg_round = m->d_seg_round;
   a.nseg = m->nseg; a.total = m->total;
-  a.queue = m->d_queue; a.done = m->d_done; a.status = m->d_status;
-  a.wait_ticks = 25000000ull;      // 0.25 s of the 100 MHz constant clock
   a.xcd_order = m->hs[0]->xcd_order;   // key 9 of the first problem's handle
-  if (mode == 0) {
-    CEEP_HIP(ceep::launch_multi_tb(a, m->poles, m->R, m->K, m->blocks, s));
-    return CEEP_OK;
-  }
   const int nrounds = (int)m->round_seg.size() - 1;
   for (int r = 0; r < nrounds; ++r) {
     const int s0 = m->round_seg[r], s1 = m->round_seg[r + 1];
@@ -2501,23 +2499,44 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) { multi_free(m); return fail(CEEP_EHIP, std::string("device query: ") + hipGetErrorString(e)); }
-  // workgroups of the persistent form (one-wavefront windows only)
-  const int per_cu = waves == 1 ? ceep::multi_tb_blocks_per_cu(poles, R, K) : 1;
-  m->blocks = std::max(1, per_cu) * cus;
+  // the persistent form (four-wavefront windows): one item per (problem, TX
+  // chunk), largest first (cells x steps x TX planes), so the last items to
+  // start are the shortest; as many 1024-thread workgroups as fit at once
+  std::vector<ceep::PersistItem> items;
+  std::vector<double> work;
+  if (waves == 4) {
+    for (int i = 0; i < n; ++i) {
+      ceep_fdtd* h = handles[i];
+      const int nch = (h->B + bpw - 1) / bpw;
+      for (int c = 0; c < nch; ++c) {
+        items.push_back(ceep::PersistItem{i, c});
+        work.push_back((double)h->M[0] * h->M[1] * h->T * (std::min(h->B, (c + 1) * bpw) - c * bpw));
+      }
+    }
+    std::vector<int> order(items.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    std::vector<ceep::PersistItem> sorted(items.size());
+    for (size_t k = 0; k < order.size(); ++k) sorted[k] = items[order[k]];
+    items.swap(sorted);
+    m->nitems = (int)items.size();
+    m->blocks = std::max(1, ceep::persist_mw_blocks_per_cu(poles, R, K, m->mat_ids)) * cus;
+  }
   rc = CEEP_OK;
   if (!rc) rc = dalloc(&m->d_desc, (size_t)n);
   if (!rc) rc = dalloc(&m->d_seg_start, seg_start.size());
   if (!rc) rc = dalloc(&m->d_seg_prob, seg_prob.size());
   if (!rc) rc = dalloc(&m->d_seg_round, seg_round.size());
+  if (!rc && m->nitems) rc = dalloc(&m->d_items, items.size());
   if (!rc) rc = dalloc(&m->d_queue, 1);
-  if (!rc) rc = dalloc(&m->d_done, (size_t)n);
-  if (!rc) rc = dalloc(&m->d_status, 1);
   if (rc) { multi_free(m); return rc; }
   e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMemcpy(m->d_desc, desc.data(), desc.size() * sizeof(desc[0]), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->d_seg_start, seg_start.data(), seg_start.size() * sizeof(long long), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->d_seg_prob, seg_prob.data(), seg_prob.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->d_seg_round, seg_round.data(), seg_round.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && m->nitems)
+    e = hipMemcpy(m->d_items, items.data(), items.size() * sizeof(items[0]), hipMemcpyHostToDevice);
   if (e != hipSuccess) { multi_free(m); return fail(CEEP_EHIP, std::string("multi setup: ") + hipGetErrorString(e)); }
   *out = m;
   return CEEP_OK;
@@ -2526,10 +2545,8 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
 int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode) {
   if (!m) return fail(CEEP_EINVAL, "null handle");
   if (mode != 0 && mode != 1) return fail(CEEP_EINVAL, "mode must be 0 (persistent) or 1 (per round)");
-  if (mode == 0)   // kept in the source, switched off: see the note at multi_tb_kernel
-    return fail(CEEP_EUNSUPPORTED,
-                "the persistent single-launch mode is switched off in this build (its runs on "
-                "MI355X did not terminate); use mode 1, one launch per round in one graph");
+  if (mode == 0 && m->waves != 4)
+    return fail(CEEP_EUNSUPPORTED, "the persistent launch takes four-wavefront window shapes");
   int rc;
   if (!m->gexec[mode]) {
     hipGraph_t g = nullptr;
@@ -2543,11 +2560,7 @@ int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode) {
     if (e != hipSuccess) { m->gexec[mode] = nullptr; return fail(CEEP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
   }
   CEEP_HIP(hipGraphLaunch(m->gexec[mode], m->stream));
-  unsigned status = 0;
-  CEEP_HIP(hipMemcpyAsync(&status, m->d_status, sizeof(unsigned), hipMemcpyDeviceToHost, m->stream));
   CEEP_HIP(hipStreamSynchronize(m->stream));
-  if (status)
-    return fail(CEEP_EHIP, "uneven-batch launch: a dependency wait ran out (results invalid)");
   for (size_t i = 0; i < m->hs.size(); ++i) {
     m->hs[i]->parity = m->rounds[i] & 1;
     m->hs[i]->has_run = true;
@@ -2570,16 +2583,13 @@ int ceep_fdtd_multi_time(ceep_fdtd_multi* m, int mode, int reps, double* ms) {
   CEEP_HIP(hipEventElapsedTime(&t, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  unsigned status = 0;
-  CEEP_HIP(hipMemcpy(&status, m->d_status, sizeof(unsigned), hipMemcpyDeviceToHost));
-  if (status) return fail(CEEP_EHIP, "uneven-batch launch: a dependency wait ran out (results invalid)");
   *ms = t / reps;
   return CEEP_OK;
 }
 
 int ceep_fdtd_multi_info(ceep_fdtd_multi* m, long long* out) {
   if (!m || !out) return fail(CEEP_EINVAL, "null argument");
-  out[0] = (long long)m->total; out[1] = m->nseg; out[2] = m->blocks;
+  out[0] = (long long)m->total; out[1] = m->nseg; out[2] = m->blocks; out[3] = m->nitems;
   return CEEP_OK;
 }
 

@@ -287,10 +287,9 @@ int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out);
 /* ---- uneven batches (no counterpart in the reference) ----------------------
  * Many 2D problems of different grid sizes and step counts, each a fully set
  * up handle (materials, sources, probes; no Lorentz poles; one pole set for
- * all), advanced together round by round: each round (K steps of every
- * problem still running) is one launch over all problems' windows, and the
- * clears and rounds are captured as one hipGraph on the first run.  Traces
- * land in each handle (ceep_fdtd_get_traces, ceep_fdtd_dft). */
+ * all), advanced together in one persistent launch or round by round, the
+ * clears and the launches captured as one hipGraph on the first run.
+ * Traces land in each handle (ceep_fdtd_get_traces, ceep_fdtd_dft). */
 typedef struct ceep_fdtd_multi ceep_fdtd_multi;
 
 /* waves = 1: one wavefront per window, K = steps per round (1..4), R =
@@ -301,17 +300,23 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
                            ceep_fdtd_multi** out);
 
 /* Zero every problem and run all of them to their own total_steps, as a
- * captured graph (captured on the first call).  mode 1: every round (K steps
- * of every problem still running) is one launch over all problems' windows;
- * the graph holds the clears and every round.  mode 0, one persistent launch
- * over a dependency-ordered work queue, is switched off in this build and
- * returns CEEP_EUNSUPPORTED. */
+ * captured graph (captured on the first call).  mode 0 (four-wavefront
+ * shapes; CEEP_EUNSUPPORTED otherwise): one persistent launch of 1024-thread
+ * workgroups, as many as fit at once, each taking whole (problem, TX chunk)
+ * runs from a counter over a largest-first list and stepping every round of
+ * one alone (its four 4-wavefront groups split each round's windows; a
+ * workgroup barrier orders rounds; no workgroup waits for another).  mode 1:
+ * every round (K steps of every problem still running) is one launch over
+ * all problems' windows.  Either mode gives every problem the bits of its
+ * own blocked run with the same window shape. */
 int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode);
 
 /* `reps` graph replays of `mode` bracketed by HIP events; *ms = mean ms per replay. */
 int ceep_fdtd_multi_time(ceep_fdtd_multi* m, int mode, int reps, double* ms);
 
-/* out[0] = queue items, out[1] = (round, problem) segments, out[2] = workgroups. */
+/* out[0] = per-round work items, out[1] = (round, problem) segments,
+ * out[2] = workgroups of the persistent launch (0 for one-wavefront shapes),
+ * out[3] = its (problem, TX chunk) items. */
 int ceep_fdtd_multi_info(ceep_fdtd_multi* m, long long* out);
 
 int ceep_fdtd_multi_destroy(ceep_fdtd_multi* m);

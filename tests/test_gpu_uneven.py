@@ -1,5 +1,6 @@
-"""Uneven batch in one persistent launch (GPU): each problem bit-identical to
-its own blocked run and within 1e-5 of the FP64 oracle."""
+"""Uneven batches (GPU), in one persistent launch and as per-round launches:
+each problem bit-identical to its own blocked run and within 1e-5 of the
+FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -97,16 +98,39 @@ def test_uneven_shape_checks(amd):
             UnevenBatch(group, k, r, 1, waves)
 
 
-@pytest.mark.skip(reason="persistent single-launch mode switched off: its MI355X runs did not "
-                         "terminate and the cause is not established (the entry point refuses it)")
 @pytest.mark.parametrize("debye", [True, False])
-@pytest.mark.parametrize("K,R,bpw", [(4, 16, 1), (3, 24, 2), (2, 16, 4)])
+@pytest.mark.parametrize("K,R,bpw", [(6, 32, 4), (4, 32, 1), (6, 48, 2), (4, 48, 3)])
 def test_uneven_persistent_launch(amd, debye, K, R, bpw):
+    """One persistent launch: workgroups take whole (problem, TX chunk) runs
+    from a largest-first list and step all their rounds alone; every problem
+    bit-identical to its own four-wavefront run of the same shape (and the
+    graph replay to the first run), B not a multiple of bpw in most problems."""
     from ceep_amd.uneven import UnevenBatch
-    _check(amd, debye, K, R, bpw, UnevenBatch.PERSISTENT)
+    _check(amd, debye, K, R, bpw, UnevenBatch.PERSISTENT, waves=4)
 
 
-def test_uneven_persistent_mode_refused(amd):
+def test_uneven_persistent_and_per_round_agree(amd):
+    """The persistent launch and the per-round launches of one group give the
+    same bits, run after run on the same group."""
+    from ceep_amd.uneven import UnevenBatch
+    tun = {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2, "forward_waves": 4}
+    group = [_make(amd, sp, True, tun)[0] for sp in SPECS]
+    ub = UnevenBatch(group, 6, 32, 2, 4)
+    out = []
+    for mode in (UnevenBatch.PERSISTENT, UnevenBatch.PER_ROUND, UnevenBatch.PERSISTENT):
+        print(f"uneven batch mode={mode}", flush=True)
+        ub.run(mode)
+        out.append([s.probe_data.copy() for s in group])
+    info = ub.info()
+    assert info["persistent_items"] == sum(-(-sp[4] // 2) for sp in SPECS)
+    assert info["persistent_workgroups"] > 0
+    for o in out[1:]:
+        for a, b in zip(o, out[0]):
+            assert np.array_equal(a, b)
+    ub.close()
+
+
+def test_uneven_persistent_needs_four_wavefront_windows(amd):
     from ceep_amd.uneven import UnevenBatch
     tun = {"temporal_depth": 4, "rows_per_wave": 16, "batch_per_wave": 1}
     group = [_make(amd, sp, True, tun) for sp in SPECS[:2]]

@@ -24,6 +24,7 @@
 #include "reduce_kernels.h"
 #include "imaging.h"
 #include "coef2d.h"
+#include "limits2d.h"
 
 namespace {
 
@@ -281,7 +282,9 @@ struct ceep_fdtd {
     return tb3d() && adj3d_k > 0 && (2 * box_floats() + plane) * 4 < (1LL << 31);
   }
   // the temporally blocked 2D kernel covers the plain and Debye cases
-  bool use_tb() const { return ndim == 2 && tb_k > 0 && !has_lorentz && !cpml(); }
+  bool use_tb() const {
+    return ndim == 2 && tb_k > 0 && !has_lorentz && !cpml() && ceep::tb2d_fits(plane);
+  }
   // 2D gradient path: reversible reconstruction instead of recompute + E store
   bool use_rev() const { return adj_rev && use_tb(); }
   int launches_per_run() const {
@@ -2443,6 +2446,8 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
     if ((rc = ready_check(h))) return rc;
     if (h->ndim != 2 || h->has_lorentz || h->cpml())
       return fail(CEEP_EUNSUPPORTED, "uneven batches take 2D cavity problems without Lorentz poles");
+    if (!ceep::tb2d_fits(h->plane))
+      return fail(CEEP_EUNSUPPORTED, "uneven batches take 2D boxes of fewer than 2^28 padded cells");
     if (poles >= 0 && poles != h->poles())
       return fail(CEEP_EUNSUPPORTED, "every problem of an uneven batch needs the same pole set");
     poles = h->poles();

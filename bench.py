@@ -382,26 +382,25 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False, persistent
     the faster.  The comparison runs the 64 problems one after another with
     the one-step kernel.  `single`: one batch in the persistent form (or per
     round with `persistent` False), nothing timed (for rocprofv3 --pmc)."""
-    from ceep_amd.autotune import autotune_uneven
+    from ceep_amd.autotune import autotune_uneven, autotune_uneven_persistent
     from ceep_amd.uneven import UnevenBatch
 
     solvers, grids = _uneven_solvers(n_problems, seed)
-    shape, log = (6, 32, 4, 4), []    # K, window rows, TX planes per window, wavefronts (the tuner's pick)
+    # K, window rows, TX planes per window (per item), wavefronts: the tuners' picks
+    shape, log = (6, 32, 4, 4), []
+    shape_p, log_p = (6, 48, 2, 4), []
     if autotune:
         shape, log = autotune_uneven(solvers)
-    ub = UnevenBatch(solvers, *shape)
+        shape_p, log_p = autotune_uneven_persistent(solvers)
     if single:                       # one batch, nothing timed (for rocprofv3 --pmc)
+        ub = UnevenBatch(solvers, *(shape_p if persistent else shape))
         ub._multi.run(UnevenBatch.PERSISTENT if persistent else UnevenBatch.PER_ROUND)
         _device_sync()
         ub.close()
         return None
-    cell_steps = ub.cell_steps
-    full_cells = sum(g * g * B * s.total_steps for g, s in zip(grids, solvers))
-    modes = {"per_round": UnevenBatch.PER_ROUND}
-    if shape[3] == 4:
-        modes["persistent"] = UnevenBatch.PERSISTENT
+    forms = {"per_round": (shape, UnevenBatch.PER_ROUND), "persistent": (shape_p, UnevenBatch.PERSISTENT)}
 
-    def batched(mode):
+    def batched(ub, mode):
         ub._multi.run(mode)
         for s in solvers:
             s._handle.dft(FREQS)
@@ -412,19 +411,23 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False, persistent
             s._handle.run(copy=False)
             s._handle.dft(FREQS)
 
-    t_mode, graph_mode = {}, {}
-    for name, mode in modes.items():
-        batched(mode)
+    t_mode, graph_mode, infos = {}, {}, {}
+    for name, (shp, mode) in forms.items():
+        ub = UnevenBatch(solvers, *shp)
+        batched(ub, mode)
         _device_sync()
         t0 = time.perf_counter()
-        batched(mode)
+        batched(ub, mode)
         _device_sync()
         t_mode[name] = time.perf_counter() - t0
         graph_mode[name] = ub.time(2, mode)
+        infos[name] = ub.info()
+        cell_steps = ub.cell_steps
+        ub.close()
+    full_cells = sum(g * g * B * s.total_steps for g, s in zip(grids, solvers))
     best = min(t_mode, key=t_mode.get)
     t_b, ms_graph = t_mode[best], graph_mode[best]
-    info = ub.info()
-    ub.close()
+    info = infos["per_round"]
     for s in solvers:
         s._handle.apply_tuning({"temporal_depth": 0, "batch_per_thread": 4, "graph": 1})
     sequential()
@@ -444,10 +447,13 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False, persistent
             "speedup_vs_sequential": round(t_s / t_b, 3),
             "launch_shape": {"temporal_depth": shape[0], "rows_per_wave": shape[1],
                              "batch_per_wave": shape[2], "wavefronts_per_window": shape[3]},
+            "launch_shape_persistent": {"temporal_depth": shape_p[0], "rows_per_wave": shape_p[1],
+                                        "batch_per_item": shape_p[2]},
             "autotune_log_ms": [[round(ms, 2), list(sh)] for ms, sh in sorted(log)],
+            "autotune_log_persistent_ms": [[round(ms, 2), list(sh)] for ms, sh in sorted(log_p)],
             "work_items": info["items"], "segments": info["segments"],
-            "persistent_items": info["persistent_items"],
-            "persistent_workgroups": info["persistent_workgroups"],
+            "persistent_items": infos["persistent"]["persistent_items"],
+            "persistent_workgroups": infos["persistent"]["persistent_workgroups"],
             "cell_steps": cell_steps,
             "grids": {str(g): grids.count(g) for g in sorted(set(grids))},
             "workload": (f"{n_problems} seeded Debye heads, grids from 256/384/512/600 squared, "

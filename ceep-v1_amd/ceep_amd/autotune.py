@@ -154,3 +154,33 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
     for s in solvers:
         s._handle.reset()
     return best, log
+
+
+def autotune_uneven_persistent(solvers, reps: int = 1,
+                               shapes=((6, 32, 1), (6, 32, 2), (4, 32, 1), (6, 48, 1), (6, 48, 2),
+                                       (4, 48, 1)),
+                               verbose: bool = False):
+    """Four-wavefront launch shape (K, window rows, TX planes per item) for the
+    persistent form of an ``UnevenBatch``, whose items are whole (problem, TX
+    chunk) runs: fewer TX per item balance the workgroups better, more
+    amortise each task's loads.  Each shape is a captured graph of the whole
+    batch, timed with HIP events after one warm replay.
+
+    Returns ``(best_shape, log)``, shapes as ``(K, R, planes, 4)``."""
+    from .uneven import UnevenBatch
+
+    best, best_ms, log = None, float("inf"), []
+    for k, r, b in shapes:
+        ub = UnevenBatch(solvers, k, r, b, 4)
+        try:
+            ms = ub.time(reps, UnevenBatch.PERSISTENT)
+        finally:
+            ub.close()
+        log.append((ms, (k, r, b, 4)))
+        if verbose:
+            print(f"  {ms:9.3f} ms  persistent K={k} R={r} planes/item={b}", flush=True)
+        if ms < best_ms:
+            best, best_ms = (k, r, b, 4), ms
+    for s in solvers:
+        s._handle.reset()
+    return best, log

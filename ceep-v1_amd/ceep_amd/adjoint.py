@@ -162,6 +162,15 @@ def s_norm2(traces_: torch.Tensor, freqs, dt: float) -> torch.Tensor:
     return (d * d).sum()
 
 
+def _leaf(a) -> torch.Tensor:
+    """A float64 leaf tensor over ``a``'s memory when ``a`` is a writeable
+    contiguous float64 array (the gradient only reads it), else over a copy."""
+    arr = np.ascontiguousarray(a, dtype=np.float64)
+    if not arr.flags.writeable:
+        arr = arr.copy()
+    return torch.from_numpy(arr).requires_grad_(True)
+
+
 def gradient(solver, eps_r, sigma_e, freqs, s_target=None, every: int = 50,
              fp64_forward: bool = False, target=None):
     """(L, dL/d eps_r, dL/d sigma) as NumPy for L = sum |S - S_target|^2 at ``freqs``.
@@ -169,8 +178,8 @@ def gradient(solver, eps_r, sigma_e, freqs, s_target=None, every: int = 50,
     ``s_target``: the target S-matrix (F, RX, TX); the residual is S minus it.
     ``target=(eps_r_t, sigma_t)`` instead: S_target is the S-matrix of that
     medium, and the residual is stepped as a difference in FP32."""
-    e = torch.as_tensor(np.asarray(eps_r, dtype=np.float64)).clone().requires_grad_(True)
-    s = torch.as_tensor(np.asarray(sigma_e, dtype=np.float64)).clone().requires_grad_(True)
+    e = _leaf(eps_r)
+    s = _leaf(sigma_e)
     if target is not None:
         if s_target is not None or fp64_forward:
             raise ValueError("target= replaces s_target and the FP64 forward")

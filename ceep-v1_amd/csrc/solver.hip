@@ -157,6 +157,7 @@ struct ceep_fdtd {
   float* partial = nullptr;
   float2* tw = nullptr;        // DFT twiddles [T][F] for tw_freqs
   float2* S = nullptr;         // DFT output [F][P][B]
+  float2* S_host = nullptr;    // its pinned host staging (hipHostMalloc), same size
   std::vector<double> tw_freqs;
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one per starting buffer parity
@@ -1111,6 +1112,7 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   dfree(h->dk1); dfree(h->dk2); dfree(h->lk1); dfree(h->lk2); dfree(h->lk3);
   dfree(h->src2); dfree(h->src3); dfree(h->wave); dfree(h->probe_head); dfree(h->probe_next); dfree(h->probe_cell); dfree(h->probe_cell3);
   dfree(h->traces); dfree(h->partial); dfree(h->tw); dfree(h->S);
+  if (h->S_host) { (void)hipHostFree(h->S_host); h->S_host = nullptr; }
   for (int i = 0; i < 2; ++i)
     for (int k = 0; k < 3; ++k) dfree(h->f64[i][k]);
   dfree(h->ca64); dfree(h->cb64); dfree(h->wave64); dfree(h->traces64);
@@ -1308,18 +1310,21 @@ int ceep_fdtd_dft(ceep_fdtd* h, const double* freqs, int n_freq, double* s_out) 
     }
     dfree(h->tw);
     dfree(h->S);
+    if (h->S_host) { (void)hipHostFree(h->S_host); h->S_host = nullptr; }
     h->tw_freqs.clear();
     if ((rc = dalloc(&h->tw, tw.size()))) return rc;
     if ((rc = dalloc(&h->S, (size_t)F * h->P * h->B))) return rc;
+    CEEP_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->S_host),
+                           (size_t)F * h->P * h->B * sizeof(float2), hipHostMallocDefault));
     CEEP_HIP(hipMemcpyAsync(h->tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
     CEEP_HIP(hipStreamSynchronize(h->stream));
     h->tw_freqs.assign(freqs, freqs + F);
   }
   const size_t ns = (size_t)F * h->P * h->B;
-  std::vector<float2> S(ns);
   CEEP_HIP(ceep::launch_dft(h->traces, h->tw, h->B, h->P, T, F, (float)h->dt, h->S, h->stream));
-  CEEP_HIP(hipMemcpyAsync(S.data(), h->S, ns * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
+  CEEP_HIP(hipMemcpyAsync(h->S_host, h->S, ns * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
   CEEP_HIP(hipStreamSynchronize(h->stream));
+  const float2* S = h->S_host;
   for (size_t i = 0; i < ns; ++i) { s_out[2 * i] = S[i].x; s_out[2 * i + 1] = S[i].y; }
   return CEEP_OK;
 }

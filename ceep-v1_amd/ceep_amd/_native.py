@@ -60,6 +60,8 @@ SIGNATURES = {
     "ceep_fdtd_adjoint": (_I, [_P, _PD, _PD, _PD]),
     "ceep_fdtd_set_reference": (_I, [_P, _PD, _PD]),
     "ceep_fdtd_run_residual": (_I, [_P, _I, _PD, _PD]),
+    "ceep_fdtd_gradient_misfit": (_I, [_P, _I, _PD, _I, _PD, _PD, _PD, _PD]),
+    "ceep_fdtd_get_reference_traces": (_I, [_P, _PD]),
     "ceep_fdtd_set_tuning": (_I, [_P, _I, _I]),
     "ceep_fdtd_multi_create": (_I, [_I, ctypes.POINTER(_P), _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "ceep_fdtd_multi_run": (_I, [_P, _I]),
@@ -147,6 +149,8 @@ TUNE_TX_GROUP_3D = 18
 TUNE_ADJ3D_DEPTH = 19
 TUNE_ADJ3D_ORDER = 20
 TUNE_MATERIAL_IDS = 22
+TUNE_SPLIT_DIFF = 23
+TUNE_ADJ_PREFETCH = 24
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
@@ -158,7 +162,9 @@ TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "adjoint_temporal_depth": TUNE_ADJ_DEPTH, "forward_waves": TUNE_FWD_WAVES,
                "temporal_depth_3d": TUNE_TB3D_DEPTH, "tx_group_3d": TUNE_TX_GROUP_3D,
                "adjoint_temporal_depth_3d": TUNE_ADJ3D_DEPTH,
-               "adjoint_order_3d": TUNE_ADJ3D_ORDER, "material_ids": TUNE_MATERIAL_IDS}
+               "adjoint_order_3d": TUNE_ADJ3D_ORDER, "material_ids": TUNE_MATERIAL_IDS,
+               "split_difference": TUNE_SPLIT_DIFF,
+               "adjoint_prefetch": TUNE_ADJ_PREFETCH}
 
 
 class Handle:
@@ -267,6 +273,26 @@ class Handle:
         check(self._lib.ceep_fdtd_run_residual(self._h, int(every), dptr(res), dptr(ref)))
         return res, ref
 
+    def gradient_misfit(self, every, freqs, s_target=None):
+        """(L, dL/d eps_r, dL/d sigma) of L = sum |S - S_target|^2 at ``freqs``
+        in one device pass: checkpointed forward (the difference form when a
+        reference medium is set, then ``s_target`` must be None), S-matrix,
+        loss and dL/dtrace on the device, adjoint."""
+        f = np.ascontiguousarray(np.atleast_1d(np.asarray(freqs, dtype=np.float64)))
+        tgt = None
+        if s_target is not None:
+            st = np.asarray(s_target, dtype=np.complex128)
+            if st.shape != (f.size, self.n_probes, self.batch):
+                raise ValueError(f"S_target must have shape {(f.size, self.n_probes, self.batch)}")
+            tgt = np.ascontiguousarray(st).view(np.float64)
+        loss = ctypes.c_double()
+        ge = np.zeros(self.shape, dtype=np.float64)
+        gs = np.zeros(self.shape, dtype=np.float64)
+        check(self._lib.ceep_fdtd_gradient_misfit(
+            self._h, int(every), dptr(f), int(f.size), dptr(tgt) if tgt is not None else None,
+            ctypes.byref(loss), dptr(ge), dptr(gs)))
+        return loss.value, ge, gs
+
     def adjoint(self, dtraces):
         g = np.ascontiguousarray(np.asarray(dtraces, dtype=np.float64))
         if g.shape != (self.batch, self.n_probes, self.total_steps):
@@ -275,6 +301,12 @@ class Handle:
         gs = np.zeros(self.shape, dtype=np.float64)
         check(self._lib.ceep_fdtd_adjoint(self._h, dptr(g), dptr(ge), dptr(gs)))
         return ge, gs
+
+    def reference_traces(self):
+        """(B, P, T) traces of the reference medium's run (the difference form)."""
+        out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)
+        check(self._lib.ceep_fdtd_get_reference_traces(self._h, dptr(out)))
+        return out
 
     def traces(self):
         out = np.zeros((self.batch, self.n_probes, self.total_steps), dtype=np.float64)

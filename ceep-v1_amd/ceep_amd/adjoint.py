@@ -38,6 +38,8 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from .batched_2d import BatchedFDTD2D
+
 
 class BatchedTraces(torch.autograd.Function):
     """traces = F(eps_r, sigma) for a built-or-buildable ``BatchedFDTD2D``."""
@@ -171,13 +173,42 @@ def _leaf(a) -> torch.Tensor:
     return torch.from_numpy(arr).requires_grad_(True)
 
 
+def _gradient_fused(solver, eps_r, sigma_e, freqs, s_target, every, target):
+    """One ``ceep_fdtd_gradient_misfit`` call: forward, S-matrix, loss,
+    dL/dtrace and adjoint without the traces leaving the device."""
+    solver.update_materials(np.ascontiguousarray(eps_r, dtype=np.float64),
+                            np.ascontiguousarray(sigma_e, dtype=np.float64))
+    h = solver._handle
+    h.set_tuning("fp64_forward", 0)
+    if target is not None:
+        _ensure_target(solver, target)
+        L, ge, gs = h.gradient_misfit(every, freqs)
+        solver._defer_probe_data(lambda: h.reference_traces() + h.traces())
+        solver._residual_run = True
+    else:
+        L, ge, gs = h.gradient_misfit(every, freqs, s_target)
+        solver._defer_probe_data(h.traces)
+        solver._residual_run = False
+    return L, ge, gs
+
+
 def gradient(solver, eps_r, sigma_e, freqs, s_target=None, every: int = 50,
-             fp64_forward: bool = False, target=None):
+             fp64_forward: bool = False, target=None, fused: bool = True):
     """(L, dL/d eps_r, dL/d sigma) as NumPy for L = sum |S - S_target|^2 at ``freqs``.
 
     ``s_target``: the target S-matrix (F, RX, TX); the residual is S minus it.
     ``target=(eps_r_t, sigma_t)`` instead: S_target is the S-matrix of that
-    medium, and the residual is stepped as a difference in FP32."""
+    medium, and the residual is stepped as a difference in FP32.
+    ``fused``: for 2D solvers with the FP32 forward, the whole gradient runs
+    in one library call (S-matrix, loss and dL/dtrace on the device);
+    ``fused=False`` takes the autograd path (``BatchedTraces`` /
+    ``BatchedResidual`` and the torch misfit on the host), the same math."""
+    if target is not None and (s_target is not None or fp64_forward):
+        raise ValueError("target= replaces s_target and the FP64 forward")
+    if target is None and s_target is None:
+        raise ValueError("give s_target or target")
+    if fused and not fp64_forward and isinstance(solver, BatchedFDTD2D):
+        return _gradient_fused(solver, eps_r, sigma_e, freqs, s_target, every, target)
     e = _leaf(eps_r)
     s = _leaf(sigma_e)
     if target is not None:

@@ -257,10 +257,30 @@ class BatchedFDTD2D:
         return self._as_dict(traces)
 
     # -- outputs ----------------------------------------------------------------------
+    # (B, P, T) float64 traces of the last run.  A gradient call that keeps its
+    # traces on the device leaves a fetch function instead, run on first access.
+    @property
+    def probe_data(self):
+        fetch = self.__dict__.get("_probe_fetch")
+        if fetch is not None:
+            self.__dict__["_probe_data"] = fetch()
+            self.__dict__["_probe_fetch"] = None
+        return self.__dict__.get("_probe_data")
+
+    @probe_data.setter
+    def probe_data(self, value):
+        self.__dict__["_probe_fetch"] = None
+        self.__dict__["_probe_data"] = value
+
+    def _defer_probe_data(self, fetch) -> None:
+        self.__dict__["_probe_fetch"] = fetch
+        self.__dict__["_probe_data"] = None
+
     def s_parameters(self, frequencies) -> MultistaticSParameters:
         """Running DFT (solvers/dft.py:55-90) of the last run's traces as an
         ``(F, RX, TX)`` ``MultistaticSParameters`` (s_params.py:70-99)."""
-        if self._handle is None or self.probe_data is None:
+        if self._handle is None or (self.__dict__.get("_probe_fetch") is None and
+                                    self.probe_data is None):
             raise RuntimeError("run() first")
         f = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
         if getattr(self, "_residual_run", False):

@@ -212,6 +212,12 @@ struct TB2DArgs {
   const unsigned char* mat_id;
   const float4* mat_table;
   int n_mat;                      // <= kMaxMaterials
+  // optional subset of the output tiles (four- and eight-wavefront forward):
+  // task = chunk * n_list + i steps tile tile_list[i]; null: every tile
+  const int* tile_list;
+  int n_list;
+  int no_inject;                  // 1: no source term (the difference form's
+                                  // tiles where the media agree)
 };
 
 constexpr int kMaxMaterials = 256;
@@ -224,7 +230,10 @@ hipError_t launch_step2d_tb(const TB2DArgs& a, int poles, int R, int K, hipStrea
 // The same update with an R x 64 window split over the four wavefronts of a
 // workgroup (R / 4 rows each, halo rows through LDS); one workgroup per task.
 // R in {32, 48, 64}, K in {4, 6, 8}, poles 0, 1 or kPolesDiff.
-hipError_t launch_step2d_mw(const TB2DArgs& a, int poles, int R, int K, hipStream_t s);
+// NW = 4 or 8 wavefronts per window; step2d_mw_shape_ok lists the (R, K)
+// instantiated for each
+bool step2d_mw_shape_ok(int R, int K, int NW);
+hipError_t launch_step2d_mw(const TB2DArgs& a, int poles, int R, int K, int NW, hipStream_t s);
 
 // Temporally blocked adjoint: K adjoint steps (step_hi-1 down to step_hi-ksteps)
 // per launch on R x 64 register windows, the transpose of step2d_tb_kernel's
@@ -271,10 +280,14 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
 // H^{s-1/2} = H^{s+1/2} -+ ch dE^s) inside the same windows as the adjoint,
 // so no segment is recomputed and no E store is read.
 hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
-// The same reversed steps with an R x 64 window split over the four
-// wavefronts of a workgroup (R / 4 rows each; halo rows through LDS); one
-// workgroup per task, R in {16, 24, 32}.
-hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, hipStream_t s);
+// The same reversed steps with an R x 64 window split over the NW
+// wavefronts of a workgroup (R / NW rows each; halo rows through LDS); one
+// workgroup per task.  NW = 4: R in {16, 24, 32}; NW = 8: R in {32, 48}.
+// prefetch: the next TX's planes are loaded while the current one is stepped
+// (NW = 4: R = 32, K = 6; NW = 8: R = 32, K = 4..6)
+bool adj2d_rev_mw_shape_ok(int R, int K, int NW, bool prefetch);
+hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, int NW, bool prefetch,
+                               hipStream_t s);
 
 // One problem of an uneven batch: its blocked-kernel arguments (buffers set
 // per round from the ping-pong pairs) and the number of K-step rounds.

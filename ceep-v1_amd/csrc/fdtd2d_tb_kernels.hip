@@ -747,23 +747,36 @@ hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s
   return hipErrorInvalidValue;
 }
 
-// Four-wavefront form of adj2d_rev_tb_kernel: one workgroup owns an
-// R x 64 window (R = 4 RW) and each of its wavefronts RW consecutive rows,
-// so a wavefront holds a quarter of the state of the one-wavefront form.
-// Each reversed step needs one row across every wavefront boundary four
-// times -- the old hy above (forward E), the new E below (forward H), q
-// below (adjoint H) and the new mu_y above (lam) -- and passes them through
-// LDS between three workgroup barriers.  The arithmetic per cell is that of
-// adj2d_rev_tb_kernel; with the same R and K the results are bit-identical.
-// The smaller per-wavefront footprint (about 100 VGPRs at RW = 8) lets four
-// or more wavefronts share a SIMD and makes R = 32 windows affordable, whose
-// (R - 2K) / R output fraction is 75 % at K = 4.
+// Multi-wavefront form of adj2d_rev_tb_kernel: one workgroup owns an
+// R x 64 window (R = NW RW, NW = 4 or 8 wavefronts) and each of its
+// wavefronts RW consecutive rows.  Each reversed step needs one row across
+// every wavefront boundary four times -- the old hy above (forward E), the
+// new E below (forward H), q below (adjoint H) and the new mu_y above (lam)
+// -- and passes them through LDS between three workgroup barriers.  The
+// arithmetic per cell is that of adj2d_rev_tb_kernel; with the same R and K
+// the results are bit-identical.  A wavefront's registers grow by about 17
+// VGPRs per row (six fields, three maps, two gradient sums and the shifted
+// neighbours): 92 at RW = 4, 128 at RW = 6 (with the occupancy bound below),
+// 162 at RW = 8.  Eight wavefronts of 6 rows therefore hold 48-row windows
+// (61 % of the window exact at K = 6, against 51 % for 32 rows) at four
+// wavefronts per SIMD, where four of 8 rows hold 32-row windows at three.
 //
-template <int RW, int K>
-__global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
-  constexpr int R = 4 * RW;
+template <int RW, int NW>
+constexpr int adj_mw_min_waves() {
+  return (NW == 8 && RW == 6) ? 4 : 1;
+}
+
+// PF: the six planes of the chunk's next TX are loaded into a second register
+// set while the current TX is stepped (its dL/dtrace samples are issued
+// first, since the in-order vector-memory counter would otherwise make their
+// wait drain the prefetch); the set moves over at the next TX.  About 24
+// more VGPRs at RW = 4, 48 at RW = 8; results are bit-identical.
+template <int RW, int K, int NW, bool PF = false>
+__global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_rev_mw_kernel(
+    AdjTB2DArgs p) {
+  constexpr int R = NW * RW;
   constexpr int NO = R - 2 * K;
-  __shared__ float x_fy[4][64], x_es[4][64], x_q[4][64], x_my[4][64];
+  __shared__ float x_fy[NW][64], x_es[NW][64], x_q[NW][64], x_my[NW][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int task = xcd_block(p.xcd_order);
@@ -789,7 +802,7 @@ __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
   auto is_out = [&](int i) -> bool {                    // window row inside [K, R - K)
     return r0 + i >= K && r0 + i < R - K;
   };
-  const bool has_up = w > 0, has_down = w < 3;
+  const bool has_up = w > 0, has_down = w < NW - 1;
 
   float mca[RW], mica[RW], mcbi[RW];
   {
@@ -821,24 +834,38 @@ __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
   const int b_begin = chunk * p.bpw;
   const int b_end = min(b_begin + p.bpw, p.B);
   float le[RW], mx[RW], my[RW], fe[RW], fx[RW], fy[RW];
+  constexpr int NP = PF ? RW : 1;
+  float nle[NP], nmx[NP], nmy[NP], nfe[NP], nfx[NP], nfy[NP];
+  auto load6 = [&](int bt, float* dle, float* dmx, float* dmy, float* dfe, float* dfx,
+                   float* dfy) {
+    const long long base = (long long)bt * p.bstride;
+    const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_in + base, plane_bytes);
+    const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_in + base, plane_bytes);
+    const __amdgpu_buffer_rsrc_t r_my = plane_rsrc(p.my_in + base, plane_bytes);
+    const __amdgpu_buffer_rsrc_t r_fe = plane_rsrc(p.fe_in + base, plane_bytes);
+    const __amdgpu_buffer_rsrc_t r_fx = plane_rsrc(p.fhx_in + base, plane_bytes);
+    const __amdgpu_buffer_rsrc_t r_fy = plane_rsrc(p.fhy_in + base, plane_bytes);
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+      const unsigned o = row_off(i);
+      dle[i] = bload(r_le, o);
+      dmx[i] = bload(r_mx, o);
+      dmy[i] = bload(r_my, o);
+      dfe[i] = bload(r_fe, o);
+      dfx[i] = bload(r_fx, o);
+      dfy[i] = bload(r_fy, o);
+    }
+  };
+  if (PF && b_begin < b_end) load6(b_begin, le, mx, my, fe, fx, fy);
   for (int bt = b_begin; bt < b_end; ++bt) {
     const long long base = (long long)bt * p.bstride;
-    {
-      const __amdgpu_buffer_rsrc_t r_le = plane_rsrc(p.le_in + base, plane_bytes);
-      const __amdgpu_buffer_rsrc_t r_mx = plane_rsrc(p.mx_in + base, plane_bytes);
-      const __amdgpu_buffer_rsrc_t r_my = plane_rsrc(p.my_in + base, plane_bytes);
-      const __amdgpu_buffer_rsrc_t r_fe = plane_rsrc(p.fe_in + base, plane_bytes);
-      const __amdgpu_buffer_rsrc_t r_fx = plane_rsrc(p.fhx_in + base, plane_bytes);
-      const __amdgpu_buffer_rsrc_t r_fy = plane_rsrc(p.fhy_in + base, plane_bytes);
+    if (!PF) {
+      load6(bt, le, mx, my, fe, fx, fy);
+    } else if (bt > b_begin) {
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
-        const unsigned o = row_off(i);
-        le[i] = bload(r_le, o);
-        mx[i] = bload(r_mx, o);
-        my[i] = bload(r_my, o);
-        fe[i] = bload(r_fe, o);
-        fx[i] = bload(r_fx, o);
-        fy[i] = bload(r_fy, o);
+        le[i] = nle[i]; mx[i] = nmx[i]; my[i] = nmy[i];
+        fe[i] = nfe[i]; fx[i] = nfx[i]; fy[i] = nfy[i];
       }
     }
     const int sc = p.src_cell[bt];
@@ -859,6 +886,7 @@ __global__ __launch_bounds__(256) void adj2d_rev_mw_kernel(AdjTB2DArgs p) {
       gq[k] = (live && step >= 1 && pq)
                   ? p.dtr[((long long)bt * p.P + lane) * p.T + (step - 1)] : 0.f;
     }
+    if (PF && bt + 1 < b_end) load6(bt + 1, nle, nmx, nmy, nfe, nfx, nfy);
 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -988,22 +1016,56 @@ template <int RW>
 static hipError_t launch_rev_mw_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
   const dim3 blk(256, 1, 1), grd((unsigned)a.ntasks, 1, 1);
   switch (K) {
-    case 1: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 1>), grd, blk, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 2>), grd, blk, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 3>), grd, blk, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 4>), grd, blk, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 5>), grd, blk, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 1, 4>), grd, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 2, 4>), grd, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 3, 4>), grd, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 4, 4>), grd, blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 5, 4>), grd, blk, 0, s, a); break;
     case 6:
       if (RW < 8) return hipErrorInvalidValue;   // the window keeps at least 20 exact rows
-      hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 6>), grd, blk, 0, s, a);
+      hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 6, 4>), grd, blk, 0, s, a);
       break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
+// eight wavefronts: 32-row windows of 4 rows per wavefront (K <= 6) or
+// 48-row windows of 6 (K = 4..6); every launch depth below K is there too,
+// since a segment's last launch may take fewer steps only through ksteps
+template <int RW, bool PF>
+static hipError_t launch_rev_mw8_r(const AdjTB2DArgs& a, int K, hipStream_t s) {
+  const dim3 blk(512, 1, 1), grd((unsigned)a.ntasks, 1, 1);
+  switch (K) {
+    case 4: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 4, 8, PF>), grd, blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 5, 8, PF>), grd, blk, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((adj2d_rev_mw_kernel<RW, 6, 8, PF>), grd, blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// four wavefronts of 8 rows with the next TX prefetched (depth 6 only)
+static hipError_t launch_rev_mw4_pf(const AdjTB2DArgs& a, hipStream_t s) {
+  const dim3 blk(256, 1, 1), grd((unsigned)a.ntasks, 1, 1);
+  hipLaunchKernelGGL((adj2d_rev_mw_kernel<8, 6, 4, true>), grd, blk, 0, s, a);
+  return hipGetLastError();
+}
+
+bool adj2d_rev_mw_shape_ok(int R, int K, int NW, bool prefetch) {
+  if (NW == 4 && prefetch) return R == 32 && K == 6;
+  if (NW == 4) return (R == 16 || R == 24 || R == 32) && K >= 1 && K <= (R == 32 ? 6 : 5);
+  if (NW == 8) return (R == 32 || (R == 48 && !prefetch)) && K >= 4 && K <= 6;
+  return false;
+}
+
+hipError_t launch_adj2d_rev_mw(const AdjTB2DArgs& a, int R, int K, int NW, bool prefetch,
+                               hipStream_t s) {
   if (a.ntasks < 1) return hipSuccess;
+  if (!adj2d_rev_mw_shape_ok(R, K, NW, prefetch)) return hipErrorInvalidValue;
+  if (NW == 8 && prefetch) return launch_rev_mw8_r<4, true>(a, K, s);
+  if (NW == 8) return R == 32 ? launch_rev_mw8_r<4, false>(a, K, s) : launch_rev_mw8_r<6, false>(a, K, s);
+  if (prefetch) return launch_rev_mw4_pf(a, s);
   switch (R) {
     case 16: return launch_rev_mw_r<4>(a, K, s);
     case 24: return launch_rev_mw_r<6>(a, K, s);
@@ -1021,26 +1083,29 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
   return hipErrorInvalidValue;
 }
 
-// Four-wavefront form of step2d_tb_kernel.  One workgroup owns an R x 64
-// window (R = 4 RW) and each of its wavefronts RW consecutive rows.  A step
-// needs one row across each wavefront boundary twice -- the old E below (H
-// update) and the new hy above (E update) -- passed through LDS between two
-// workgroup barriers.  With a quarter of the rows per wavefront the window
-// can be tall (R = 48 or 64) and the blocking deep (K up to 8) within the
-// register file: the loads per output cell-step drop from
+// Multi-wavefront form of step2d_tb_kernel.  One workgroup owns an R x 64
+// window (R = NW RW, NW = 4 or 8 wavefronts) and each of its wavefronts RW
+// consecutive rows.  A step needs one row across each wavefront boundary
+// twice -- the old E below (H update) and the new hy above (E update) --
+// passed through LDS between two workgroup barriers.  With RW rows per
+// wavefront the window can be tall (R = 48 or 64) and the blocking deep (K up
+// to 8) within the register file: the loads per output cell-step drop from
 // (R/(R-2K) (64/(64-2K)) + 1)/K = 0.82 at R = 16, K = 4 to 0.38 at R = 48,
-// K = 8.  Per-cell arithmetic is that of step2d_tb_kernel.
-// One task (output tile, TX chunk) advanced p.ksteps steps by the four
+// K = 8, and the exactly computed fraction of the window at K = 6 rises from
+// (20/32)(52/64) = 51 % at R = 32 to 66 % at R = 64.  Eight wavefronts of 8
+// rows give 64-row windows at the register footprint of four of 8.
+// Per-cell arithmetic is that of step2d_tb_kernel.
+// One task (output tile, TX chunk) advanced p.ksteps steps by the NW
 // wavefronts of the calling workgroup (or of its group: the task's two
 // workgroup barriers per step then hold every group); x_e / x_h are its LDS
 // exchange rows.
-template <int RW, int K, int POLES, bool MID>
+template <int RW, int K, int POLES, bool MID, int NW = 4>
 __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, int w,
                                         float (*x_e)[64], float (*x_h)[64]) {
-  constexpr int R = 4 * RW;
-  const int tiles = p.tiles_x * p.tiles_y;
-  const int tile = task % tiles;
+  constexpr int R = NW * RW;
+  const int tiles = p.tile_list ? p.n_list : p.tiles_x * p.tiles_y;
   const int chunk = task / tiles;
+  const int tile = p.tile_list ? p.tile_list[task - chunk * tiles] : task - chunk * tiles;
   const int ty = tile % p.tiles_y;
   const int tx = tile / p.tiles_y;
   const int a0 = tx * (R - 2 * K) - K;                // row of window row 0
@@ -1056,7 +1121,7 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
     return (a >= 0 && a < p.Mx) ? lane_off + (unsigned)i * pitch4 : kOffNone;
   };
   auto is_out = [&](int i) -> bool { return r0 + i >= K && r0 + i < R - K; };
-  const bool has_up = w > 0, has_down = w < 3;
+  const bool has_up = w > 0, has_down = w < NW - 1;
 
   float mca[RW], mcb[RW], mk1[RW], mk2[RW];
   if (MID) {
@@ -1163,7 +1228,7 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
     const int sc = p.src_cell[bt];
     const int s_row = sc / p.pitch - (a0 + r0);        // local row
     const int s_lane = sc % p.pitch - cb0;
-    const bool src_here = s_row >= 0 && s_row < RW && s_lane >= 0 && s_lane < 64;
+    const bool src_here = !p.no_inject && s_row >= 0 && s_row < RW && s_lane >= 0 && s_lane < 64;
 
 #pragma unroll
     for (int s = 0; s < K; ++s) {
@@ -1282,14 +1347,14 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
   }
 }
 
-template <int RW, int K, int POLES, bool MID>
-__global__ __launch_bounds__(256) void step2d_mw_kernel(TB2DArgs p) {
-  __shared__ float x_e[4][64], x_h[4][64];
+template <int RW, int K, int POLES, bool MID, int NW>
+__global__ __launch_bounds__(64 * NW) void step2d_mw_kernel(TB2DArgs p) {
+  __shared__ float x_e[NW][64], x_h[NW][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int task = xcd_block(p.xcd_order);
   if (task >= p.ntasks) return;                       // uniform per workgroup
-  mw_task<RW, K, POLES, MID>(p, task, lane, w, x_e, x_h);
+  mw_task<RW, K, POLES, MID, NW>(p, task, lane, w, x_e, x_h);
 }
 
 // Uneven batch, one launch per round, four wavefronts per window: each
@@ -1460,15 +1525,15 @@ int persist_mw_blocks_per_cu(int poles, int R, int K, bool mat_ids) {
   return n;
 }
 
-template <int RW, int K>
+template <int RW, int K, int NW>
 static hipError_t launch_mw_rk(const TB2DArgs& a, int poles, hipStream_t s) {
-  const dim3 blk(256, 1, 1), grd((unsigned)a.ntasks, 1, 1);
+  const dim3 blk(64 * NW, 1, 1), grd((unsigned)a.ntasks, 1, 1);
   const bool ids = a.mat_id != nullptr;
-  if (poles == 0 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, true>), grd, blk, 0, s, a);
-  else if (poles == 0) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, false>), grd, blk, 0, s, a);
-  else if (poles == 1 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, true>), grd, blk, 0, s, a);
-  else if (poles == 1) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, false>), grd, blk, 0, s, a);
-  else if (poles == kPolesDiff) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, kPolesDiff, false>), grd, blk, 0, s, a);
+  if (poles == 0 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, true, NW>), grd, blk, 0, s, a);
+  else if (poles == 0) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, false, NW>), grd, blk, 0, s, a);
+  else if (poles == 1 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, true, NW>), grd, blk, 0, s, a);
+  else if (poles == 1) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, false, NW>), grd, blk, 0, s, a);
+  else if (poles == kPolesDiff) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, kPolesDiff, false, NW>), grd, blk, 0, s, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -1476,15 +1541,33 @@ static hipError_t launch_mw_rk(const TB2DArgs& a, int poles, hipStream_t s) {
 template <int RW>
 static hipError_t launch_mw_r(const TB2DArgs& a, int poles, int K, hipStream_t s) {
   switch (K) {
-    case 4: return launch_mw_rk<RW, 4>(a, poles, s);
-    case 6: return launch_mw_rk<RW, 6>(a, poles, s);
-    case 8: return launch_mw_rk<RW, 8>(a, poles, s);
+    case 4: return launch_mw_rk<RW, 4, 4>(a, poles, s);
+    case 6: return launch_mw_rk<RW, 6, 4>(a, poles, s);
+    case 8: return launch_mw_rk<RW, 8, 4>(a, poles, s);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_step2d_mw(const TB2DArgs& a, int poles, int R, int K, hipStream_t s) {
+bool step2d_mw_shape_ok(int R, int K, int NW) {
+  if (NW == 4) return (R == 32 || R == 48 || R == 64) && (K == 4 || K == 6 || K == 8);
+  if (NW == 8) return (R == 32 && K == 4) || (R == 48 && (K == 4 || K == 6)) ||
+                      (R == 64 && (K == 4 || K == 6 || K == 8));
+  return false;
+}
+
+hipError_t launch_step2d_mw(const TB2DArgs& a, int poles, int R, int K, int NW, hipStream_t s) {
   if (a.ntasks < 1) return hipSuccess;
+  if (!step2d_mw_shape_ok(R, K, NW)) return hipErrorInvalidValue;
+  if (NW == 8) {
+    if (R == 32) return launch_mw_rk<4, 4, 8>(a, poles, s);
+    if (R == 48) return K == 4 ? launch_mw_rk<6, 4, 8>(a, poles, s) : launch_mw_rk<6, 6, 8>(a, poles, s);
+    switch (K) {
+      case 4: return launch_mw_rk<8, 4, 8>(a, poles, s);
+      case 6: return launch_mw_rk<8, 6, 8>(a, poles, s);
+      case 8: return launch_mw_rk<8, 8, 8>(a, poles, s);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (R) {
     case 32: return launch_mw_r<8>(a, poles, K, s);
     case 48: return launch_mw_r<12>(a, poles, K, s);

@@ -22,6 +22,7 @@
 #include "fdtd2d_kernels.h"
 #include "fdtd3d_kernels.h"
 #include "reduce_kernels.h"
+#include "misfit.h"
 #include "imaging.h"
 #include "coef2d.h"
 #include "limits2d.h"
@@ -209,6 +210,15 @@ struct ceep_fdtd {
   float *ca_ref = nullptr, *cb_ref = nullptr, *d_a = nullptr, *d_b = nullptr;
   unsigned char* diff_tile = nullptr;
   int diff_tile_n = 0;
+  // the flagged tiles and the others as lists ([n_diff] then [n_plain] ints):
+  // each blocked step runs the difference kernel over the flagged tiles on
+  // stream2 while the plain kernel (no source, 80 VGPRs against 127) steps
+  // the rest on stream; events ev_plain / ev_diff join the two per step
+  int* tile_lists = nullptr;
+  int n_diff_tiles = 0, n_plain_tiles = 0;
+  int split_diff = 1;          // tuning key 23: 0 = one difference kernel over every tile
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_plain = nullptr, ev_diff = nullptr;
   float* e_ref = nullptr;      // (T + 1) slots of [B][plane]: E_ref^s
   std::vector<float*> ref_ck;  // (nck + 1) x 3: reference state every ref_every steps
   std::vector<float> ref_traces;   // [B][P][T]
@@ -236,13 +246,21 @@ struct ceep_fdtd {
   int adj_r = 0;               // 2D blocked adjoint: window rows (0: 32 for the four-wavefront
                                // reversible kernel, 16 for the one-wavefront kernels)
   int adj_bpw = 8;             // 2D blocked adjoint: batch planes per wavefront
-  int adj_waves = 4;           // 2D reversible adjoint: wavefronts per window (1 or 4)
+  int adj_waves = 4;           // 2D reversible adjoint: wavefronts per window (1, 4 or 8)
+  int adj_prefetch = 0;        // 2D reversible adjoint: next TX's planes loaded ahead (key 24)
+  // device S-misfit of ceep_fdtd_gradient_misfit: FP64 twiddles [T][F] for
+  // misfit_freqs, the misfit D [F][P][B], S_target, and (loss, dL/dtrace bound)
+  double2* tw64 = nullptr;
+  std::vector<double> misfit_freqs;
+  double2* misfit_d = nullptr;
+  double2* misfit_tgt = nullptr;
+  double* misfit_out = nullptr;
   float* pin_dtr = nullptr;    // pinned host staging: dL/dtrace (FP32) ...
   float* pin_g = nullptr;      // ... and the two gradient planes
   size_t pin_dtr_n = 0, pin_g_n = 0;
   int adj_k = 0;               // 2D blocked adjoint: reversed steps per launch (0: 6 for the
                                // four-wavefront reversible kernel on 32-row windows, 4 otherwise)
-  int fwd_waves = 1;           // 2D blocked forward: wavefronts per window (1 or 4)
+  int fwd_waves = 1;           // 2D blocked forward: wavefronts per window (1, 4 or 8)
   int xcd3d = 0;               // 3D H/E kernels: block order (0 dispatch, 1/2 per XCD)
   int tb3d_k = 2;              // 3D blocked form: steps per launch (1 or 2)
   int tx_group3d = 4;          // 3D blocked form, block order 3: TX per group
@@ -317,6 +335,10 @@ struct ceep_fdtd {
     if (g_cbi) (void)hipFree(g_cbi);
     if (d_dtr) (void)hipFree(d_dtr);
     g_ca = g_cbi = d_dtr = nullptr;
+    for (double2** p : {&tw64, &misfit_d, &misfit_tgt}) { if (*p) (void)hipFree(*p); *p = nullptr; }
+    if (misfit_out) (void)hipFree(misfit_out);
+    misfit_out = nullptr;
+    misfit_freqs.clear();
     if (pin_dtr) (void)hipHostFree(pin_dtr);
     if (pin_g) (void)hipHostFree(pin_g);
     pin_dtr = pin_g = nullptr;
@@ -328,6 +350,9 @@ struct ceep_fdtd {
     for (float** p : {&e_ref, &ca_ref, &cb_ref, &d_a, &d_b}) { if (*p) (void)hipFree(*p); *p = nullptr; }
     if (diff_tile) (void)hipFree(diff_tile);
     diff_tile = nullptr;
+    if (tile_lists) (void)hipFree(tile_lists);
+    tile_lists = nullptr;
+    n_diff_tiles = n_plain_tiles = 0;
     diff_tile_n = 0;
     ref_traces.clear();
     ref_dirty = true;
@@ -655,25 +680,59 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     a.src_cell = h->src2; a.wave = h->wave; a.wave_stride_b = h->wave_stride;
     a.T = h->T; a.P = h->P; a.probe_cell = h->probe_cell; a.traces = h->traces;
     const int K = h->tb_k, R = h->tb_r;
-    const bool shape_ok = h->fwd_waves == 4
-        ? (R == 32 || R == 48 || R == 64) && (K == 4 || K == 6 || K == 8)
-        : (R == 16 || R == 24 || R == 32) && K >= 1 && K <= 4;
+    const bool mw = h->fwd_waves == 4 || h->fwd_waves == 8;
+    const bool shape_ok = mw ? ceep::step2d_mw_shape_ok(R, K, h->fwd_waves)
+                             : (R == 16 || R == 24 || R == 32) && K >= 1 && K <= 4;
     if (!shape_ok)
       return fail(CEEP_EINVAL, "2D blocked launch shape: one wavefront per window takes 16, 24 or 32 "
-                               "rows and depth 1-4; four take 32, 48 or 64 rows and depth 4, 6 or 8");
+                               "rows and depth 1-4; four take 32, 48 or 64 rows and depth 4, 6 or 8; "
+                               "eight take 32 rows at depth 4, 48 at 4 or 6, 64 at 4, 6 or 8");
     a.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
     a.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
     a.bpw = h->tb_bpw;
     a.ntasks = a.tiles_x * a.tiles_y * ((h->B + a.bpw - 1) / a.bpw);
     int kpoles = h->poles();
     if (h->diff_active) {   // E - E_ref with the reference run's source term
-      if (h->fwd_waves != 4 || kpoles != 0 || !h->e_ref || !h->diff_tile ||
+      if (!mw || kpoles != 0 || !h->e_ref || !h->diff_tile ||
           h->diff_tile_n != a.tiles_x * a.tiles_y)
         return fail(CEEP_ESTATE, "difference form: needs the four-wavefront blocked forward, "
                                  "non-dispersive media and a reference run for this launch shape");
       kpoles = ceep::kPolesDiff;
       a.d_a = h->d_a; a.d_b = h->d_b; a.e_ref = h->e_ref; a.ref_slot = h->box_floats();
       a.diff_tile = h->diff_tile;
+    }
+    // difference form (key 23): 1 = one difference launch over the tiles
+    // with the flagged ones first (they take longest and start earliest),
+    // 2 = the flagged tiles with the difference kernel on stream2 and the
+    // others with the plain kernel (no source) on stream, 3 = the same two
+    // launches one after the other on stream
+    const bool lists = kpoles == ceep::kPolesDiff && h->tile_lists && !e_save &&
+                       h->n_diff_tiles > 0 && h->n_plain_tiles > 0;
+    const bool split = lists && (h->split_diff == 2 || h->split_diff == 3);
+    const bool two_streams = split && h->split_diff == 2;
+    ceep::TB2DArgs pa = a;
+    const int chunks = (h->B + a.bpw - 1) / a.bpw;
+    if (lists && h->split_diff == 1) {
+      a.tile_list = h->tile_lists;
+      a.n_list = h->n_diff_tiles + h->n_plain_tiles;
+    }
+    if (split) {
+      a.tile_list = h->tile_lists;
+      a.n_list = h->n_diff_tiles;
+      a.ntasks = h->n_diff_tiles * chunks;
+      pa.tile_list = h->tile_lists + h->n_diff_tiles;
+      pa.n_list = h->n_plain_tiles;
+      pa.ntasks = h->n_plain_tiles * chunks;
+      pa.no_inject = 1;
+      pa.mat_id = nullptr;   // the difference form steps with the full maps
+      pa.d_a = pa.d_b = nullptr;
+      pa.e_ref = nullptr;
+      pa.diff_tile = nullptr;
+      // stream2 starts after everything already queued on stream
+      if (two_streams) {
+        CEEP_HIP(hipEventRecord(h->ev_plain, s));
+        CEEP_HIP(hipStreamWaitEvent(h->stream2, h->ev_plain, 0));
+      }
     }
     int cur = start;
     for (int st = step0; st < step_end; st += K) {
@@ -685,10 +744,26 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       a.ksteps = std::min(K, step_end - st);
       a.slot = h->box_floats();
       a.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() : nullptr;
-      if (h->fwd_waves == 4)
-        CEEP_HIP(ceep::launch_step2d_mw(a, kpoles, R, K, s));
-      else
+      if (split) {
+        // both launches read set cur and write set o; each waits for the
+        // other's launch of the step before (which read o and wrote cur)
+        pa.ez_in = a.ez_in; pa.hx_in = a.hx_in; pa.hy_in = a.hy_in;
+        pa.ez_out = a.ez_out; pa.hx_out = a.hx_out; pa.hy_out = a.hy_out;
+        pa.jd_in = a.jd_in; pa.jd_out = a.jd_out;
+        pa.step0 = a.step0; pa.ksteps = a.ksteps; pa.slot = a.slot; pa.e_save = nullptr;
+        CEEP_HIP(ceep::launch_step2d_mw(a, kpoles, R, K, h->fwd_waves, two_streams ? h->stream2 : s));
+        CEEP_HIP(ceep::launch_step2d_mw(pa, 0, R, K, h->fwd_waves, s));
+        if (two_streams) {
+          CEEP_HIP(hipEventRecord(h->ev_diff, h->stream2));
+          CEEP_HIP(hipEventRecord(h->ev_plain, s));
+          CEEP_HIP(hipStreamWaitEvent(s, h->ev_diff, 0));
+          CEEP_HIP(hipStreamWaitEvent(h->stream2, h->ev_plain, 0));
+        }
+      } else if (mw) {
+        CEEP_HIP(ceep::launch_step2d_mw(a, kpoles, R, K, h->fwd_waves, s));
+      } else {
         CEEP_HIP(ceep::launch_step2d_tb(a, kpoles, R, K, s));
+      }
       cur = o;
     }
   } else if (h->cpml() && h->ndim == 2) {
@@ -1121,6 +1196,10 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   dfree(h->mat_id); dfree(h->mat_table);
   dfree(h->ref_eps64); dfree(h->ref_sig64);
   for (int k = 0; k < 12; ++k) dfree(h->psi[k]);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+  if (h->ev_plain) (void)hipEventDestroy(h->ev_plain);
+  if (h->ev_diff) (void)hipEventDestroy(h->ev_diff);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CEEP_OK;
@@ -1794,6 +1873,43 @@ namespace {
 // all T steps with this handle's sources, probes and launch shape, keeping
 // E after every step (slot s of e_ref holds E_ref^s; slot 0 = 0), the state
 // at every multiple of `every` and the traces.
+int adjoint2d_device(ceep_fdtd* h, double scale, double* grad_eps_r, double* grad_sigma);
+
+// The difference form's tile lists: the tiles diff_tile flags, then the
+// others (host round trip of ntiles bytes), and the second stream and the
+// two events the split launches synchronise with (created once).
+int build_tile_lists(ceep_fdtd* h, int ntiles) {
+  hipStream_t st = h->stream;
+  std::vector<unsigned char> flags((size_t)ntiles);
+  CEEP_HIP(hipMemcpyAsync(flags.data(), h->diff_tile, (size_t)ntiles, hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipStreamSynchronize(st));
+  std::vector<int> lists;
+  lists.reserve((size_t)ntiles);
+  for (int t = 0; t < ntiles; ++t)
+    if (flags[t]) lists.push_back(t);
+  const int nd = (int)lists.size();
+  for (int t = 0; t < ntiles; ++t)
+    if (!flags[t]) lists.push_back(t);
+  if (h->tile_lists && (h->n_diff_tiles + h->n_plain_tiles) != ntiles) {
+    dfree(h->tile_lists);
+    h->n_diff_tiles = h->n_plain_tiles = 0;
+  }
+  int rc;
+  if (!h->tile_lists && (rc = dalloc(&h->tile_lists, (size_t)ntiles))) return rc;
+  CEEP_HIP(hipMemcpyAsync(h->tile_lists, lists.data(), (size_t)ntiles * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+  CEEP_HIP(hipStreamSynchronize(st));
+  h->n_diff_tiles = nd;
+  h->n_plain_tiles = ntiles - nd;
+  if (!h->stream2) {
+    hipError_t e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
+    if (e != hipSuccess) { h->stream2 = nullptr; return fail(CEEP_EHIP, "second stream"); }
+  }
+  if (!h->ev_plain) CEEP_HIP(hipEventCreateWithFlags(&h->ev_plain, hipEventDisableTiming));
+  if (!h->ev_diff) CEEP_HIP(hipEventCreateWithFlags(&h->ev_diff, hipEventDisableTiming));
+  return CEEP_OK;
+}
+
 int run_reference(ceep_fdtd* h, int every) {
   int rc;
   hipStream_t st = h->stream;
@@ -1941,8 +2057,8 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
   if (!h->has_ref) return fail(CEEP_ESTATE, "no reference medium (ceep_fdtd_set_reference)");
   if (h->ndim != 2 || h->cpml() || h->poles())
     return fail(CEEP_EUNSUPPORTED, "the difference form covers 2D non-dispersive cavity runs");
-  if (!h->use_rev() || h->fwd_waves != 4 || h->fwd64)
-    return fail(CEEP_EUNSUPPORTED, "the difference form needs the four-wavefront blocked FP32 "
+  if (!h->use_rev() || (h->fwd_waves != 4 && h->fwd_waves != 8) || h->fwd64)
+    return fail(CEEP_EUNSUPPORTED, "the difference form needs the multi-wavefront blocked FP32 "
                                    "forward and the reversible adjoint");
   if (every < 1) return fail(CEEP_EINVAL, "checkpoint interval must be >= 1");
   hipStream_t st = h->stream;
@@ -1963,6 +2079,7 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
   }
   CEEP_HIP(ceep::launch_diff_tiles(h->d_a, h->d_b, h->M[0], h->M[1], h->pitch, R, K, tx, ty,
                                    h->diff_tile, st));
+  if ((rc = build_tile_lists(h, tx * ty))) return rc;
   const int nck = (h->T + every - 1) / every;
   h->ck_valid = false;
   if (h->ck.size() != (size_t)(nck + 1) * 3) {
@@ -2014,8 +2131,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   if (!h->ck_valid) return fail(CEEP_ESTATE, "no checkpointed forward run (ceep_fdtd_run_checkpointed)");
   if (h->ndim == 3) return adjoint3d(h, dtraces, grad_eps_r, grad_sigma);
   int rc;
-  const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
-  const size_t nbox = (size_t)h->box_floats(), bytes = nbox * sizeof(float);
+  const int B = h->B, P = h->P, T = h->T;
   // dL/dtrace is linear in the adjoint: scale it by a power of two so the
   // FP32 adjoint fields stay far from the subnormal range, unscale at the end
   double mx = 0.0;
@@ -2028,12 +2144,111 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   if ((rc = pinned(&h->pin_dtr, &h->pin_dtr_n, ntr))) return rc;
   float* dtr = h->pin_dtr;
   for (size_t i = 0; i < ntr; ++i) dtr[i] = (float)(dtraces[i] * scale);
+  if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
+  CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr, ntr * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  return adjoint2d_device(h, scale, grad_eps_r, grad_sigma);
+}
+
+int ceep_fdtd_get_reference_traces(ceep_fdtd* h, double* out) {
+  if (!h || !out) return fail(CEEP_EINVAL, "null argument");
+  if (h->ref_traces.empty()) return fail(CEEP_ESTATE, "no reference run yet");
+  for (size_t i = 0; i < h->ref_traces.size(); ++i) out[i] = h->ref_traces[i];
+  return CEEP_OK;
+}
+
+int ceep_fdtd_gradient_misfit(ceep_fdtd* h, int every, const double* freqs, int n_freq,
+                              const double* s_target, double* loss_out, double* grad_eps_r,
+                              double* grad_sigma) {
+  if (!h || !freqs || !loss_out || !grad_eps_r || !grad_sigma || n_freq < 1)
+    return fail(CEEP_EINVAL, "bad argument");
+  if (h->ndim != 2) return fail(CEEP_EUNSUPPORTED, "the fused misfit gradient covers 2D handles");
+  if (h->fwd64) return fail(CEEP_EUNSUPPORTED, "the fused misfit gradient runs the FP32 forward "
+                                               "(FP64 forward: ceep_fdtd_adjoint)");
+  if (h->has_ref && s_target)
+    return fail(CEEP_EINVAL, "with a reference medium the residual is S - S_target itself: "
+                             "pass no S_target");
+  if (!h->has_ref && !s_target)
+    return fail(CEEP_EINVAL, "give S_target or set a reference medium");
+  int rc = h->has_ref ? ceep_fdtd_run_residual(h, every, nullptr, nullptr)
+                      : ceep_fdtd_run_checkpointed(h, every, nullptr);
+  if (rc) return rc;
+  const int B = h->B, P = h->P, T = h->T, F = n_freq;
+  const size_t nf = (size_t)h->full_cells, nd = (size_t)F * P * B;
+  hipStream_t st = h->stream;
+  if (P == 0) {
+    *loss_out = 0.0;
+    std::memset(grad_eps_r, 0, nf * sizeof(double));
+    std::memset(grad_sigma, 0, nf * sizeof(double));
+    return CEEP_OK;
+  }
+  // FP64 twiddles, cached per frequency list
+  if (h->misfit_freqs.size() != (size_t)F ||
+      !std::equal(freqs, freqs + F, h->misfit_freqs.begin())) {
+    std::vector<double2> tw((size_t)T * F);
+    for (int s = 0; s < T; ++s)
+      for (int f = 0; f < F; ++f) {
+        const double ph = 2.0 * M_PI * freqs[f] * (s * h->dt);
+        tw[(size_t)s * F + f] = make_double2(std::cos(ph), -std::sin(ph));
+      }
+    dfree(h->tw64);
+    dfree(h->misfit_d);
+    dfree(h->misfit_tgt);
+    h->misfit_freqs.clear();
+    if ((rc = dalloc(&h->tw64, tw.size()))) return rc;
+    if ((rc = dalloc(&h->misfit_d, nd))) return rc;
+    CEEP_HIP(hipMemcpyAsync(h->tw64, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    CEEP_HIP(hipStreamSynchronize(st));
+    h->misfit_freqs.assign(freqs, freqs + F);
+  }
+  if (!h->misfit_out && (rc = dalloc(&h->misfit_out, 2))) return rc;
+  const double2* tgt = nullptr;
+  if (s_target) {
+    if (!h->misfit_tgt && (rc = dalloc(&h->misfit_tgt, nd))) return rc;
+    CEEP_HIP(hipMemcpyAsync(h->misfit_tgt, s_target, nd * sizeof(double2), hipMemcpyHostToDevice, st));
+    tgt = h->misfit_tgt;
+  }
+  CEEP_HIP(ceep::launch_misfit_s(h->traces, h->tw64, B, P, T, F, h->dt, tgt, h->misfit_d, st));
+  CEEP_HIP(ceep::launch_misfit_sum(h->misfit_d, B, P, F, h->dt, h->misfit_out, st));
+  double out[2] = {0.0, 0.0};
+  CEEP_HIP(hipMemcpyAsync(out, h->misfit_out, sizeof(out), hipMemcpyDeviceToHost, st));
+  CEEP_HIP(hipStreamSynchronize(st));
+  *loss_out = out[0];
+  if (!(out[1] > 0.0)) {   // the misfit vanishes: so does its gradient
+    std::memset(grad_eps_r, 0, nf * sizeof(double));
+    std::memset(grad_sigma, 0, nf * sizeof(double));
+    return CEEP_OK;
+  }
+  if (!std::isfinite(out[1])) return fail(CEEP_EINVAL, "the misfit is not finite");
+  // dL/dtrace scaled by a power of two at or below 1 / its bound, as
+  // ceep_fdtd_adjoint scales the host's dL/dtrace
+  const double scale = std::ldexp(1.0, -std::ilogb(out[1]));
+  const size_t ntr = (size_t)B * P * T;
+  if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
+  CEEP_HIP(ceep::launch_misfit_grad(h->misfit_d, h->tw64, B, P, T, F, h->dt, scale, h->d_dtr, st));
+  return adjoint2d_device(h, scale, grad_eps_r, grad_sigma);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The 2D adjoint from dL/dtrace already in h->d_dtr (FP32, multiplied by
+// `scale`, a power of two): reversed segments, gradient sums, chain rule,
+// and the two gradient maps copied to the host.
+int adjoint2d_device(ceep_fdtd* h, double scale, double* grad_eps_r, double* grad_sigma) {
+  int rc;
+  const int B = h->B, P = h->P, T = h->T, every = h->ck_every;
+  const size_t nbox = (size_t)h->box_floats(), bytes = nbox * sizeof(float);
   const bool rev = h->ck_rev && h->use_rev();
-  const bool mw_rev = rev && h->adj_waves == 4;
+  const bool mw_rev = rev && (h->adj_waves == 4 || h->adj_waves == 8);
   const int adj_k = h->adj_k ? h->adj_k
-                             : (mw_rev ? ((h->adj_r == 0 || h->adj_r == 32) ? 6 : 5) : 4);
+                             : (mw_rev ? ((h->adj_r == 0 || h->adj_r >= 32) ? 6 : 5) : 4);
   if (h->use_tb() && adj_k > 4 && !mw_rev)
-    return fail(CEEP_EINVAL, "adjoint temporal depth above 4 needs the four-wavefront reversible adjoint");
+    return fail(CEEP_EINVAL, "adjoint temporal depth above 4 needs the multi-wavefront reversible adjoint");
+  if (mw_rev && !ceep::adj2d_rev_mw_shape_ok(h->adj_r ? h->adj_r : 32, adj_k, h->adj_waves,
+                                              h->adj_prefetch))
+    return fail(CEEP_EINVAL, "reversible adjoint shape: four wavefronts take 16 or 24 rows at depth "
+                             "1-5 and 32 at 1-6; eight take 32 or 48 rows at depth 4-6");
   if (!rev && !h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * nbox))) return rc;
   for (int i = 0; i < 2; ++i)
     for (int f = 0; f < 3; ++f)
@@ -2042,9 +2257,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   const int nchunks = h->use_tb() ? (B + h->adj_bpw - 1) / h->adj_bpw : 1;
   if (!h->g_ca && (rc = dalloc(&h->g_ca, (size_t)B * h->plane))) return rc;
   if (!h->g_cbi && (rc = dalloc(&h->g_cbi, (size_t)B * h->plane))) return rc;
-  if (!h->d_dtr && (rc = dalloc(&h->d_dtr, ntr))) return rc;
   hipStream_t st = h->stream;
-  CEEP_HIP(hipMemcpyAsync(h->d_dtr, dtr, ntr * sizeof(float), hipMemcpyHostToDevice, st));
   for (int f = 0; f < 3; ++f) CEEP_HIP(hipMemsetAsync(h->adj[0][f], 0, bytes, st));
   CEEP_HIP(hipMemsetAsync(h->g_ca, 0, (size_t)nchunks * h->plane * sizeof(float), st));
   CEEP_HIP(hipMemsetAsync(h->g_cbi, 0, (size_t)nchunks * h->plane * sizeof(float), st));
@@ -2064,7 +2277,7 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
     // checkpoint, or the final state) and step forward and adjoint back together
     const int s0 = k * every, n = std::min(every, T - s0);
     const int K = adj_k;
-    const int R = h->adj_r ? h->adj_r : (h->adj_waves == 4 ? 32 : 16);
+    const int R = h->adj_r ? h->adj_r : (mw_rev ? 32 : 16);
     ceep::AdjTB2DArgs t{};
     t.xcd_order = h->xcd_order;
     t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
@@ -2090,8 +2303,8 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
       first = false;
       t.step_hi = hi;
       t.ksteps = std::min(K, hi - s0);
-      if (h->adj_waves == 4)
-        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, st));
+      if (mw_rev)
+        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, h->adj_waves, h->adj_prefetch, st));
       else
         CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, st));
       acur = o;
@@ -2178,6 +2391,10 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   return CEEP_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   if (!h) return fail(CEEP_EINVAL, "null handle");
@@ -2198,14 +2415,15 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
     h->tb_bpw = value;
   } else if (key == 10) {
-    if (value != 16 && value != 24 && value != 32)
-      return fail(CEEP_EINVAL, "adjoint rows per wavefront must be 16, 24 or 32");
+    if (value != 16 && value != 24 && value != 32 && value != 48)
+      return fail(CEEP_EINVAL, "adjoint window rows must be 16, 24, 32 or 48");
     h->adj_r = value;
   } else if (key == 11) {
     if (value < 1 || value > 1024) return fail(CEEP_EINVAL, "adjoint batch planes per wavefront must be >= 1");
     h->adj_bpw = value;
   } else if (key == 14) {
-    if (value != 1 && value != 4) return fail(CEEP_EINVAL, "adjoint wavefronts per window must be 1 or 4");
+    if (value != 1 && value != 4 && value != 8)
+      return fail(CEEP_EINVAL, "adjoint wavefronts per window must be 1, 4 or 8");
     h->adj_waves = value;
   } else if (key == 15) {
     if (value < 1 || value > 6)
@@ -2218,6 +2436,12 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 0 || value > 2)
       return fail(CEEP_EINVAL, "3D adjoint steps per launch must be 0 (three-kernel form), 1 or 2");
     h->adj3d_k = value;
+  } else if (key == 23) {
+    if (value < 0 || value > 3) return fail(CEEP_EINVAL, "difference-form tile order must be 0..3");
+    h->split_diff = value;
+  } else if (key == 24) {
+    if (value != 0 && value != 1) return fail(CEEP_EINVAL, "adjoint prefetch must be 0 or 1");
+    h->adj_prefetch = value;
   } else if (key == 22) {
     if (value < 0 || value > 2) return fail(CEEP_EINVAL, "material ids must be 0, 1 or 2");
     if (h->mat_ids_on != value) { h->mat_ids_on = value; h->mats_dirty = true; }
@@ -2225,7 +2449,8 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
     if (value < 0 || value > 2) return fail(CEEP_EINVAL, "3D adjoint block order must be 0, 1 or 2");
     h->adj3d_order = value;
   } else if (key == 16) {
-    if (value != 1 && value != 4) return fail(CEEP_EINVAL, "forward wavefronts per window must be 1 or 4");
+    if (value != 1 && value != 4 && value != 8)
+      return fail(CEEP_EINVAL, "forward wavefronts per window must be 1, 4 or 8");
     h->fwd_waves = value;
   } else if (key == 12) {
     if (value < 0 || value > 3) return fail(CEEP_EINVAL, "3D block order must be 0, 1, 2 or 3");

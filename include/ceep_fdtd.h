@@ -178,6 +178,25 @@ int ceep_fdtd_set_reference(ceep_fdtd* h, const double* eps_r, const double* sig
  * after ceep_fdtd_run_checkpointed.  No counterpart in the reference. */
 int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double* reference_out);
 
+/* The whole gradient of L = sum over (f, rx, tx) of |S - S_target|^2 at
+ * n_freq frequencies (2D, non-dispersive, FP32 forward) in one call, with
+ * no trace or dL/dtrace round trip through the host: the checkpointed forward
+ * (with a reference medium set: ceep_fdtd_run_residual, whose residual is
+ * S - S_target itself and s_target must be NULL; without one:
+ * ceep_fdtd_run_checkpointed, s_target = (n_freq, n_probes, batch) complex
+ * as interleaved float64 pairs), the S-matrix of the traces, the loss and
+ * dL/dtrace on the device in FP64 sums, then the adjoint as ceep_fdtd_adjoint.
+ * loss_out gets L; grad_eps_r and grad_sigma (full grid, float64) dL/d eps_r
+ * and dL/d sigma.  No counterpart in the reference (no gradient path; the
+ * S-matrix is that of src/ceep/solvers/dft.py:75-90, s_params.py:70-99). */
+int ceep_fdtd_gradient_misfit(ceep_fdtd* h, int every, const double* freqs, int n_freq,
+                              const double* s_target, double* loss_out, double* grad_eps_r,
+                              double* grad_sigma);
+
+/* The reference medium's traces (batch, n_probes, total_steps) float64 from
+ * its last run (ceep_fdtd_run_residual's reference_out). */
+int ceep_fdtd_get_reference_traces(ceep_fdtd* h, double* out);
+
 /* Tuning knobs (keys 0-4 give the same results to FP32 rounding for every setting):
  *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8; the
  *          3D one-step kernels use at most 4, the 3D adjoint kernels all four)
@@ -213,20 +232,21 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
  *   key 9: 2D blocked kernels: 1 = renumber workgroups so each XCD runs a
  *          contiguous range of tiles (neighbouring tiles share an L2; default),
  *          0 = dispatch order.  Results are identical.
- *   key 10, 11: window rows (16, 24, 32; default 32 for the four-wavefront
- *          reversible kernel, 16 otherwise) and batch planes per window
+ *   key 10, 11: window rows (16, 24, 32, 48; default 32 for the multi-
+ *          wavefront reversible kernel, 16 otherwise) and batch planes per window
  *          (default 8) of the 2D blocked adjoint, which holds twice the
  *          forward's state per cell and prefers other shapes than the forward
  *          (keys 3, 4).
- *   key 14: 2D reversible adjoint: wavefronts per window, 1 or 4 (default 4:
- *          each wavefront holds a quarter of the window's rows, halo rows
- *          pass through LDS).
+ *   key 14: 2D reversible adjoint: wavefronts per window, 1, 4 or 8
+ *          (default 4: each wavefront holds a quarter of the window's rows,
+ *          halo rows pass through LDS; 8 takes 32- or 48-row windows).
  *   key 15: 2D blocked adjoint: reversed steps per launch, 1..6 (5 and 6
- *          only with the four-wavefront reversible kernel, 6 only on 32-row
- *          windows; default 6 there, 5 on smaller windows of that kernel, 4
- *          otherwise).
- *   key 16: 2D blocked forward: wavefronts per window, 1 (default) or 4.
- *          Results are bit-identical between the two forms.
+ *          only with the multi-wavefront reversible kernel, 6 only on 32- or
+ *          48-row windows; default 6 there, 5 on smaller windows of that
+ *          kernel, 4 otherwise; eight wavefronts take 4..6).
+ *   key 16: 2D blocked forward: wavefronts per window, 1 (default), 4 or 8
+ *          (eight: 32 rows at depth 4, 48 at 4 or 6, 64 at 4, 6 or 8).
+ *          Results are bit-identical between the forms.
  *   key 12: 3D workgroup order: 0 = dispatch order (default; blocked form:
  *          y tiles fastest, then z tiles, then TX), 1 = the same renumbered so
  *          each XCD runs a contiguous range, 2 = per XCD with the TX (batch
@@ -252,7 +272,15 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
  *          the 2D blocked forward, residual and uneven-round kernels and the
  *          3D blocked adjoint, 2 = also the 3D blocked forward.  Media with
  *          more records, Lorentz poles in 2D, mu_r / sigma_m maps in 3D and
- *          the physical-CPML mode use the maps.  Results are bit-identical. */
+ *          the physical-CPML mode use the maps.  Results are bit-identical.
+ *   key 23: difference form (ceep_fdtd_run_residual), tile order of each
+ *          blocked step: 0 = the difference kernel over every tile in
+ *          order; 1 (default) = the same with the tiles whose window meets
+ *          cells where the media differ dispatched first; 2 = those tiles
+ *          with the difference kernel on a second stream and the others
+ *          with the plain kernel (no source term), joined by events each
+ *          step; 3 = the same two launches one after the other.  Results
+ *          are identical up to the plain kernel's rounding of the update. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,

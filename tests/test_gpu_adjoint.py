@@ -151,12 +151,12 @@ def test_adjoint_kernel_variants(amd, reference, tuning):
     assert ee < 1e-3 and es < 1e-3
 
 
-def test_adjoint_depth5_needs_four_wavefront_form(amd, reference):
+def test_adjoint_depth5_needs_multi_wavefront_form(amd, reference):
     from ceep_amd import adjoint
     r = reference
     s = _solver(amd, r["dt"], r["w"], r["pos"])
     s.tuning = {"temporal_depth": 4, "adjoint_waves": 1, "adjoint_temporal_depth": 5}
-    with pytest.raises(ValueError, match="four-wavefront"):
+    with pytest.raises(ValueError, match="multi-wavefront"):
         adjoint.gradient(s, r["eps"], r["sig"], FREQS, r["St"], every=45)
 
 

@@ -166,5 +166,55 @@ def test_residual_refuses_other_forms(amd):
     with pytest.raises(RuntimeError, match="reference medium"):
         s._handle.run_residual(50)
     s._handle.set_reference(c["eps_t"], c["sig_t"])
-    with pytest.raises(NotImplementedError, match="four-wavefront"):
+    with pytest.raises(NotImplementedError, match="multi-wavefront"):
         s._handle.run_residual(50)
+
+
+@pytest.mark.parametrize("form", ["difference", "s_target"])
+def test_fused_gradient_matches_autograd_path(amd, form):
+    """``ceep_fdtd_gradient_misfit`` (S-matrix, loss and dL/dtrace on the
+    device in FP64 sums) against the autograd path (traces to the host, the
+    torch misfit, dL/dtrace back): the same loss and gradients to FP32
+    rounding of dL/dtrace, for the difference form and for S_target given as
+    data; the traces and S-matrix read afterwards are the same as well."""
+    from ceep_amd import adjoint
+    c = _config1_case(amd)
+    freqs = np.linspace(1e9, 2.5e9, 16)
+    s = _solver(amd, c["n"], c["T"], c["ncp"], c["pos"], c["w"], c["f0"])
+    if form == "difference":
+        kw = dict(target=(c["eps_t"], c["sig_t"]))
+    else:
+        args = (c["n"], c["n"], DX, c["dt"], c["ncp"], c["pos"], c["pos"], c["w"])
+        St = dft.running_dft(fdtd2d.run_cavity(*args, c["eps_t"], c["sig_t"]), freqs, c["dt"])
+        kw = dict(s_target=St)
+    L0, ge0, gs0 = adjoint.gradient(s, c["eps"], c["sig"], freqs, every=50, fused=False, **kw)
+    tr0 = np.array(s.probe_data)
+    S0 = s.s_parameters(freqs).s_matrix
+    L1, ge1, gs1 = adjoint.gradient(s, c["eps"], c["sig"], freqs, every=50, **kw)
+    tr1 = np.array(s.probe_data)
+    S1 = s.s_parameters(freqs).s_matrix
+    print(f"\n{form}: loss {L1:.9e} vs {L0:.9e}; dL/deps rel-L2 {rel_l2(ge1, ge0):.2e}, "
+          f"dL/dsigma {rel_l2(gs1, gs0):.2e}")
+    assert L1 == pytest.approx(L0, rel=1e-9)
+    assert rel_l2(ge1, ge0) < 1e-5 and rel_l2(gs1, gs0) < 1e-5
+    assert np.array_equal(tr1, tr0)
+    assert np.allclose(S1, S0, rtol=0, atol=1e-12 * np.abs(S0).max())
+    # zero misfit: zero gradient, no adjoint run
+    if form == "difference":
+        L2, ge2, gs2 = adjoint.gradient(s, c["eps"], c["sig"], freqs, every=50,
+                                        target=(c["eps"], c["sig"]))
+        assert L2 == 0.0 and not ge2.any() and not gs2.any()
+
+
+def test_fused_gradient_rejects_bad_arguments(amd):
+    c = _config1_case(amd)
+    s = _solver(amd, c["n"], c["T"], c["ncp"], c["pos"], c["w"], c["f0"])
+    s._build()
+    freqs = np.linspace(1e9, 2.5e9, 4)
+    with pytest.raises(ValueError, match="S_target or set a reference"):
+        s._handle.gradient_misfit(50, freqs)
+    with pytest.raises(ValueError, match="S_target must have shape"):
+        s._handle.gradient_misfit(50, freqs, np.zeros((4, 8, 7), dtype=complex))
+    s._handle.set_reference(c["eps_t"], c["sig_t"])
+    with pytest.raises(ValueError, match="pass no S_target"):
+        s._handle.gradient_misfit(50, freqs, np.zeros((4, 8, 8), dtype=complex))

@@ -280,6 +280,8 @@ def _adjoint_ratio(every=50):
     t_res, _ = timed(lambda: h.run_residual(every))
     t_grad, (loss, ge, gs) = timed(lambda: adjoint.gradient(s, eps, sig, freqs, every=every,
                                                             target=(eps_t, sig_t)))
+    t_grad_ag, _ = timed(lambda: adjoint.gradient(s, eps, sig, freqs, every=every,
+                                                  target=(eps_t, sig_t), fused=False))
     worst, L64, St64 = _fp64_fd(solver, dt, freqs, eps, sig, eps_t, sig_t, ge, gs)
     t_grad64, _ = timed(lambda: adjoint.gradient(s, eps, sig, freqs, St64, every=every,
                                                  fp64_forward=True))
@@ -291,8 +293,12 @@ def _adjoint_ratio(every=50):
             "loss": float(f"{loss:.6e}"), "loss_fp64": float(f"{L64:.6e}"),
             "ms_target_run_once": round((t_first - t_res) * 1e3, 3),
             "ratio_fp64_forward": round(t_grad64 / t_fwd, 3),
+            "ratio_autograd_path": round(t_grad_ag / t_fwd, 3),
             "note": ("ratio: the FP32 difference form (forward of E(eps) - E(target) with "
-                     "checkpoints every 50 steps, reversible adjoint, chain rule on the device); "
+                     "checkpoints every 50 steps, reversible adjoint, chain rule on the device) "
+                     "in one library call, S-matrix, loss and dL/dtrace on the device; "
+                     "ratio_autograd_path: the same gradient through torch.autograd with the "
+                     "misfit on the host (the path for other losses); "
                      "fd_worst_rel_dev: that gradient against FP64 central differences "
                      "(h = 1e-4, 2 eps_r + 2 sigma pixels) from the engine's FP64 forward; "
                      "ms_target_run_once: the target's own run with its E kept at every step, "
@@ -523,16 +529,32 @@ def _bench_3d_gradient(T3=1000, every=100, single=False, tuning=None):
         adjoint.gradient(s, eps, sig, freqs, S_t, every=every)
         _device_sync()
         t_grad = time.perf_counter() - t0
+        slots, extra, _ = h.store_info()
+        # the same with the E store of one segment (every segment but the last
+        # recomputed in full)
+        h.set_tuning("estore_extra_3d", 0)
+        adjoint.gradient(s, eps, sig, freqs, S_t, every=every)       # re-sizes the store
+        _device_sync()
+        t0 = time.perf_counter()
+        adjoint.gradient(s, eps, sig, freqs, S_t, every=every)
+        _device_sync()
+        t_grad1 = time.perf_counter() - t0
+        h.set_tuning("estore_extra_3d", -1)
     except Exception as exc:   # e.g. the E store does not fit
         s.close()
         return {"error": f"{type(exc).__name__}: {exc}"}
     box = h.box_info()
     cells = int(box[0] * box[1] * box[2])
     s.close()
-    gb = (every + 1) * 3 * cells * 16 * 4 / 1e9
+    gb = slots * 3 * cells * 16 * 4 / 1e9
     return {"ratio": round(t_grad / t_fwd, 3), "ms_forward": round(t_fwd * 1e3, 1),
             "ms_forward_plus_adjoint": round(t_grad * 1e3, 1), "checkpoint_every": every,
-            "e_store_gb": round(gb, 1),
+            "e_store_gb": round(gb, 1), "e_store_slots": slots, "e_store_extra_slots": extra,
+            "ratio_one_segment_store": round(t_grad1 / t_fwd, 3),
+            "note": ("ratio: the E store takes what free HBM allows (e_store_extra_slots "
+                     "beyond one segment), so the segment before the last is recomputed "
+                     "only in part; ratio_one_segment_store: every segment but the last "
+                     "recomputed in full"),
             "workload": ("3D 200^3 Debye+Lorentz head seed 0, 16 TX/RX, 1000 steps; "
                          "L = sum |S - S_target|^2 over 101 freqs 0.5-3 GHz vs head seed 1; "
                          "dL/deps_r and dL/dsigma over the full volume")}
@@ -579,9 +601,17 @@ def _traffic(kernel, summary="step2d_debye_600x600x16_traffic.json"):
         return None
     with open(path) as f:
         d = json.load(f)
-    if "kernels" in d:
-        return (d["kernels"].get(kernel) or {}).get("bytes_per_launch")
-    return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
+    # summaries taken before the window kernels named their wavefront count
+    # list the four-wavefront kernels without it (same code)
+    names = [kernel]
+    if kernel.startswith("step2d_mw_kernel<") and kernel.endswith(", 4>"):
+        names.append(kernel[:-4] + ">")
+    for name in names:
+        if "kernels" in d and name in d["kernels"]:
+            return d["kernels"][name].get("bytes_per_launch")
+        if d.get("kernel") == name:
+            return d.get("bytes_per_launch")
+    return None
 
 
 def main():
@@ -670,10 +700,11 @@ def main():
     box = h.box_info()
     cells = int(box[0] * box[1])
     K = (tuning or {}).get("temporal_depth", 0)
-    if K and (tuning or {}).get("forward_waves", 1) == 4:
+    nw = (tuning or {}).get("forward_waves", 1)
+    if K and nw in (4, 8):
         # the head phantom is piecewise constant: material ids unless switched off
         ids = "true" if (tuning or {}).get("material_ids", 1) else "false"
-        kernel = f"step2d_mw_kernel<{tuning['rows_per_wave'] // 4}, {K}, 1, {ids}>"
+        kernel = f"step2d_mw_kernel<{tuning['rows_per_wave'] // nw}, {K}, 1, {ids}, {nw}>"
         launches = -(-T // K)
         steps_per_launch = K
     elif K:

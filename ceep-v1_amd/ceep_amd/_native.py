@@ -62,6 +62,7 @@ SIGNATURES = {
     "ceep_fdtd_run_residual": (_I, [_P, _I, _PD, _PD]),
     "ceep_fdtd_gradient_misfit": (_I, [_P, _I, _PD, _I, _PD, _PD, _PD, _PD]),
     "ceep_fdtd_get_reference_traces": (_I, [_P, _PD]),
+    "ceep_fdtd_store_info": (_I, [_P, _PI64]),
     "ceep_fdtd_set_tuning": (_I, [_P, _I, _I]),
     "ceep_fdtd_multi_create": (_I, [_I, ctypes.POINTER(_P), _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "ceep_fdtd_multi_run": (_I, [_P, _I]),
@@ -151,6 +152,7 @@ TUNE_ADJ3D_ORDER = 20
 TUNE_MATERIAL_IDS = 22
 TUNE_SPLIT_DIFF = 23
 TUNE_ADJ_PREFETCH = 24
+TUNE_ESTORE_EXTRA_3D = 25
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
@@ -164,7 +166,8 @@ TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "adjoint_temporal_depth_3d": TUNE_ADJ3D_DEPTH,
                "adjoint_order_3d": TUNE_ADJ3D_ORDER, "material_ids": TUNE_MATERIAL_IDS,
                "split_difference": TUNE_SPLIT_DIFF,
-               "adjoint_prefetch": TUNE_ADJ_PREFETCH}
+               "adjoint_prefetch": TUNE_ADJ_PREFETCH,
+               "estore_extra_3d": TUNE_ESTORE_EXTRA_3D}
 
 
 class Handle:
@@ -301,6 +304,12 @@ class Handle:
         gs = np.zeros(self.shape, dtype=np.float64)
         check(self._lib.ceep_fdtd_adjoint(self._h, dptr(g), dptr(ge), dptr(gs)))
         return ge, gs
+
+    def store_info(self):
+        """(E-store slots, of them beyond one segment, checkpoints)."""
+        out = np.zeros(3, dtype=np.int64)
+        check(self._lib.ceep_fdtd_store_info(self._h, iptr(out)))
+        return tuple(int(v) for v in out)
 
     def reference_traces(self):
         """(B, P, T) traces of the reference medium's run (the difference form)."""

@@ -21,8 +21,8 @@ _LOADED = False
 
 # measured best on MI355X for the 600^2 x 16 Debye workload; used when nothing
 # was tuned for a shape
-DEFAULT_2D = {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2,
-              "batch_per_thread": 4, "graph": 1, "xcd_order": 1, "forward_waves": 4}
+DEFAULT_2D = {"temporal_depth": 6, "rows_per_wave": 64, "batch_per_wave": 2,
+              "batch_per_thread": 4, "graph": 1, "xcd_order": 1, "forward_waves": 8}
 
 
 def _key(ndim, shape, n, batch, debye, lorentz):
@@ -63,6 +63,11 @@ def candidates_2d(lorentz=False):
         for k, r, bpw in ((4, 32, 1), (4, 32, 2), (4, 32, 4), (6, 32, 2), (4, 48, 2), (6, 48, 2),
                           (8, 64, 2)):
             out.append({"forward_waves": 4, "temporal_depth": k, "rows_per_wave": r,
+                        "batch_per_wave": bpw})
+        # eight wavefronts of 8 (6) rows: 64- (48-) row windows at the register
+        # footprint of the four-wavefront 32-row form
+        for k, r, bpw in ((6, 64, 2), (6, 64, 4), (8, 64, 2), (6, 48, 2)):
+            out.append({"forward_waves": 8, "temporal_depth": k, "rows_per_wave": r,
                         "batch_per_wave": bpw})
     return out
 

@@ -752,9 +752,11 @@ hipError_t launch_adj2d_rev_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s
 // wavefronts RW consecutive rows.  Each reversed step needs one row across
 // every wavefront boundary four times -- the old hy above (forward E), the
 // new E below (forward H), q below (adjoint H) and the new mu_y above (lam)
-// -- and passes them through LDS between three workgroup barriers.  The
-// arithmetic per cell is that of adj2d_rev_tb_kernel; with the same R and K
-// the results are bit-identical.  A wavefront's registers grow by about 17
+// -- and passes them through LDS: E and q at one workgroup barrier, mu_y
+// together with the hy of the next reversed step at a second, so a launch
+// of K reversed steps per TX takes 2K + 1 barriers.  The
+// arithmetic per cell is that of adj2d_rev_tb_kernel; the results agree to
+// FP32 rounding (instantiations may fuse different multiply-adds).  A wavefront's registers grow by about 17
 // VGPRs per row (six fields, three maps, two gradient sums and the shifted
 // neighbours): 92 at RW = 4, 128 at RW = 6 (with the occupancy bound below),
 // 162 at RW = 8.  Eight wavefronts of 6 rows therefore hold 48-row windows
@@ -888,20 +890,22 @@ __global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_r
     }
     if (PF && bt + 1 < b_end) load6(bt + 1, nle, nmx, nmy, nfe, nfx, nfy);
 
+    // forward E one step back needs the forward hy of the row above this
+    // wavefront's first row, from the wavefront above: for the first reversed
+    // step here, for later ones with mu_y at the step before's second barrier
+    x_fy[w][lane] = fy[RW - 1];
+    lds_barrier();
+    float fy_up = has_up ? x_fy[w - 1][lane] : 0.f;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (k >= p.ksteps) break;                        // uniform per workgroup
       const int step = p.step_hi - 1 - k;
+      const bool more = k + 1 < p.ksteps;              // uniform
       if (src_rows) {
 #pragma unroll
         for (int i = 0; i < RW; ++i)
           if (i == s_row) fe[i] -= src_lane ? wsrc[k] : 0.f;
       }
-      // forward E one step back: the row above this wavefront's first row
-      // comes from the wavefront above (its current hy)
-      x_fy[w][lane] = fy[RW - 1];
-      lds_barrier();
-      const float fy_up = has_up ? x_fy[w - 1][lane] : 0.f;
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const float hxl = from_prev_lane(fx[i]);
@@ -937,10 +941,13 @@ __global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_r
         mx[i] = mx[i] + (qR - q[i]);
         my[i] = my[i] + (q[i] - qD);
       }
-      // adjoint E step: the new mu_y above comes from the wavefront above
+      // adjoint E step: the new mu_y above comes from the wavefront above,
+      // and with it the forward hy the next reversed step starts from
       x_my[w][lane] = my[RW - 1];
+      if (more) x_fy[w][lane] = fy[RW - 1];
       lds_barrier();
       const float my_up = has_up ? x_my[w - 1][lane] : 0.f;
+      if (more) fy_up = has_up ? x_fy[w - 1][lane] : 0.f;
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const float mxL = from_prev_lane(mx[i]);

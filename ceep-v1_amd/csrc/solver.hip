@@ -145,6 +145,7 @@ struct ceep_fdtd {
   // Lorentz J x, y, z; Lorentz J one step older x, y, z; absent poles omitted)
   float* mpack = nullptr;
   float* jpack = nullptr;
+  size_t mpack_n = 0, jpack_n = 0;   // their float counts (kept across material changes)
   int nj() const { return (has_debye ? 3 : 0) + (has_lorentz ? 6 : 0); }
   int* src2 = nullptr;
   long long* src3 = nullptr;
@@ -182,6 +183,17 @@ struct ceep_fdtd {
   float* g3tb = nullptr;
   bool ck_valid = false;
   int estore_seg = -1;         // segment whose forward E the store holds, or -1
+  int estore_off = 0;          // ... from this slot on (3D: estore_x for the tail segment)
+  // 3D: extra E-store slots.  The store holds every + 1 + estore_x slots; the
+  // checkpointed forward leaves the last segment in slots [0, every] and the
+  // top estore_x steps of the segment before it (estore_tail) behind them, at
+  // slot estore_x + j for its step j, so that segment's adjoint recomputes only
+  // its first every - estore_x steps (into slots estore_x + [0, m], freed by
+  // the last segment).  estore_extra (tuning key 25): -1 = as many as the free
+  // device memory holds after what the adjoint still allocates, else that many.
+  int estore_x = 0;
+  int estore_tail = -1;
+  int estore_extra = -1;
 
   // FP64 forward of the gradient path (tuning key 5): fields, maps, waveform,
   // traces and checkpoints in doubles; the adjoint itself stays FP32
@@ -326,6 +338,9 @@ struct ceep_fdtd {
     if (estore) (void)hipFree(estore);
     estore = nullptr;
     estore_seg = -1;
+    estore_off = 0;
+    estore_x = 0;
+    estore_tail = -1;
     for (auto& row : adj)
       for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
     for (auto& row : adj3)
@@ -557,9 +572,12 @@ int commit_materials(ceep_fdtd* h) {
         (rc = upload_f32(h->lk3, lk3, s)))
       return rc;
   }
-  // 3D blocked form: packed maps, and the ADE currents packed per cell in two sets
-  dfree(h->mpack);
-  dfree(h->jpack);
+  // 3D blocked form: packed maps, and the ADE currents packed per cell in two
+  // sets (buffers of an unchanged size are kept)
+  const size_t want_mp = h->tb3d() ? (size_t)nbox * ceep::map_pack(h->poles(), !da.empty()).n : 0;
+  const size_t want_jp = h->tb3d() ? 2 * (size_t)h->nj() * h->box_floats() : 0;
+  if (want_mp != h->mpack_n) { dfree(h->mpack); h->mpack_n = 0; }
+  if (want_jp != h->jpack_n) { dfree(h->jpack); h->jpack_n = 0; }
   if (h->tb3d()) {
     // only the groups the kernels read for this material set (no room for
     // absent poles or, without mu_r / sigma_m maps, for da and dbi)
@@ -572,12 +590,14 @@ int commit_materials(ceep_fdtd* h) {
       if (h->has_lorentz) { q[mk.lk] = lk1[o]; q[mk.lk + 1] = lk2[o]; q[mk.lk + 2] = lk3[o]; }
       if (!da.empty()) { q[mk.mag] = da[o]; q[mk.mag + 1] = dbi[o]; }
     }
-    if ((rc = dalloc(&h->mpack, mp.size())) || (rc = upload_f32(h->mpack, mp, s))) return rc;
+    if (!h->mpack && (rc = dalloc(&h->mpack, mp.size()))) return rc;
+    h->mpack_n = mp.size();
+    if ((rc = upload_f32(h->mpack, mp, s))) return rc;
     CEEP_HIP(hipStreamSynchronize(s));   // mp goes out of scope
-    const size_t nj = (size_t)h->nj();
-    if (nj) {
-      if ((rc = dalloc(&h->jpack, 2 * nj * h->box_floats()))) return rc;
-      CEEP_HIP(hipMemsetAsync(h->jpack, 0, 2 * nj * h->box_floats() * sizeof(float), s));
+    if (want_jp) {
+      if (!h->jpack && (rc = dalloc(&h->jpack, want_jp))) return rc;
+      h->jpack_n = want_jp;
+      CEEP_HIP(hipMemsetAsync(h->jpack, 0, want_jp * sizeof(float), s));
     }
   }
   // ADE current buffers follow the pole set (the blocked 3D form uses jpack)
@@ -1211,6 +1231,16 @@ int ceep_fdtd_set_materials(ceep_fdtd* h, const double* eps_r, const double* sig
   for (long long i = 0; i < h->full_cells; ++i)
     if (!(eps_r[i] > 0) || !std::isfinite(sigma_e[i]))
       return fail(CEEP_EINVAL, "eps_r must be > 0 and sigma_e finite everywhere");
+  // the same maps again (e.g. a gradient at the medium the handle holds):
+  // nothing to rebuild, and the coefficient maps stay as they are
+  const size_t nb = (size_t)h->full_cells * sizeof(double);
+  auto same = [&](const std::vector<double>& have, const double* in) {
+    return have.size() == (size_t)h->full_cells && std::memcmp(have.data(), in, nb) == 0;
+  };
+  const bool mag_in = h->ndim == 3 && (mu_r || sigma_m);
+  if (same(h->eps_r, eps_r) && same(h->sigma_e, sigma_e) && !mag_in && !h->magnetic &&
+      h->mu_r.empty())
+    return CEEP_OK;
   h->eps_r.assign(eps_r, eps_r + h->full_cells);
   h->sigma_e.assign(sigma_e, sigma_e + h->full_cells);
   h->magnetic = false;
@@ -1607,10 +1637,41 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
   if (every != h->ck_every) dfree(h->estore);
   h->ck_every = every;
   h->estore_seg = -1;
-  if (!h->estore && (rc = dalloc(&h->estore, (size_t)(every + 1) * 3 * nbox))) {
-    h->free_adjoint();
-    return rc;
+  h->estore_off = 0;
+  h->estore_tail = -1;
+  const size_t slot = (size_t)3 * nbox;
+  if (!h->estore) {
+    int x = 0;
+    if (nck >= 2 && h->adjtb3d()) {
+      if (h->estore_extra >= 0) {
+        x = std::min(h->estore_extra, every - 1);
+      } else {
+        // what the blocked adjoint still allocates, plus a margin
+        const size_t nj = (size_t)h->nj();
+        const size_t later = ((h->adjtb ? 0 : 12) + (h->adjtb_j ? 0 : 2 * nj) + (h->g3tb ? 0 : 2)) *
+                                 (size_t)nbox * sizeof(float) +
+                             (size_t)h->B * h->P * h->T * sizeof(float) + (size_t(4) << 30);
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+          const size_t need = (size_t)(every + 1) * slot * sizeof(float) + later;
+          if (fr > need) x = (int)std::min<size_t>((fr - need) / (slot * sizeof(float)), (size_t)every - 1);
+        }
+      }
+    }
+    if (dalloc(&h->estore, (size_t)(every + 1 + x) * slot) != CEEP_OK) {
+      x = 0;
+      if ((rc = dalloc(&h->estore, (size_t)(every + 1) * slot))) {
+        h->free_adjoint();
+        return rc;
+      }
+    }
+    h->estore_x = x;
   }
+  // the tail segment's first m steps run without the store (m a multiple of
+  // the launch depth, so the launches fall as in one range)
+  const int kd = h->tb3d() ? h->tb3d_k : 1;
+  const int m = h->estore_x > 0 ? (every - h->estore_x + kd - 1) / kd * kd : every;
+  const bool tail = nck >= 2 && m < every;
   if ((rc = zero_state(h))) return rc;
   CEEP_HIP(hipMemsetAsync(h->traces, 0, (size_t)h->B * h->P * h->T * sizeof(float), st));
   h->traces64_valid = false;
@@ -1626,6 +1687,15 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
         CEEP_HIP(ceep::launch_copy(live[c], h->estore + (size_t)c * nbox, nbox, st));
       es = h->estore;
     }
+    if (tail && k == nck - 2) {
+      // E after step s0 + j goes to slot estore_x + j for j > m
+      if ((rc = launch_range(h, cur, s0, m, nullptr))) return rc;
+      cur = range_end(h, cur, m);
+      if ((rc = launch_range(h, cur, s0 + m, n - m, h->estore + (size_t)(h->estore_x + m) * slot)))
+        return rc;
+      cur = range_end(h, cur, n - m);
+      continue;
+    }
     if ((rc = launch_range(h, cur, s0, n, es))) return rc;
     cur = range_end(h, cur, n);
   }
@@ -1634,6 +1704,7 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
   h->has_run = true;
   h->ck_valid = true;
   h->estore_seg = nck - 1;
+  h->estore_tail = tail ? nck - 2 : -1;
   if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
   return CEEP_OK;
 }
@@ -1761,13 +1832,15 @@ int adjoint3d_abc(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<float
     // form's sets carry their roles, so it restarts from set 0
     const int par = h->tb3d() ? 0 : (s0 & 1);
     const std::vector<float*> live = state3d(h, par);
-    if (k != h->estore_seg) {   // recompute the segment from its checkpoint, storing E
+    // (this form reads segments from slot 0 only)
+    if (k != h->estore_seg || h->estore_off != 0) {   // recompute the segment, storing E
       for (size_t i = 0; i < nstate; ++i)
         CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + i], live[i], nbox, st));
       for (int c = 0; c < 3; ++c)
         CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + c], h->estore + (size_t)c * nbox, nbox, st));
       if ((rc = launch_range(h, par, s0, n, h->estore))) return rc;
       h->estore_seg = k;
+      h->estore_off = 0;
     }
     for (int s = s0 + n - 1; s >= s0; --s) {
       a.e_s = h->estore + (size_t)(s - s0) * slot;
@@ -1832,22 +1905,31 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
   const size_t nstate = state3d(h, 0).size();
   const size_t slot = (size_t)3 * nbox;
   int set = 0;
+  const int kd = h->tb3d_k;
   for (int k = h->ck_count - 1; k >= 0; --k) {
     const int s0 = k * every, n = std::min(every, T - s0);
     if (k != h->estore_seg) {   // recompute the segment from its checkpoint, storing E
+      // the tail segment: only its first m steps, into the slots after estore_x
+      // (the forward pass left the rest behind slot every)
+      const bool tail = k == h->estore_tail && h->estore_x > 0;
+      const int m = tail ? (every - h->estore_x + kd - 1) / kd * kd : n;
+      const int off = tail ? h->estore_x : 0;
+      float* base = h->estore + (size_t)off * slot;
       const std::vector<float*> live = state3d(h, 0);
       for (size_t i = 0; i < nstate; ++i)
         CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + i], live[i], nbox, st));
       for (int c = 0; c < 3; ++c)
-        CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + c], h->estore + (size_t)c * nbox, nbox, st));
-      if ((rc = launch_range(h, 0, s0, n, h->estore))) return rc;
+        CEEP_HIP(ceep::launch_copy(h->ck[(size_t)k * nstate + c], base + (size_t)c * nbox, nbox, st));
+      if ((rc = launch_range(h, 0, s0, m, base))) return rc;
       h->estore_seg = k;
+      h->estore_off = off;
     }
+    const float* ebase = h->estore + (size_t)h->estore_off * slot;
     for (int s = s0 + n - 1; s >= s0;) {
       const int K = std::min(h->adj3d_k, s - s0 + 1);
       t.in_set = set;
       t.step = s;
-      t.e_lo = h->estore + (size_t)(s - K + 1 - s0) * slot;
+      t.e_lo = ebase + (size_t)(s - K + 1 - s0) * slot;
       t.tiles_y = (t.My - 1 + (16 - 2 * K) - 1) / (16 - 2 * K);
       t.tiles_z = (t.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K);
       CEEP_HIP(ceep::launch_adj3d_tb(t, K, h->poles(), st));
@@ -2149,6 +2231,14 @@ int ceep_fdtd_adjoint(ceep_fdtd* h, const double* dtraces, double* grad_eps_r,
   return adjoint2d_device(h, scale, grad_eps_r, grad_sigma);
 }
 
+int ceep_fdtd_store_info(ceep_fdtd* h, int64_t* out) {
+  if (!h || !out) return fail(CEEP_EINVAL, "null argument");
+  out[0] = h->estore ? (int64_t)h->ck_every + 1 + (h->ndim == 3 ? h->estore_x : 0) : 0;
+  out[1] = h->ndim == 3 ? h->estore_x : 0;
+  out[2] = h->ck_count;
+  return CEEP_OK;
+}
+
 int ceep_fdtd_get_reference_traces(ceep_fdtd* h, double* out) {
   if (!h || !out) return fail(CEEP_EINVAL, "null argument");
   if (h->ref_traces.empty()) return fail(CEEP_ESTATE, "no reference run yet");
@@ -2442,6 +2532,17 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 24) {
     if (value != 0 && value != 1) return fail(CEEP_EINVAL, "adjoint prefetch must be 0 or 1");
     h->adj_prefetch = value;
+  } else if (key == 25) {
+    if (value < -1) return fail(CEEP_EINVAL, "3D extra E-store slots must be -1 (auto) or >= 0");
+    if (value != h->estore_extra) {
+      h->estore_extra = value;
+      dfree(h->estore);   // re-sized at the next checkpointed run
+      h->estore_seg = -1;
+      h->estore_off = 0;
+      h->estore_x = 0;
+      h->estore_tail = -1;
+      h->ck_valid = false;
+    }
   } else if (key == 22) {
     if (value < 0 || value > 2) return fail(CEEP_EINVAL, "material ids must be 0, 1 or 2");
     if (h->mat_ids_on != value) { h->mat_ids_on = value; h->mats_dirty = true; }

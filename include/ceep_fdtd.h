@@ -197,6 +197,10 @@ int ceep_fdtd_gradient_misfit(ceep_fdtd* h, int every, const double* freqs, int 
  * its last run (ceep_fdtd_run_residual's reference_out). */
 int ceep_fdtd_get_reference_traces(ceep_fdtd* h, double* out);
 
+/* Gradient-path storage: out[0] = E-store slots (0 when there is none),
+ * out[1] = of them beyond one segment (3D, tuning key 25), out[2] = checkpoints. */
+int ceep_fdtd_store_info(ceep_fdtd* h, int64_t* out);
+
 /* Tuning knobs (keys 0-4 give the same results to FP32 rounding for every setting):
  *   key 0: batch planes per thread of the one-step kernels (1, 2, 4, 8; the
  *          3D one-step kernels use at most 4, the 3D adjoint kernels all four)
@@ -280,7 +284,18 @@ int ceep_fdtd_get_reference_traces(ceep_fdtd* h, double* out);
  *          with the difference kernel on a second stream and the others
  *          with the plain kernel (no source term), joined by events each
  *          step; 3 = the same two launches one after the other.  Results
- *          are identical up to the plain kernel's rounding of the update. */
+ *          are identical up to the plain kernel's rounding of the update.
+ *   key 24: 2D reversible adjoint: 1 = load the next TX's six planes while
+ *          the current one is stepped (four wavefronts: 32 rows, depth 6;
+ *          eight: 32 rows, depth 4-6); 0 (default) = load each TX when it
+ *          starts.  Results are identical.
+ *   key 25: 3D gradient (blocked adjoint): extra E-store slots beyond one
+ *          segment.  The checkpointed forward also keeps E of the last
+ *          `extra` steps of the segment before the last, so that segment's
+ *          adjoint recomputes only its first every - extra steps.  -1
+ *          (default) = as many as the free device memory holds after what the
+ *          adjoint still allocates and a 4 GiB margin (at most every - 1),
+ *          >= 0 = that many.  Results are identical. */
 int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value);
 
 /* CPML profile parameters of a CEEP_BOX_CPML handle (defaults: order 4,

@@ -314,3 +314,64 @@ def test_blocked_3d_many_materials_use_the_maps(amd):
     for o in out[1:]:
         for a, b in zip(out[0], o):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("every", [17, 30])
+def test_blocked_3d_gradient_extra_store_slots(amd, every):
+    """E-store slots beyond one segment (key 25): the checkpointed forward also
+    keeps E of the top `extra` steps of the segment before the last, whose
+    adjoint then recomputes only the rest.  None (0), an odd count (the
+    recomputed part rounds up to whole two-step launches), most of a segment
+    and the automatic size give bit-identical gradients; a second adjoint
+    call on the same checkpoints (the kept steps still in place) as well."""
+    kinds = ("debye", "lorentz")
+    tun = [{"fused_3d": 4, "batch_per_thread": 1, "estore_extra_3d": x}
+           for x in (0, 3, every - 2, -1)]
+    res = _gradients(amd, (26, 24, 70), 3, 75, every, kinds, False, tun)
+    for (L, ge, gs), t in zip(res[1:], tun[1:]):
+        print(f"  extra={t['estore_extra_3d']}: bitwise "
+              f"{np.array_equal(ge, res[0][1]) and np.array_equal(gs, res[0][2])}")
+        assert L == res[0][0]
+        assert np.array_equal(ge, res[0][1]) and np.array_equal(gs, res[0][2])
+    # two adjoint calls after one checkpointed run
+    dx = 1e-3
+    dt = W.dt_3d_courant(dx, 0.5)
+    w = W.modulated_gaussian(75, dt, 30e9, 40e9, delay_factor=3.0)
+    eps, sig, m = _maps((26, 24, 70), 0)
+    s = _solver(amd, (26, 24, 70), 3, 75, dx, [(13, 12, 35)], [(16, 10, 39), (9, 15, 29)], w,
+                eps, sig, m, kinds, None, None)
+    s.tuning = {"fused_3d": 4, "batch_per_thread": 1, "estore_extra_3d": 5}
+    s._build()
+    h = s._handle
+    tr = h.run_checkpointed(every)
+    g = np.random.default_rng(3).standard_normal(tr.shape)
+    a1 = h.adjoint(g)
+    a2 = h.adjoint(g)
+    assert np.array_equal(a1[0], a2[0]) and np.array_equal(a1[1], a2[1])
+    assert np.abs(a1[0]).max() > 0
+
+
+def test_unchanged_materials_are_not_rebuilt(amd):
+    """Setting the maps a handle already holds leaves it as it is (no host
+    rebuild of the coefficient maps); changed maps still take effect."""
+    dx = 1e-3
+    dt = W.dt_3d_courant(dx, 0.5)
+    w = W.modulated_gaussian(40, dt, 30e9, 40e9, delay_factor=3.0)
+    eps, sig, m = _maps((26, 24, 30), 0)
+    s = _solver(amd, (26, 24, 30), 3, 40, dx, [(13, 12, 15)], [(16, 10, 17)], w, eps, sig, m,
+                ("debye",), None, None)
+    s.tuning = {"fused_3d": 4}
+    s.run()
+    t0 = s.probe_data.copy()
+    s.update_materials(eps.copy(), sig.copy())
+    s._handle.reset()
+    s.run()
+    assert np.array_equal(s.probe_data, t0)
+    s.update_materials(eps * 1.5, sig)
+    s._handle.reset()
+    s.run()
+    assert not np.array_equal(s.probe_data, t0)
+    s.update_materials(eps, sig)
+    s._handle.reset()
+    s.run()
+    assert np.array_equal(s.probe_data, t0)

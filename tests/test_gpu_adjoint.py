@@ -135,6 +135,15 @@ def test_fp64_forward_blocked_equals_one_step(amd, reference, every):
     {"temporal_depth": 6, "forward_waves": 4, "rows_per_wave": 32, "batch_per_wave": 2,
      "adjoint_waves": 4, "adjoint_temporal_depth": 6, "adjoint_batch_per_wave": 5},
     {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_temporal_depth": 5},
+    {"temporal_depth": 6, "forward_waves": 8, "rows_per_wave": 64, "batch_per_wave": 3,
+     "adjoint_waves": 8, "adjoint_rows_per_wave": 32, "adjoint_batch_per_wave": 5},
+    {"temporal_depth": 4, "forward_waves": 8, "rows_per_wave": 48, "batch_per_wave": 2,
+     "adjoint_waves": 8, "adjoint_rows_per_wave": 48, "adjoint_temporal_depth": 4,
+     "adjoint_batch_per_wave": 3},
+    {"temporal_depth": 4, "adjoint_waves": 8, "adjoint_rows_per_wave": 32, "adjoint_prefetch": 1,
+     "adjoint_temporal_depth": 5, "adjoint_batch_per_wave": 7},
+    {"temporal_depth": 4, "adjoint_waves": 4, "adjoint_rows_per_wave": 32, "adjoint_prefetch": 1,
+     "adjoint_temporal_depth": 6, "adjoint_batch_per_wave": 3},
 ])
 def test_adjoint_kernel_variants(amd, reference, tuning):
     """One-step and blocked kernels, reversible (default) and recompute + E store

@@ -57,7 +57,8 @@ def _oracle(s, dt, w, m, debye):
 # shapes include the default forward (K = 6, R = 32, 2 planes)
 CASES = [(1, 16, 1, 1), (2, 24, 2, 1), (3, 32, 1, 1), (4, 16, 1, 1), (4, 24, 3, 1),
          (4, 32, 2, 1), (2, 16, 8, 1),
-         (4, 32, 1, 4), (6, 32, 2, 4), (6, 32, 3, 4), (8, 48, 2, 4), (6, 64, 1, 4), (4, 48, 3, 4)]
+         (4, 32, 1, 4), (6, 32, 2, 4), (6, 32, 3, 4), (8, 48, 2, 4), (6, 64, 1, 4), (4, 48, 3, 4),
+         (6, 64, 2, 8), (4, 32, 3, 8), (6, 48, 1, 8), (8, 64, 2, 8), (4, 64, 5, 8)]
 
 
 @pytest.mark.parametrize("debye", [False, True])
@@ -177,6 +178,14 @@ def edge_autograd(amd):
       "adjoint_temporal_depth": 5, "adjoint_batch_per_wave": 2}, 31),
     ({"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2, "forward_waves": 4,
       "adjoint_temporal_depth": 4, "adjoint_rows_per_wave": 24, "adjoint_batch_per_wave": 8}, 50),
+    # eight-wavefront forward and adjoint windows, and the prefetching adjoint
+    ({"temporal_depth": 6, "rows_per_wave": 64, "batch_per_wave": 2, "forward_waves": 8,
+      "adjoint_waves": 8, "adjoint_rows_per_wave": 48, "adjoint_batch_per_wave": 3}, 50),
+    ({"temporal_depth": 4, "rows_per_wave": 32, "batch_per_wave": 3, "forward_waves": 8,
+      "adjoint_waves": 8, "adjoint_rows_per_wave": 32, "adjoint_temporal_depth": 5,
+      "adjoint_prefetch": 1, "adjoint_batch_per_wave": 2}, 31),
+    ({"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2, "forward_waves": 4,
+      "adjoint_prefetch": 1, "adjoint_batch_per_wave": 3}, 50),
 ])
 def test_four_wavefront_reversible_adjoint_edges(amd, edge_autograd, tuning, every):
     """The default four-wavefront forward and reversible adjoint on the odd

@@ -111,12 +111,14 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
                     batch_per_wave=(1, 2, 4), start=(4, 16, 2),
                     four_wave=((4, 32, 2), (4, 32, 4), (6, 32, 2), (6, 32, 4), (4, 48, 2),
                                (6, 48, 2), (6, 48, 4)),
+                    eight_wave=((6, 64, 2), (6, 64, 4)),
                     verbose: bool = False):
     """Launch shape (temporal depth K, window rows R, TX planes per window,
     wavefronts per window) for an ``UnevenBatch`` of ``solvers``: coordinate
     search over one-wavefront windows (K over ``depths`` at the starting R and
     planes, then R at the best K, then the planes at the best (K, R)), then
-    the ``four_wave`` shapes (K, R, planes) with four wavefronts per window.
+    the ``four_wave`` shapes (K, R, planes) with four wavefronts per window
+    and the ``eight_wave`` ones with eight.
     Each candidate is a captured graph of the whole batch, timed with HIP
     events after one warm replay.
 
@@ -152,10 +154,11 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
             cand = tuple(cand)
             if measure(cand) < measure(best):
                 best = cand
-    for k, r, b in four_wave:
-        cand = (k, r, b, 4)
-        if measure(cand) < measure(best):
-            best = cand
+    for nw, shapes in ((4, four_wave), (8, eight_wave)):
+        for k, r, b in shapes:
+            cand = (k, r, b, nw)
+            if measure(cand) < measure(best):
+                best = cand
     for s in solvers:
         s._handle.reset()
     return best, log

@@ -21,8 +21,9 @@ class UnevenBatch:
                  batch_per_wave: int = 1, waves: int = 1):
         """Launch shape: K = temporal_depth steps per round on windows of
         rows_per_wave rows x 64 columns, batch_per_wave TX planes per window;
-        waves = 1 (one wavefront per window; K 1..4, rows 16/24/32) or 4 (four
-        wavefronts per window; K 4 or 6, rows 32 or 48)."""
+        waves = 1 (one wavefront per window; K 1..4, rows 16/24/32), 4 (four
+        wavefronts per window; K 4 or 6, rows 32 or 48) or 8 (eight; K 6,
+        rows 64)."""
         self.solvers = list(solvers)
         for s in self.solvers:
             if not s._built:

@@ -334,9 +334,9 @@ int persist_mw_blocks_per_cu(int poles, int R, int K, bool mat_ids);
 // One round's items [first, first + count) (segments [seg0, seg1)) as one launch.
 hipError_t launch_round_tb(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
                            long long first, long long count, hipStream_t s);
-// The same rounds with four wavefronts per window (R = 32 or 48 window rows,
-// K = 4 or 6): one workgroup per work item.
-hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
+// The same rounds with NW wavefronts per window (NW = 4: R = 32 or 48 window
+// rows, K = 4 or 6; NW = 8: R = 64, K = 6): one workgroup per work item.
+hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int NW, int seg0, int seg1,
                            long long first, long long count, bool mat_ids, hipStream_t s);
 // dst[i] = sum over k < nparts of src[k * stride + i], i < n
 hipError_t launch_sum_parts(const float* src, int nparts, long long stride, long long n,

@@ -1364,12 +1364,12 @@ __global__ __launch_bounds__(64 * NW) void step2d_mw_kernel(TB2DArgs p) {
   mw_task<RW, K, POLES, MID, NW>(p, task, lane, w, x_e, x_h);
 }
 
-// Uneven batch, one launch per round, four wavefronts per window: each
-// workgroup is one work item (problem, tile, TX chunk) of the round.
-template <int RW, int K, int POLES, bool MID>
-__global__ __launch_bounds__(256) void round_mw_kernel(MultiArgs m, int seg0, int seg1,
-                                                       long long first, long long count) {
-  __shared__ float x_e[4][64], x_h[4][64];
+// Uneven batch, one launch per round, NW = 4 or 8 wavefronts per window:
+// each workgroup is one work item (problem, tile, TX chunk) of the round.
+template <int RW, int K, int POLES, bool MID, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void round_mw_kernel(MultiArgs m, int seg0, int seg1,
+                                                           long long first, long long count) {
+  __shared__ float x_e[NW][64], x_h[NW][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long wi = xcd_block(m.xcd_order);
@@ -1394,23 +1394,29 @@ __global__ __launch_bounds__(256) void round_mw_kernel(MultiArgs m, int seg0, in
   a.e_save = nullptr;
   // every problem of the batch is or is not given material ids (the host
   // sets them for all or none)
-  mw_task<RW, K, POLES, MID>(a, task, lane, w, x_e, x_h);
+  mw_task<RW, K, POLES, MID, NW>(a, task, lane, w, x_e, x_h);
 }
 
-template <int RW, int K, bool MID>
+template <int RW, int K, bool MID, int NW = 4>
 static hipError_t launch_round_mw_rk(const MultiArgs& m, int poles, int seg0, int seg1,
                                      long long first, long long count, hipStream_t s) {
-  const dim3 blk(256, 1, 1), grd((unsigned)count, 1, 1);
-  if (poles == 0) hipLaunchKernelGGL((round_mw_kernel<RW, K, 0, MID>), grd, blk, 0, s, m, seg0, seg1, first, count);
-  else if (poles == 1) hipLaunchKernelGGL((round_mw_kernel<RW, K, 1, MID>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  const dim3 blk(64 * NW, 1, 1), grd((unsigned)count, 1, 1);
+  if (poles == 0) hipLaunchKernelGGL((round_mw_kernel<RW, K, 0, MID, NW>), grd, blk, 0, s, m, seg0, seg1, first, count);
+  else if (poles == 1) hipLaunchKernelGGL((round_mw_kernel<RW, K, 1, MID, NW>), grd, blk, 0, s, m, seg0, seg1, first, count);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int seg0, int seg1,
+hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int NW, int seg0, int seg1,
                            long long first, long long count, bool mat_ids, hipStream_t s) {
   if (count < 1) return hipSuccess;
   if (count > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (NW == 8) {   // 64-row windows of eight 8-row wavefronts, depth 6
+    if (R != 64 || K != 6) return hipErrorInvalidValue;
+    return mat_ids ? launch_round_mw_rk<8, 6, true, 8>(m, poles, seg0, seg1, first, count, s)
+                   : launch_round_mw_rk<8, 6, false, 8>(m, poles, seg0, seg1, first, count, s);
+  }
+  if (NW != 4) return hipErrorInvalidValue;
 #define CEEP_RMW(RW)                                                                         \
   switch (K) {                                                                               \
     case 4: return mat_ids ? launch_round_mw_rk<RW, 4, true>(m, poles, seg0, seg1, first, count, s)   \

@@ -2743,8 +2743,8 @@ int multi_enqueue(ceep_fdtd_multi* m, int mode) {
     const int s0 = m->round_seg[r], s1 = m->round_seg[r + 1];
     const long long first = m->seg_start_h[s0];
     const long long last = s1 < m->nseg ? m->seg_start_h[s1] : (long long)m->total;
-    if (m->waves == 4)
-      CEEP_HIP(ceep::launch_round_mw(a, m->poles, m->R, m->K, s0, s1, first, last - first,
+    if (m->waves == 4 || m->waves == 8)
+      CEEP_HIP(ceep::launch_round_mw(a, m->poles, m->R, m->K, m->waves, s0, s1, first, last - first,
                                      m->mat_ids, s));
     else
       CEEP_HIP(ceep::launch_round_tb(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
@@ -2766,8 +2766,10 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   } else if (waves == 4) {
     if (K != 4 && K != 6) return fail(CEEP_EINVAL, "four-wavefront rounds: temporal depth 4 or 6");
     if (R != 32 && R != 48) return fail(CEEP_EINVAL, "four-wavefront rounds: window rows 32 or 48");
+  } else if (waves == 8) {
+    if (K != 6 || R != 64) return fail(CEEP_EINVAL, "eight-wavefront rounds: depth 6 on 64-row windows");
   } else {
-    return fail(CEEP_EINVAL, "wavefronts per window must be 1 or 4");
+    return fail(CEEP_EINVAL, "wavefronts per window must be 1, 4 or 8");
   }
   if (bpw < 1) return fail(CEEP_EINVAL, "batch planes per wavefront must be >= 1");
   int rc;
@@ -2787,7 +2789,7 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   m->hs.assign(handles, handles + n);
   m->K = K; m->R = R; m->bpw = bpw; m->poles = poles; m->waves = waves;
   // material ids only when every problem has them (one kernel form per round)
-  m->mat_ids = waves == 4;
+  m->mat_ids = waves == 4 || waves == 8;
   for (int i = 0; i < n; ++i) m->mat_ids = m->mat_ids && handles[i]->n_mat > 0;
   std::vector<ceep::MultiDesc> desc(n);
   std::vector<int> ntasks(n);

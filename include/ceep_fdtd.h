@@ -337,7 +337,8 @@ typedef struct ceep_fdtd_multi ceep_fdtd_multi;
 
 /* waves = 1: one wavefront per window, K = steps per round (1..4), R =
  * window rows (16, 24, 32); waves = 4: four wavefronts per window (halo rows
- * through LDS), K = 4 or 6, R = 32 or 48.  bpw = TX planes per window.  The
+ * through LDS), K = 4 or 6, R = 32 or 48; waves = 8: eight, K = 6, R = 64
+ * (per-round mode only).  bpw = TX planes per window.  The
  * handles must outlive the group. */
 int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw, int waves,
                            ceep_fdtd_multi** out);

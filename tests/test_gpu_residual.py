@@ -218,3 +218,35 @@ def test_fused_gradient_rejects_bad_arguments(amd):
     s._handle.set_reference(c["eps_t"], c["sig_t"])
     with pytest.raises(ValueError, match="pass no S_target"):
         s._handle.gradient_misfit(50, freqs, np.zeros((4, 8, 8), dtype=complex))
+
+
+def test_config3_pulse_crop_matches_autograd(amd):
+    """Configuration 3's autograd check: a 96 x 96 crop of the configuration-1
+    geometry with its pulse (2 GHz, 1.5 GHz bandwidth), 8 TX = RX on a ring,
+    400 steps, the misfit at 16 frequencies in 1-2.5 GHz against a target
+    disk, checkpoints every 50.  The FP32 difference-form gradient (the fused
+    call the bench times) against autograd through the FP64 torch loop."""
+    from ceep_amd import adjoint
+    NX, N, T = 96, 8, 400
+    freqs = np.linspace(1e9, 2.5e9, 16)
+    dt = W.dt_2d_courant(DX, 0.5)
+    w = W.modulated_gaussian(T, dt, 2e9, 1.5e9)
+    pos = amd.phantoms.ring_positions((48, 48), 30, 8)
+    eps, sig = amd.phantoms.DiskPhantom2D(NX, NX, 0, r=(8, 14), max_offset=8).get_eps_map(2e9)
+    eps_t, sig_t = amd.phantoms.DiskPhantom2D(NX, NX, 1, r=(8, 14), max_offset=8).get_eps_map(2e9)
+    St = dft.running_dft(fdtd2d.run_cavity(NX, NX, DX, dt, N, pos, pos, w, eps_t, sig_t), freqs, dt)
+    e = torch.tensor(eps, requires_grad=True)
+    sg = torch.tensor(sig, requires_grad=True)
+    S = s_matrix_torch(run_cavity_torch(NX, NX, DX, dt, N, pos, pos, w, e, sg), freqs, dt)
+    d = S - torch.tensor(St)
+    L = (d.real ** 2 + d.imag ** 2).sum()
+    L.backward()
+    s = amd.BatchedFDTD2D(nx=NX, ny=NX, dx=DX, total_steps=T, cpml_thickness=N,
+                          source_positions=pos, probe_positions=pos, frequency=2e9,
+                          courant=0.5, waveform=w)
+    loss, ge, gs = adjoint.gradient(s, eps, sig, freqs, every=50, target=(eps_t, sig_t))
+    ee, es = rel_l2(ge, e.grad.numpy()), rel_l2(gs, sg.grad.numpy())
+    print(f"\nresidual |S - S_t| / |S| = {float(L.detach()) ** 0.5 / float((S.abs() ** 2).sum().detach()) ** 0.5:.1e}; "
+          f"loss {loss:.6e} vs {float(L.detach()):.6e}; dL/deps rel-L2 {ee:.2e}, dL/dsigma {es:.2e}")
+    assert loss == pytest.approx(float(L.detach()), rel=1e-4)
+    assert ee < 1e-3 and es < 1e-3

@@ -90,10 +90,19 @@ def test_uneven_four_wavefront_rounds(amd, debye, K, R, bpw):
     _check(amd, debye, K, R, bpw, UnevenBatch.PER_ROUND, waves=4)
 
 
+@pytest.mark.parametrize("debye", [True, False])
+@pytest.mark.parametrize("bpw", [2, 3])
+def test_uneven_eight_wavefront_rounds(amd, debye, bpw):
+    """Rounds on 64-row windows of eight wavefronts: every problem
+    bit-identical to its own run with the eight-wavefront forward kernel."""
+    from ceep_amd.uneven import UnevenBatch
+    _check(amd, debye, 6, 64, bpw, UnevenBatch.PER_ROUND, waves=8)
+
+
 def test_uneven_shape_checks(amd):
     from ceep_amd.uneven import UnevenBatch
     group = [_make(amd, sp, True, None)[0] for sp in SPECS[:2]]
-    for k, r, waves in ((4, 16, 4), (5, 32, 4), (4, 48, 1), (4, 16, 2)):
+    for k, r, waves in ((4, 16, 4), (5, 32, 4), (4, 48, 1), (4, 16, 2), (4, 64, 8), (6, 32, 8)):
         with pytest.raises(ValueError):
             UnevenBatch(group, k, r, 1, waves)
 

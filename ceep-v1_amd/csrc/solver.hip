@@ -13,6 +13,7 @@
 #include <array>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <string>
 #include <vector>
@@ -1771,14 +1772,24 @@ int adjoint3d(ceep_fdtd* h, const double* dtraces, double* grad_eps_r, double* g
   hipStream_t st = h->stream;
   if ((rc = zero_state(h))) return rc;   // the recomputation overwrote the fields
   CEEP_HIP(hipStreamSynchronize(st));
-  for (int x = 1; x < h->M[0] - 1; ++x)
-    for (int y = 1; y < h->M[1] - 1; ++y)
-      for (int z = 1; z < h->M[2] - 1; ++z) {
-        const long long o = h->box_index(x, y, z);
-        const long long g = h->full_index(x + h->org, y + h->org, z + h->org);
-        chain_rule(h->eps_r[g], eps_add_at(h, g), h->sigma_e[g], h->dt, gca[o] / scale,
-                   gcb[o] / scale, &grad_eps_r[g], &grad_sigma[g]);
-      }
+  // the chain rule per cell, x slabs split over host threads (each writes its own cells)
+  auto slabs = [&](int x0, int x1) {
+    for (int x = x0; x < x1; ++x)
+      for (int y = 1; y < h->M[1] - 1; ++y)
+        for (int z = 1; z < h->M[2] - 1; ++z) {
+          const long long o = h->box_index(x, y, z);
+          const long long g = h->full_index(x + h->org, y + h->org, z + h->org);
+          chain_rule(h->eps_r[g], eps_add_at(h, g), h->sigma_e[g], h->dt, gca[o] / scale,
+                     gcb[o] / scale, &grad_eps_r[g], &grad_sigma[g]);
+        }
+  };
+  const int xs = h->M[0] - 2;
+  const int nt = std::max(1, std::min({(int)std::thread::hardware_concurrency(), 16, xs}));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t)
+    pool.emplace_back(slabs, 1 + (int)((long long)xs * t / nt), 1 + (int)((long long)xs * (t + 1) / nt));
+  slabs(1, 1 + xs / nt);
+  for (auto& th : pool) th.join();
   return CEEP_OK;
 }
 

@@ -313,6 +313,8 @@ class BatchedFDTD2D:
 
     def close(self) -> None:
         if self._handle is not None:
+            if self.__dict__.get("_probe_fetch") is not None:
+                _ = self.probe_data             # traces left on the device by a gradient call
             self._handle.close()
             self._handle = None
             self._built = False

@@ -194,6 +194,7 @@ struct ceep_fdtd {
   // device memory holds after what the adjoint still allocates, else that many.
   int estore_x = 0;
   int estore_tail = -1;
+  int estore_m = 0;            // the tail segment's steps not kept (recomputed by the adjoint)
   int estore_extra = -1;
 
   // FP64 forward of the gradient path (tuning key 5): fields, maps, waveform,
@@ -342,6 +343,7 @@ struct ceep_fdtd {
     estore_off = 0;
     estore_x = 0;
     estore_tail = -1;
+    estore_m = 0;
     for (auto& row : adj)
       for (auto& p : row) { if (p) (void)hipFree(p); p = nullptr; }
     for (auto& row : adj3)
@@ -1706,6 +1708,7 @@ int run_checkpointed3d(ceep_fdtd* h, int every, double* traces_out) {
   h->ck_valid = true;
   h->estore_seg = nck - 1;
   h->estore_tail = tail ? nck - 2 : -1;
+  h->estore_m = tail ? m : 0;
   if (traces_out) return ceep_fdtd_get_traces(h, traces_out);
   return CEEP_OK;
 }
@@ -1916,14 +1919,13 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
   const size_t nstate = state3d(h, 0).size();
   const size_t slot = (size_t)3 * nbox;
   int set = 0;
-  const int kd = h->tb3d_k;
   for (int k = h->ck_count - 1; k >= 0; --k) {
     const int s0 = k * every, n = std::min(every, T - s0);
     if (k != h->estore_seg) {   // recompute the segment from its checkpoint, storing E
       // the tail segment: only its first m steps, into the slots after estore_x
       // (the forward pass left the rest behind slot every)
       const bool tail = k == h->estore_tail && h->estore_x > 0;
-      const int m = tail ? (every - h->estore_x + kd - 1) / kd * kd : n;
+      const int m = tail ? h->estore_m : n;   // as the checkpointed pass split it
       const int off = tail ? h->estore_x : 0;
       float* base = h->estore + (size_t)off * slot;
       const std::vector<float*> live = state3d(h, 0);

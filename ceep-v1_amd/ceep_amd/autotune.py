@@ -21,8 +21,9 @@ _LOADED = False
 
 # measured best on MI355X for the 600^2 x 16 Debye workload; used when nothing
 # was tuned for a shape
-DEFAULT_2D = {"temporal_depth": 6, "rows_per_wave": 64, "batch_per_wave": 2,
-              "batch_per_thread": 4, "graph": 1, "xcd_order": 1, "forward_waves": 8}
+DEFAULT_2D = {"temporal_depth": 8, "rows_per_wave": 64, "batch_per_wave": 2,
+              "batch_per_thread": 4, "graph": 1, "xcd_order": 1, "forward_waves": 8,
+              "tx_streams": 2}
 
 
 def _key(ndim, shape, n, batch, debye, lorentz):
@@ -69,6 +70,10 @@ def candidates_2d(lorentz=False):
         for k, r, bpw in ((6, 64, 2), (6, 64, 4), (8, 64, 2), (6, 48, 2)):
             out.append({"forward_waves": 8, "temporal_depth": k, "rows_per_wave": r,
                         "batch_per_wave": bpw})
+        # the TX groups on one stream instead of two
+        for nw, k, r in ((8, 8, 64), (8, 6, 64), (4, 6, 32)):
+            out.append({"forward_waves": nw, "temporal_depth": k, "rows_per_wave": r,
+                        "batch_per_wave": 2, "tx_streams": 1})
     return out
 
 

@@ -225,14 +225,18 @@ struct ceep_fdtd {
   unsigned char* diff_tile = nullptr;
   int diff_tile_n = 0;
   // the flagged tiles and the others as lists ([n_diff] then [n_plain] ints):
-  // each blocked step runs the difference kernel over the flagged tiles on
-  // stream2 while the plain kernel (no source, 80 VGPRs against 127) steps
-  // the rest on stream; events ev_plain / ev_diff join the two per step
+  // by default one difference launch per step takes the flagged tiles first;
+  // key 23 = 2 runs them with the difference kernel on an extra stream while
+  // the plain kernel (no source, 80 VGPRs against 127) steps the rest
   int* tile_lists = nullptr;
   int n_diff_tiles = 0, n_plain_tiles = 0;
   int split_diff = 1;          // tuning key 23: 0 = one difference kernel over every tile
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_plain = nullptr, ev_diff = nullptr;
+  int tx_streams = 2;          // tuning key 27: TX groups of the blocked kernels on 1, 2 or 4 streams
+  // extra streams (default priority): xs[0] also serves the difference
+  // form's split modes; ev_fork / ev_join[i] fork work onto xs[i] and join it
+  hipStream_t xs[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
   float* e_ref = nullptr;      // (T + 1) slots of [B][plane]: E_ref^s
   std::vector<float*> ref_ck;  // (nck + 1) x 3: reference state every ref_every steps
   std::vector<float> ref_traces;   // [B][P][T]
@@ -259,7 +263,7 @@ struct ceep_fdtd {
   int xcd_order = 1;           // 2D blocked kernels: contiguous tile ranges per XCD
   int adj_r = 0;               // 2D blocked adjoint: window rows (0: 32 for the four-wavefront
                                // reversible kernel, 16 for the one-wavefront kernels)
-  int adj_bpw = 8;             // 2D blocked adjoint: batch planes per wavefront
+  int adj_bpw = 4;             // 2D blocked adjoint: batch planes per wavefront
   int adj_waves = 4;           // 2D reversible adjoint: wavefronts per window (1, 4 or 8)
   int adj_prefetch = 0;        // 2D reversible adjoint: next TX's planes loaded ahead (key 24)
   // device S-misfit of ceep_fdtd_gradient_misfit: FP64 twiddles [T][F] for
@@ -653,6 +657,53 @@ int zero_state(ceep_fdtd* h) {
 
 int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save);
 
+// The handle's second stream (default priority) and the two events that fork
+// work onto it and join it back, created once.
+int ensure_streams(ceep_fdtd* h, int n) {
+  for (int i = 0; i < n && i < 3; ++i) {
+    if (!h->xs[i]) {
+      hipError_t e = hipStreamCreateWithFlags(&h->xs[i], hipStreamNonBlocking);
+      if (e != hipSuccess) { h->xs[i] = nullptr; return fail(CEEP_EHIP, "extra stream"); }
+    }
+    if (!h->ev_join[i]) CEEP_HIP(hipEventCreateWithFlags(&h->ev_join[i], hipEventDisableTiming));
+  }
+  if (!h->ev_fork) CEEP_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+  return CEEP_OK;
+}
+
+// Work queued on s so far happens before anything queued on the first n extra streams next.
+int fork_streams(ceep_fdtd* h, hipStream_t s, int n) {
+  int rc = ensure_streams(h, n);
+  if (rc) return rc;
+  CEEP_HIP(hipEventRecord(h->ev_fork, s));
+  for (int i = 0; i < n; ++i) CEEP_HIP(hipStreamWaitEvent(h->xs[i], h->ev_fork, 0));
+  return CEEP_OK;
+}
+
+// Work queued on the first n extra streams happens before anything queued on s next.
+int join_streams(ceep_fdtd* h, hipStream_t s, int n) {
+  for (int i = 0; i < n; ++i) {
+    CEEP_HIP(hipEventRecord(h->ev_join[i], h->xs[i]));
+    CEEP_HIP(hipStreamWaitEvent(s, h->ev_join[i], 0));
+  }
+  return CEEP_OK;
+}
+
+// TX groups of the blocked kernels: g of G takes whole chunks of bpw planes,
+// chunks [c0, c0 + nc), planes [b0, b0 + nb); empty groups are dropped.
+struct TxGroup { int b0, nb, c0, nc; };
+std::vector<TxGroup> tx_groups(int B, int bpw, int G) {
+  const int chunks = (B + bpw - 1) / bpw;
+  std::vector<TxGroup> out;
+  for (int g = 0; g < G; ++g) {
+    const int c0 = (int)((long long)chunks * g / G), c1 = (int)((long long)chunks * (g + 1) / G);
+    if (c1 <= c0) continue;
+    const int b0 = c0 * bpw, b1 = std::min(B, c1 * bpw);
+    out.push_back(TxGroup{b0, b1 - b0, c0, c1 - c0});
+  }
+  return out;
+}
+
 // The buffer set (ping-pong parity) live after launch_range(h, start, ., nsteps):
 // the 3D blocked form flips it once per launch (tb3d_k steps), the other forms once per step.
 int range_end(const ceep_fdtd* h, int start, int nsteps) {
@@ -726,7 +777,7 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     }
     // difference form (key 23): 1 = one difference launch over the tiles
     // with the flagged ones first (they take longest and start earliest),
-    // 2 = the flagged tiles with the difference kernel on stream2 and the
+    // 2 = the flagged tiles with the difference kernel on an extra stream and the
     // others with the plain kernel (no source) on stream, 3 = the same two
     // launches one after the other on stream
     const bool lists = kpoles == ceep::kPolesDiff && h->tile_lists && !e_save &&
@@ -751,11 +802,51 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
       pa.d_a = pa.d_b = nullptr;
       pa.e_ref = nullptr;
       pa.diff_tile = nullptr;
-      // stream2 starts after everything already queued on stream
+      // the extra stream starts after everything already queued on stream
       if (two_streams) {
-        CEEP_HIP(hipEventRecord(h->ev_plain, s));
-        CEEP_HIP(hipStreamWaitEvent(h->stream2, h->ev_plain, 0));
+        int rc2 = fork_streams(h, s, 1);
+        if (rc2) return rc2;
       }
+    }
+    // TX groups on separate streams (key 27): the groups share no state, so
+    // the extra streams join once before the range and rejoin after it, and
+    // each stream's launches fill the partly idle last rounds of the others'
+    const std::vector<TxGroup> groups =
+        (h->tx_streams > 1 && !split) ? tx_groups(h->B, a.bpw, h->tx_streams) : std::vector<TxGroup>();
+    if (groups.size() > 1) {
+      const int nx = (int)groups.size() - 1;
+      int rc2 = fork_streams(h, s, nx);
+      if (rc2) return rc2;
+      for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const TxGroup& g = groups[gi];
+        const long long po = (long long)g.b0 * h->plane;
+        auto at = [po](float* q) { return q ? q + po : q; };
+        ceep::TB2DArgs ha = a;
+        ha.B = g.nb;
+        ha.ntasks = (a.tile_list ? a.n_list : a.tiles_x * a.tiles_y) * g.nc;
+        ha.src_cell = a.src_cell + g.b0;
+        ha.wave = a.wave + (long long)g.b0 * a.wave_stride_b;
+        ha.traces = a.traces + (long long)g.b0 * h->P * h->T;
+        if (ha.e_ref) ha.e_ref = a.e_ref + po;
+        const hipStream_t hs = gi ? h->xs[gi - 1] : s;
+        int cur = start;
+        for (int st = step0; st < step_end; st += K) {
+          const int o = cur ^ 1;
+          ha.ez_in = at(h->f[cur][0]); ha.hx_in = at(h->f[cur][1]); ha.hy_in = at(h->f[cur][2]);
+          ha.ez_out = at(h->f[o][0]); ha.hx_out = at(h->f[o][1]); ha.hy_out = at(h->f[o][2]);
+          ha.jd_in = at(h->jd[cur]); ha.jd_out = at(h->jd[o]);
+          ha.step0 = st;
+          ha.ksteps = std::min(K, step_end - st);
+          ha.slot = h->box_floats();
+          ha.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * h->box_floats() + po : nullptr;
+          if (mw)
+            CEEP_HIP(ceep::launch_step2d_mw(ha, kpoles, R, K, h->fwd_waves, hs));
+          else
+            CEEP_HIP(ceep::launch_step2d_tb(ha, kpoles, R, K, hs));
+          cur = o;
+        }
+      }
+      return join_streams(h, s, nx);
     }
     int cur = start;
     for (int st = step0; st < step_end; st += K) {
@@ -774,13 +865,13 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
         pa.ez_out = a.ez_out; pa.hx_out = a.hx_out; pa.hy_out = a.hy_out;
         pa.jd_in = a.jd_in; pa.jd_out = a.jd_out;
         pa.step0 = a.step0; pa.ksteps = a.ksteps; pa.slot = a.slot; pa.e_save = nullptr;
-        CEEP_HIP(ceep::launch_step2d_mw(a, kpoles, R, K, h->fwd_waves, two_streams ? h->stream2 : s));
+        CEEP_HIP(ceep::launch_step2d_mw(a, kpoles, R, K, h->fwd_waves, two_streams ? h->xs[0] : s));
         CEEP_HIP(ceep::launch_step2d_mw(pa, 0, R, K, h->fwd_waves, s));
         if (two_streams) {
-          CEEP_HIP(hipEventRecord(h->ev_diff, h->stream2));
-          CEEP_HIP(hipEventRecord(h->ev_plain, s));
-          CEEP_HIP(hipStreamWaitEvent(s, h->ev_diff, 0));
-          CEEP_HIP(hipStreamWaitEvent(h->stream2, h->ev_plain, 0));
+          CEEP_HIP(hipEventRecord(h->ev_join[0], h->xs[0]));
+          CEEP_HIP(hipEventRecord(h->ev_fork, s));
+          CEEP_HIP(hipStreamWaitEvent(s, h->ev_join[0], 0));
+          CEEP_HIP(hipStreamWaitEvent(h->xs[0], h->ev_fork, 0));
         }
       } else if (mw) {
         CEEP_HIP(ceep::launch_step2d_mw(a, kpoles, R, K, h->fwd_waves, s));
@@ -1219,10 +1310,12 @@ int ceep_fdtd_destroy(ceep_fdtd* h) {
   dfree(h->mat_id); dfree(h->mat_table);
   dfree(h->ref_eps64); dfree(h->ref_sig64);
   for (int k = 0; k < 12; ++k) dfree(h->psi[k]);
-  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
-  if (h->ev_plain) (void)hipEventDestroy(h->ev_plain);
-  if (h->ev_diff) (void)hipEventDestroy(h->ev_diff);
-  if (h->stream2) (void)hipStreamDestroy(h->stream2);
+  for (int i = 0; i < 3; ++i) {
+    if (h->xs[i]) (void)hipStreamSynchronize(h->xs[i]);
+    if (h->ev_join[i]) (void)hipEventDestroy(h->ev_join[i]);
+    if (h->xs[i]) (void)hipStreamDestroy(h->xs[i]);
+  }
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CEEP_OK;
@@ -1996,13 +2089,7 @@ int build_tile_lists(ceep_fdtd* h, int ntiles) {
   CEEP_HIP(hipStreamSynchronize(st));
   h->n_diff_tiles = nd;
   h->n_plain_tiles = ntiles - nd;
-  if (!h->stream2) {
-    hipError_t e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
-    if (e != hipSuccess) { h->stream2 = nullptr; return fail(CEEP_EHIP, "second stream"); }
-  }
-  if (!h->ev_plain) CEEP_HIP(hipEventCreateWithFlags(&h->ev_plain, hipEventDisableTiming));
-  if (!h->ev_diff) CEEP_HIP(hipEventCreateWithFlags(&h->ev_diff, hipEventDisableTiming));
-  return CEEP_OK;
+  return ensure_streams(h, 1);
 }
 
 int run_reference(ceep_fdtd* h, int every) {
@@ -2267,13 +2354,12 @@ int ceep_fdtd_gradient_misfit(ceep_fdtd* h, int every, const double* freqs, int 
   if (h->ndim != 2) return fail(CEEP_EUNSUPPORTED, "the fused misfit gradient covers 2D handles");
   if (h->fwd64) return fail(CEEP_EUNSUPPORTED, "the fused misfit gradient runs the FP32 forward "
                                                "(FP64 forward: ceep_fdtd_adjoint)");
-  if (h->has_ref && s_target)
-    return fail(CEEP_EINVAL, "with a reference medium the residual is S - S_target itself: "
-                             "pass no S_target");
   if (!h->has_ref && !s_target)
     return fail(CEEP_EINVAL, "give S_target or set a reference medium");
-  int rc = h->has_ref ? ceep_fdtd_run_residual(h, every, nullptr, nullptr)
-                      : ceep_fdtd_run_checkpointed(h, every, nullptr);
+  // S_target given: the total-field misfit (a reference medium, if set, is
+  // kept for later calls); otherwise the difference form against the reference
+  int rc = s_target ? ceep_fdtd_run_checkpointed(h, every, nullptr)
+                    : ceep_fdtd_run_residual(h, every, nullptr, nullptr);
   if (rc) return rc;
   const int B = h->B, P = h->P, T = h->T, F = n_freq;
   const size_t nf = (size_t)h->full_cells, nd = (size_t)F * P * B;
@@ -2375,45 +2461,63 @@ int adjoint2d_device(ceep_fdtd* h, double scale, double* grad_eps_r, double* gra
   a.probe_head = h->probe_head; a.probe_next = h->probe_next;
   a.g_ca = h->g_ca; a.g_cbi = h->g_cbi;
   int acur = 0;
-  for (int k = h->ck_count - 1; k >= 0 && rev; --k) {
-    // reversible: start from the exact state at the segment's end (the next
-    // checkpoint, or the final state) and step forward and adjoint back together
-    const int s0 = k * every, n = std::min(every, T - s0);
-    const int K = adj_k;
-    const int R = h->adj_r ? h->adj_r : (mw_rev ? 32 : 16);
-    ceep::AdjTB2DArgs t{};
-    t.xcd_order = h->xcd_order;
-    t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = B; t.bstride = h->plane;
-    t.ca = h->ca; t.cb = h->cb; t.ch = h->ch; t.idx = h->idx;
-    t.src_cell = h->src2; t.wave = h->wave; t.wave_stride_b = h->wave_stride;
-    t.T = T; t.P = P; t.dtr = h->d_dtr; t.probe_cell = h->probe_cell;
-    t.g_ca = h->g_ca; t.g_cbi = h->g_cbi; t.gstride = h->plane;
-    t.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
-    t.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
-    t.bpw = h->adj_bpw;
-    t.ntasks = t.tiles_x * t.tiles_y * ((B + t.bpw - 1) / t.bpw);
-    // the first launch reads the segment-end checkpoint in place; later ones
-    // ping-pong between the two field buffers
-    int fcur = 0;
-    bool first = true;
-    for (int hi = s0 + n; hi > s0; hi -= K) {
-      const int o = acur ^ 1, fo = fcur ^ 1;
-      t.le_in = h->adj[acur][0]; t.mx_in = h->adj[acur][1]; t.my_in = h->adj[acur][2];
-      t.le_out = h->adj[o][0]; t.mx_out = h->adj[o][1]; t.my_out = h->adj[o][2];
-      float* const* fin = first ? &h->ck[(size_t)(k + 1) * 3] : h->f[fcur];
-      t.fe_in = fin[0]; t.fhx_in = fin[1]; t.fhy_in = fin[2];
-      t.fe_out = h->f[fo][0]; t.fhx_out = h->f[fo][1]; t.fhy_out = h->f[fo][2];
-      first = false;
-      t.step_hi = hi;
-      t.ksteps = std::min(K, hi - s0);
-      if (mw_rev)
-        CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, h->adj_waves, h->adj_prefetch, st));
-      else
-        CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, st));
-      acur = o;
-      fcur = fo;
+  // TX groups on separate streams (key 27), as the forward: a group's chunks
+  // run every reversed segment on their own stream, fork once, join once
+  const std::vector<TxGroup> agroups =
+      (rev && h->tx_streams > 1) ? tx_groups(B, h->adj_bpw, h->tx_streams)
+                                 : std::vector<TxGroup>{TxGroup{0, B, 0, (B + h->adj_bpw - 1) / h->adj_bpw}};
+  const int anx = rev ? (int)agroups.size() - 1 : 0;
+  if (anx > 0 && (rc = fork_streams(h, st, anx))) return rc;
+  const int acur0 = acur;
+  for (size_t gi = 0; gi < agroups.size() && rev; ++gi) {
+    const TxGroup& g = agroups[gi];
+    const hipStream_t gs = gi ? h->xs[gi - 1] : st;
+    const long long po = (long long)g.b0 * h->plane;
+    auto at = [po](float* q) { return q + po; };
+    acur = acur0;
+    for (int k = h->ck_count - 1; k >= 0; --k) {
+      // reversible: start from the exact state at the segment's end (the next
+      // checkpoint, or the final state) and step forward and adjoint back together
+      const int s0 = k * every, n = std::min(every, T - s0);
+      const int K = adj_k;
+      const int R = h->adj_r ? h->adj_r : (mw_rev ? 32 : 16);
+      ceep::AdjTB2DArgs t{};
+      t.xcd_order = h->xcd_order;
+      t.Mx = h->M[0]; t.My = h->M[1]; t.pitch = h->pitch; t.B = g.nb; t.bstride = h->plane;
+      t.ca = h->ca; t.cb = h->cb; t.ch = h->ch; t.idx = h->idx;
+      t.src_cell = h->src2 + g.b0; t.wave = h->wave + (long long)g.b0 * h->wave_stride;
+      t.wave_stride_b = h->wave_stride;
+      t.T = T; t.P = P; t.dtr = h->d_dtr + (size_t)g.b0 * P * T; t.probe_cell = h->probe_cell;
+      t.g_ca = h->g_ca + (size_t)g.c0 * h->plane; t.g_cbi = h->g_cbi + (size_t)g.c0 * h->plane;
+      t.gstride = h->plane;
+      t.tiles_x = (h->M[0] + (R - 2 * K) - 1) / (R - 2 * K);
+      t.tiles_y = (h->M[1] + (64 - 2 * K) - 1) / (64 - 2 * K);
+      t.bpw = h->adj_bpw;
+      t.ntasks = t.tiles_x * t.tiles_y * g.nc;
+      // the first launch reads the segment-end checkpoint in place; later ones
+      // ping-pong between the two field buffers
+      int fcur = 0;
+      bool first = true;
+      for (int hi = s0 + n; hi > s0; hi -= K) {
+        const int o = acur ^ 1, fo = fcur ^ 1;
+        t.le_in = at(h->adj[acur][0]); t.mx_in = at(h->adj[acur][1]); t.my_in = at(h->adj[acur][2]);
+        t.le_out = at(h->adj[o][0]); t.mx_out = at(h->adj[o][1]); t.my_out = at(h->adj[o][2]);
+        float* const* fin = first ? &h->ck[(size_t)(k + 1) * 3] : h->f[fcur];
+        t.fe_in = at(fin[0]); t.fhx_in = at(fin[1]); t.fhy_in = at(fin[2]);
+        t.fe_out = at(h->f[fo][0]); t.fhx_out = at(h->f[fo][1]); t.fhy_out = at(h->f[fo][2]);
+        first = false;
+        t.step_hi = hi;
+        t.ksteps = std::min(K, hi - s0);
+        if (mw_rev)
+          CEEP_HIP(ceep::launch_adj2d_rev_mw(t, R, K, h->adj_waves, h->adj_prefetch, gs));
+        else
+          CEEP_HIP(ceep::launch_adj2d_rev_tb(t, R, K, gs));
+        acur = o;
+        fcur = fo;
+      }
     }
   }
+  if (anx > 0 && (rc = join_streams(h, st, anx))) return rc;
   for (int k = h->ck_count - 1; k >= 0 && !rev; --k) {
     const int s0 = k * every, n = std::min(every, T - s0);
     // recompute the segment from its checkpoint, storing E (unless the
@@ -2545,6 +2649,9 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 24) {
     if (value != 0 && value != 1) return fail(CEEP_EINVAL, "adjoint prefetch must be 0 or 1");
     h->adj_prefetch = value;
+  } else if (key == 27) {
+    if (value != 1 && value != 2 && value != 4) return fail(CEEP_EINVAL, "TX streams must be 1, 2 or 4");
+    h->tx_streams = value;
   } else if (key == 25) {
     if (value < -1) return fail(CEEP_EINVAL, "3D extra E-store slots must be -1 (auto) or >= 0");
     if (value != h->estore_extra) {

@@ -181,10 +181,10 @@ int ceep_fdtd_run_residual(ceep_fdtd* h, int every, double* residual_out, double
 /* The whole gradient of L = sum over (f, rx, tx) of |S - S_target|^2 at
  * n_freq frequencies (2D, non-dispersive, FP32 forward) in one call, with
  * no trace or dL/dtrace round trip through the host: the checkpointed forward
- * (with a reference medium set: ceep_fdtd_run_residual, whose residual is
- * S - S_target itself and s_target must be NULL; without one:
- * ceep_fdtd_run_checkpointed, s_target = (n_freq, n_probes, batch) complex
- * as interleaved float64 pairs), the S-matrix of the traces, the loss and
+ * (s_target = (n_freq, n_probes, batch) complex as interleaved float64 pairs:
+ * ceep_fdtd_run_checkpointed; s_target NULL with a reference medium set:
+ * ceep_fdtd_run_residual, whose residual is S - S_target itself), the
+ * S-matrix of the traces, the loss and
  * dL/dtrace on the device in FP64 sums, then the adjoint as ceep_fdtd_adjoint.
  * loss_out gets L; grad_eps_r and grad_sigma (full grid, float64) dL/d eps_r
  * and dL/d sigma.  No counterpart in the reference (no gradient path; the
@@ -238,7 +238,7 @@ int ceep_fdtd_store_info(ceep_fdtd* h, int64_t* out);
  *          0 = dispatch order.  Results are identical.
  *   key 10, 11: window rows (16, 24, 32, 48; default 32 for the multi-
  *          wavefront reversible kernel, 16 otherwise) and batch planes per window
- *          (default 8) of the 2D blocked adjoint, which holds twice the
+ *          (default 4) of the 2D blocked adjoint, which holds twice the
  *          forward's state per cell and prefers other shapes than the forward
  *          (keys 3, 4).
  *   key 14: 2D reversible adjoint: wavefronts per window, 1, 4 or 8
@@ -289,6 +289,13 @@ int ceep_fdtd_store_info(ceep_fdtd* h, int64_t* out);
  *          the current one is stepped (four wavefronts: 32 rows, depth 6;
  *          eight: 32 rows, depth 4-6); 0 (default) = load each TX when it
  *          starts.  Results are identical.
+ *   key 27: TX groups on streams: 1, 2 (default) or 4.  The 2D blocked
+ *          forward (also the difference form and the reference run) and the
+ *          reversible 2D adjoint split the batch into that many groups of
+ *          whole TX chunks, each stepped on its own stream; the streams fork
+ *          once before a run (or a checkpoint segment) and join after it, so
+ *          one group's launches fill the partly idle last rounds of the
+ *          others'.  Results are bit-identical.
  *   key 25: 3D gradient (blocked adjoint): extra E-store slots beyond one
  *          segment.  The checkpointed forward also keeps E of the last
  *          `extra` steps of the segment before the last, so that segment's

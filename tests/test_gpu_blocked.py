@@ -231,3 +231,49 @@ def test_material_ids_bit_identical(amd):
         ub.close()
     assert np.array_equal(ub_out[0][0], ub_out[1][0]) and np.array_equal(ub_out[0][1], ub_out[1][1])
     assert np.array_equal(ub_out[1][0], out[1])
+
+
+@pytest.mark.parametrize("streams", [2, 4])
+@pytest.mark.parametrize("tuning", [
+    {"temporal_depth": 6, "rows_per_wave": 64, "batch_per_wave": 2, "forward_waves": 8},
+    {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 3, "forward_waves": 4},
+    {"temporal_depth": 3, "rows_per_wave": 24, "batch_per_wave": 1, "forward_waves": 1},
+])
+def test_tx_groups_on_streams_bit_identical(amd, streams, tuning):
+    """TX groups of the blocked forward on 2 or 4 streams (tuning key 27; the
+    batch of 5 TX split into whole chunks, some groups empty or ragged):
+    traces and final fields bit-identical to one stream, with and without
+    the captured graph and for a second run continuing the first."""
+    out = {}
+    for ts in (1, streams):
+        for graph in (1, 0):
+            s, *_ = _solver(amd, True, {**tuning, "tx_streams": ts, "graph": graph})
+            s.run()
+            first = s.probe_data.copy()
+            s.run()
+            out[(ts, graph)] = (first, s.probe_data.copy(), s.ez.copy(), s.hy.copy())
+    ref = out[(1, 1)]
+    assert np.abs(ref[0]).max() > 0
+    for key, val in out.items():
+        for a, b in zip(val, ref):
+            assert np.array_equal(a, b), key
+
+
+@pytest.mark.parametrize("streams", [2, 4])
+def test_tx_groups_gradient_bit_identical(amd, edge_autograd, streams):
+    """The difference-form forward, its reference run and the reversible
+    adjoint with TX groups on 2 or 4 streams: loss and gradients bit-identical
+    to one stream (per-chunk gradient partials are summed in the same order)."""
+    from ceep_amd import adjoint
+    r = edge_autograd
+    res = []
+    for ts in (1, streams):
+        s, *_ = _solver(amd, False, {"temporal_depth": 6, "rows_per_wave": 32, "batch_per_wave": 2,
+                                     "forward_waves": 4, "adjoint_batch_per_wave": 2,
+                                     "tx_streams": ts})
+        res.append(adjoint.gradient(s, r["eps"], r["sig"], r["freqs"], every=31,
+                                    target=(r["eps_t"], r["sig"])))
+        res.append(adjoint.gradient(s, r["eps"], r["sig"], r["freqs"], r["St"], every=31))
+    for a, b in zip(res[2:], res[:2]):
+        assert a[0] == b[0]
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])

@@ -560,14 +560,29 @@ def _bench_3d_gradient(T3=1000, every=100, single=False, tuning=None):
                          "dL/deps_r and dL/dsigma over the full volume")}
 
 
-def _roofline(kernel, steps, bytes_launch, launch_ms, summary):
+def _roofline(kernel, steps, bytes_launch, launch_ms, summary, dispatches=1,
+              launch_ms_one=None):
     """Roofline object of one kernel: `achieved` / `frac` from the HBM bytes a
     launch moves by rocprofv3 counter (`traffic`, the committed summary) over
     the live mean launch time; the algorithmic figure (every cell's state read
-    and written once per step, so blocked launches exceed the peak) beside it."""
-    traffic = _traffic(kernel, summary)
+    and written once per step, so blocked launches exceed the peak) beside it.
+    K steps over the whole batch may be `dispatches` kernel dispatches (TX
+    groups on separate streams, running side by side): the summary's mean
+    bytes per dispatch then count that many times, and `launch_ms` is the
+    time per K steps of the whole batch.  `launch_ms_one`: the same K steps
+    as one dispatch over the whole batch (`one_dispatch`, with that form's
+    measured bytes)."""
+    per_dispatch = _traffic(kernel, summary)
+    traffic = per_dispatch * dispatches if per_dispatch else None
     alg = bytes_launch / (launch_ms * 1e-3) / 1e9
-    out = {"bound": "hbm", "kernel": kernel, "steps_per_launch": steps}
+    out = {"bound": "hbm", "kernel": kernel, "steps_per_launch": steps,
+           "dispatches_per_launch": dispatches}
+    if launch_ms_one:
+        one = _traffic(kernel + " whole batch", summary)
+        out["one_dispatch"] = {
+            "launch_ms": round(launch_ms_one, 5), "traffic": one,
+            "achieved": round(one / (launch_ms_one * 1e-3) / 1e9, 1) if one else None,
+            "frac": round(one / (launch_ms_one * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if one else None}
     if traffic:
         ach = traffic / (launch_ms * 1e-3) / 1e9
         out.update({"achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -716,6 +731,13 @@ def main():
         launches = T
         steps_per_launch = 1
 
+    # the blocked kernels split the batch into TX groups of whole chunks of
+    # batch_per_wave planes, one group per stream
+    dispatches = 1
+    if K:
+        dispatches = min((tuning or {}).get("tx_streams", 1),
+                         -(-B // tuning.get("batch_per_wave", 1)))
+
     def step():
         h.reset()
         h.run(copy=False)
@@ -735,6 +757,15 @@ def main():
     # kernel-only timing with HIP events on the library's stream
     ms_run = h.time_runs(3)
     ms_launch = ms_run / launches
+    ms_launch_one = None
+    if dispatches > 1:
+        # the same kernel with the whole batch in one dispatch per K steps:
+        # the time a kernel trace of that form reports per dispatch
+        h.apply_tuning(dict(tuning, tx_streams=1))
+        h.time_runs(1)
+        ms_launch_one = h.time_runs(3) / launches
+        h.apply_tuning(tuning)
+        h.time_runs(1)
     # algorithmic bytes: every time step reads and writes the state once, so
     # a K-step launch is credited K steps' worth (this exceeds the peak when
     # the blocking saves traffic; `traffic` is the measured HBM side)
@@ -783,7 +814,8 @@ def main():
             "batch": B, "time_steps": T, "sync": sync, "tuning": tuning,
         },
         "roofline": _roofline(kernel, steps_per_launch, bytes_launch, ms_launch,
-                              "step2d_debye_600x600x16_traffic.json"),
+                              "step2d_debye_600x600x16_traffic.json", dispatches,
+                              ms_launch_one),
         "fwd_adjoint_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
         "bench_3d": b3,

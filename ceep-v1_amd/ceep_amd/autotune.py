@@ -116,7 +116,7 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
                     batch_per_wave=(1, 2, 4), start=(4, 16, 2),
                     four_wave=((4, 32, 2), (4, 32, 4), (6, 32, 2), (6, 32, 4), (4, 48, 2),
                                (6, 48, 2), (6, 48, 4)),
-                    eight_wave=((6, 64, 2), (6, 64, 4)),
+                    eight_wave=((6, 64, 2), (6, 64, 4), (8, 64, 2), (8, 64, 4)),
                     verbose: bool = False):
     """Launch shape (temporal depth K, window rows R, TX planes per window,
     wavefronts per window) for an ``UnevenBatch`` of ``solvers``: coordinate

@@ -22,8 +22,8 @@ class UnevenBatch:
         """Launch shape: K = temporal_depth steps per round on windows of
         rows_per_wave rows x 64 columns, batch_per_wave TX planes per window;
         waves = 1 (one wavefront per window; K 1..4, rows 16/24/32), 4 (four
-        wavefronts per window; K 4 or 6, rows 32 or 48) or 8 (eight; K 6,
-        rows 64)."""
+        wavefronts per window; K 4 or 6, rows 32 or 48) or 8 (eight; K 6
+        or 8, rows 64)."""
         self.solvers = list(solvers)
         for s in self.solvers:
             if not s._built:
@@ -50,7 +50,10 @@ class UnevenBatch:
         """PERSISTENT (four-wavefront shapes; NotImplementedError otherwise):
         one launch in which each workgroup steps whole (problem, TX chunk)
         runs, every round of one alone.  PER_ROUND: one launch per K-step
-        round over every problem.  Either is replayed as one hipGraph."""
+        round over every problem; with ``tx_streams`` = G > 1 in the first
+        solver's tuning, G launches per round, each over one group of every
+        problem's TX chunks, the groups' rounds on G streams.  Either is
+        replayed as one hipGraph."""
         self._multi.run(mode)
         out = []
         for s in self.solvers:

@@ -1366,9 +1366,20 @@ __global__ __launch_bounds__(64 * NW) void step2d_mw_kernel(TB2DArgs p) {
 
 // Uneven batch, one launch per round, NW = 4 or 8 wavefronts per window:
 // each workgroup is one work item (problem, tile, TX chunk) of the round.
+// Eight wavefronts per window: four per SIMD (at most 128 VGPRs), as the
+// uniform-batch kernel gets by itself.  Without the bound the Debye form
+// takes 134 VGPRs, three wavefronts per SIMD, which leaves one eight-wavefront
+// workgroup per CU (64 heads: 400 ms per batch at K = 6, 338 with the bound,
+// which spills 5-6 VGPRs).  Four-wavefront workgroups fit three per CU at
+// 134 VGPRs; the same bound there made them slower (285 -> 357 ms).
+template <int NW>
+constexpr int round_mw_min_waves() {
+  return NW == 8 ? 4 : 1;
+}
+
 template <int RW, int K, int POLES, bool MID, int NW = 4>
-__global__ __launch_bounds__(64 * NW) void round_mw_kernel(MultiArgs m, int seg0, int seg1,
-                                                           long long first, long long count) {
+__global__ __launch_bounds__(64 * NW, (round_mw_min_waves<NW>())) void round_mw_kernel(
+    MultiArgs m, int seg0, int seg1, long long first, long long count) {
   __shared__ float x_e[NW][64], x_h[NW][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1411,10 +1422,15 @@ hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int NW, 
                            long long first, long long count, bool mat_ids, hipStream_t s) {
   if (count < 1) return hipSuccess;
   if (count > 0x7fffffffLL) return hipErrorInvalidValue;
-  if (NW == 8) {   // 64-row windows of eight 8-row wavefronts, depth 6
-    if (R != 64 || K != 6) return hipErrorInvalidValue;
-    return mat_ids ? launch_round_mw_rk<8, 6, true, 8>(m, poles, seg0, seg1, first, count, s)
-                   : launch_round_mw_rk<8, 6, false, 8>(m, poles, seg0, seg1, first, count, s);
+  if (NW == 8) {   // 64-row windows of eight 8-row wavefronts, depth 6 or 8
+    if (R != 64) return hipErrorInvalidValue;
+    if (K == 6)
+      return mat_ids ? launch_round_mw_rk<8, 6, true, 8>(m, poles, seg0, seg1, first, count, s)
+                     : launch_round_mw_rk<8, 6, false, 8>(m, poles, seg0, seg1, first, count, s);
+    if (K == 8)
+      return mat_ids ? launch_round_mw_rk<8, 8, true, 8>(m, poles, seg0, seg1, first, count, s)
+                     : launch_round_mw_rk<8, 8, false, 8>(m, poles, seg0, seg1, first, count, s);
+    return hipErrorInvalidValue;
   }
   if (NW != 4) return hipErrorInvalidValue;
 #define CEEP_RMW(RW)                                                                         \

@@ -2802,24 +2802,35 @@ int ceep_fdtd_box_info(ceep_fdtd* h, int64_t* out) {
 
 // ---- uneven batch: many 2D problems in one persistent launch -----------------
 
-struct ceep_fdtd_multi {
-  std::vector<ceep_fdtd*> hs;
-  int K = 4, R = 16, bpw = 1, poles = 0, blocks = 0, waves = 1;
-  bool mat_ids = false;        // every problem reads material ids (four-wavefront rounds)
+// One TX group of the per-round form: every problem's TX chunks
+// [c0, c1) with c0 = chunks * g / G, c1 = chunks * (g + 1) / G (problems with
+// no chunk in the group are left out), its own work items enumerated as
+// segments (round-major, then problem), launched on its own stream.
+struct MultiGroup {
   ceep::MultiDesc* d_desc = nullptr;
   long long* d_seg_start = nullptr;
   int* d_seg_prob = nullptr;
   int* d_seg_round = nullptr;
   int nseg = 0;
   unsigned long long total = 0;
+  std::vector<int> round_seg;     // [max_rounds + 1] first segment of each round
+  std::vector<long long> seg_start_h;
+};
+
+struct ceep_fdtd_multi {
+  std::vector<ceep_fdtd*> hs;
+  int K = 4, R = 16, bpw = 1, poles = 0, blocks = 0, waves = 1;
+  bool mat_ids = false;        // every problem reads material ids (four-wavefront rounds)
+  ceep::MultiDesc* d_desc = nullptr;      // whole problems (the persistent launch)
+  std::vector<MultiGroup> groups;         // per-round launches, one group per stream
   ceep::PersistItem* d_items = nullptr;   // persistent launch: (problem, TX chunk), largest first
   int nitems = 0;
   unsigned* d_queue = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t xs[3] = {nullptr, nullptr, nullptr};   // the other groups' streams
+  hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // per mode
   std::vector<int> rounds;
-  std::vector<int> round_seg;     // [max_rounds + 1] first segment of each round
-  std::vector<long long> seg_start_h;
 };
 
 namespace {
@@ -2828,8 +2839,16 @@ void multi_free(ceep_fdtd_multi* m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (auto& g : m->gexec)
     if (g) (void)hipGraphExecDestroy(g);
-  dfree(m->d_desc); dfree(m->d_seg_start); dfree(m->d_seg_prob); dfree(m->d_seg_round);
+  dfree(m->d_desc);
+  for (MultiGroup& g : m->groups) {
+    dfree(g.d_desc); dfree(g.d_seg_start); dfree(g.d_seg_prob); dfree(g.d_seg_round);
+  }
   dfree(m->d_items); dfree(m->d_queue);
+  for (int i = 0; i < 3; ++i) {
+    if (m->xs[i]) (void)hipStreamDestroy(m->xs[i]);
+    if (m->ev_join[i]) (void)hipEventDestroy(m->ev_join[i]);
+  }
+  if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -2853,21 +2872,38 @@ int multi_enqueue(ceep_fdtd_multi* m, int mode) {
     CEEP_HIP(ceep::launch_persist_mw(p, m->poles, m->R, m->K, m->mat_ids, m->blocks, s));
     return CEEP_OK;
   }
-  ceep::MultiArgs a{};
-  a.probs = m->d_desc; a.nprob = (int)m->hs.size();
-  a.seg_start = m->d_seg_start; a.seg_prob = m->d_seg_prob; a.seg_round = m->d_seg_round;
-  a.nseg = m->nseg; a.total = m->total;
-  a.xcd_order = m->hs[0]->xcd_order;   // key 9 of the first problem's handle
-  const int nrounds = (int)m->round_seg.size() - 1;
-  for (int r = 0; r < nrounds; ++r) {
-    const int s0 = m->round_seg[r], s1 = m->round_seg[r + 1];
-    const long long first = m->seg_start_h[s0];
-    const long long last = s1 < m->nseg ? m->seg_start_h[s1] : (long long)m->total;
-    if (m->waves == 4 || m->waves == 8)
-      CEEP_HIP(ceep::launch_round_mw(a, m->poles, m->R, m->K, m->waves, s0, s1, first, last - first,
-                                     m->mat_ids, s));
-    else
-      CEEP_HIP(ceep::launch_round_tb(a, m->poles, m->R, m->K, s0, s1, first, last - first, s));
+  // the groups share no state: the other streams start after the clears
+  // above and rejoin s at the end, so each stream's rounds fill the partly
+  // idle last workgroup rounds of the others' launches
+  const int ng = (int)m->groups.size();
+  if (ng > 1) {
+    CEEP_HIP(hipEventRecord(m->ev_fork, s));
+    for (int g = 1; g < ng; ++g) CEEP_HIP(hipStreamWaitEvent(m->xs[g - 1], m->ev_fork, 0));
+  }
+  for (int g = 0; g < ng; ++g) {
+    const MultiGroup& G = m->groups[g];
+    const hipStream_t gs = g ? m->xs[g - 1] : s;
+    ceep::MultiArgs a{};
+    a.probs = G.d_desc; a.nprob = (int)m->hs.size();
+    a.seg_start = G.d_seg_start; a.seg_prob = G.d_seg_prob; a.seg_round = G.d_seg_round;
+    a.nseg = G.nseg; a.total = G.total;
+    a.xcd_order = m->hs[0]->xcd_order;   // key 9 of the first problem's handle
+    const int nrounds = (int)G.round_seg.size() - 1;
+    for (int r = 0; r < nrounds; ++r) {
+      const int s0 = G.round_seg[r], s1 = G.round_seg[r + 1];
+      if (s1 <= s0) continue;
+      const long long first = G.seg_start_h[s0];
+      const long long last = s1 < G.nseg ? G.seg_start_h[s1] : (long long)G.total;
+      if (m->waves == 4 || m->waves == 8)
+        CEEP_HIP(ceep::launch_round_mw(a, m->poles, m->R, m->K, m->waves, s0, s1, first, last - first,
+                                       m->mat_ids, gs));
+      else
+        CEEP_HIP(ceep::launch_round_tb(a, m->poles, m->R, m->K, s0, s1, first, last - first, gs));
+    }
+  }
+  for (int g = 1; g < ng; ++g) {
+    CEEP_HIP(hipEventRecord(m->ev_join[g - 1], m->xs[g - 1]));
+    CEEP_HIP(hipStreamWaitEvent(s, m->ev_join[g - 1], 0));
   }
   return CEEP_OK;
 }
@@ -2887,7 +2923,8 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
     if (K != 4 && K != 6) return fail(CEEP_EINVAL, "four-wavefront rounds: temporal depth 4 or 6");
     if (R != 32 && R != 48) return fail(CEEP_EINVAL, "four-wavefront rounds: window rows 32 or 48");
   } else if (waves == 8) {
-    if (K != 6 || R != 64) return fail(CEEP_EINVAL, "eight-wavefront rounds: depth 6 on 64-row windows");
+    if ((K != 6 && K != 8) || R != 64)
+      return fail(CEEP_EINVAL, "eight-wavefront rounds: depth 6 or 8 on 64-row windows");
   } else {
     return fail(CEEP_EINVAL, "wavefronts per window must be 1, 4 or 8");
   }
@@ -2912,7 +2949,6 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   m->mat_ids = waves == 4 || waves == 8;
   for (int i = 0; i < n; ++i) m->mat_ids = m->mat_ids && handles[i]->n_mat > 0;
   std::vector<ceep::MultiDesc> desc(n);
-  std::vector<int> ntasks(n);
   int max_rounds = 0;
   for (int i = 0; i < n; ++i) {
     ceep_fdtd* h = handles[i];
@@ -2932,27 +2968,63 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
       desc[i].jd[b] = h->jd[b];
     }
     desc[i].nrounds = (h->T + K - 1) / K;
-    ntasks[i] = a.ntasks;
     m->rounds.push_back(desc[i].nrounds);
     max_rounds = std::max(max_rounds, desc[i].nrounds);
   }
-  std::vector<long long> seg_start;
-  std::vector<int> seg_prob, seg_round;
-  unsigned long long cum = 0;
-  for (int r = 0; r < max_rounds; ++r)
-    for (int i = 0; i < n; ++i)
-      if (r < desc[i].nrounds) {
-        seg_start.push_back((long long)cum);
-        seg_prob.push_back(i);
-        seg_round.push_back(r);
-        cum += (unsigned long long)ntasks[i];
+  // TX groups of the per-round form (key 27 of the first problem's handle):
+  // group g takes chunks [chunks * g / G, chunks * (g + 1) / G) of every
+  // problem, its descriptors offset to those planes
+  const int G = std::max(1, std::min(4, handles[0]->tx_streams));
+  std::vector<std::vector<ceep::MultiDesc>> gdesc;
+  std::vector<std::vector<long long>> gseg_start;
+  std::vector<std::vector<int>> gseg_prob, gseg_round;
+  for (int g = 0; g < G; ++g) {
+    std::vector<ceep::MultiDesc> dg = desc;
+    std::vector<int> nt(n, 0);
+    for (int i = 0; i < n; ++i) {
+      ceep_fdtd* h = handles[i];
+      const int chunks = (h->B + bpw - 1) / bpw;
+      const int c0 = (int)((long long)chunks * g / G), c1 = (int)((long long)chunks * (g + 1) / G);
+      if (c1 <= c0) continue;
+      const int b0 = c0 * bpw, nb = std::min(h->B, c1 * bpw) - b0;
+      const long long po = (long long)b0 * h->plane;
+      ceep::TB2DArgs& a = dg[i].a;
+      a.B = nb;
+      a.ntasks = a.tiles_x * a.tiles_y * (c1 - c0);
+      a.src_cell = desc[i].a.src_cell + b0;
+      a.wave = desc[i].a.wave + (long long)b0 * desc[i].a.wave_stride_b;
+      a.traces = desc[i].a.traces + (long long)b0 * h->P * h->T;
+      for (int b = 0; b < 2; ++b) {
+        for (int f = 0; f < 3; ++f) dg[i].f[b][f] = desc[i].f[b][f] + po;
+        if (desc[i].jd[b]) dg[i].jd[b] = desc[i].jd[b] + po;
       }
-  m->nseg = (int)seg_start.size();
-  m->total = cum;
-  m->seg_start_h = seg_start;
-  for (int k = 0, r = -1; k < m->nseg; ++k)
-    while (r < seg_round[k]) { m->round_seg.push_back(k); ++r; }
-  m->round_seg.push_back(m->nseg);
+      nt[i] = a.ntasks;
+    }
+    std::vector<long long> seg_start;
+    std::vector<int> seg_prob, seg_round;
+    unsigned long long cum = 0;
+    for (int r = 0; r < max_rounds; ++r)
+      for (int i = 0; i < n; ++i)
+        if (r < desc[i].nrounds && nt[i] > 0) {
+          seg_start.push_back((long long)cum);
+          seg_prob.push_back(i);
+          seg_round.push_back(r);
+          cum += (unsigned long long)nt[i];
+        }
+    if (cum == 0) continue;              // no problem has a chunk in this group
+    MultiGroup mg;
+    mg.nseg = (int)seg_start.size();
+    mg.total = cum;
+    mg.seg_start_h = seg_start;
+    for (int k = 0, r = -1; k < mg.nseg; ++k)
+      while (r < seg_round[k]) { mg.round_seg.push_back(k); ++r; }
+    mg.round_seg.push_back(mg.nseg);
+    m->groups.push_back(mg);
+    gdesc.push_back(dg);
+    gseg_start.push_back(seg_start);
+    gseg_prob.push_back(seg_prob);
+    gseg_round.push_back(seg_round);
+  }
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -2982,17 +3054,30 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
   }
   rc = CEEP_OK;
   if (!rc) rc = dalloc(&m->d_desc, (size_t)n);
-  if (!rc) rc = dalloc(&m->d_seg_start, seg_start.size());
-  if (!rc) rc = dalloc(&m->d_seg_prob, seg_prob.size());
-  if (!rc) rc = dalloc(&m->d_seg_round, seg_round.size());
+  for (size_t g = 0; g < m->groups.size() && !rc; ++g) {
+    MultiGroup& mg = m->groups[g];
+    rc = dalloc(&mg.d_desc, (size_t)n);
+    if (!rc) rc = dalloc(&mg.d_seg_start, gseg_start[g].size());
+    if (!rc) rc = dalloc(&mg.d_seg_prob, gseg_prob[g].size());
+    if (!rc) rc = dalloc(&mg.d_seg_round, gseg_round[g].size());
+  }
   if (!rc && m->nitems) rc = dalloc(&m->d_items, items.size());
   if (!rc) rc = dalloc(&m->d_queue, 1);
   if (rc) { multi_free(m); return rc; }
   e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  for (size_t g = 1; g < m->groups.size() && e == hipSuccess; ++g) {
+    e = hipStreamCreateWithFlags(&m->xs[g - 1], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_join[g - 1], hipEventDisableTiming);
+  }
+  if (e == hipSuccess && m->groups.size() > 1) e = hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMemcpy(m->d_desc, desc.data(), desc.size() * sizeof(desc[0]), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(m->d_seg_start, seg_start.data(), seg_start.size() * sizeof(long long), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(m->d_seg_prob, seg_prob.data(), seg_prob.size() * sizeof(int), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(m->d_seg_round, seg_round.data(), seg_round.size() * sizeof(int), hipMemcpyHostToDevice);
+  for (size_t g = 0; g < m->groups.size() && e == hipSuccess; ++g) {
+    MultiGroup& mg = m->groups[g];
+    e = hipMemcpy(mg.d_desc, gdesc[g].data(), gdesc[g].size() * sizeof(desc[0]), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(mg.d_seg_start, gseg_start[g].data(), gseg_start[g].size() * sizeof(long long), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(mg.d_seg_prob, gseg_prob[g].data(), gseg_prob[g].size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(mg.d_seg_round, gseg_round[g].data(), gseg_round[g].size() * sizeof(int), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess && m->nitems)
     e = hipMemcpy(m->d_items, items.data(), items.size() * sizeof(items[0]), hipMemcpyHostToDevice);
   if (e != hipSuccess) { multi_free(m); return fail(CEEP_EHIP, std::string("multi setup: ") + hipGetErrorString(e)); }
@@ -3002,6 +3087,9 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
 
 int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode) {
   if (!m) return fail(CEEP_EINVAL, "null handle");
+  if (mode == 2)   // every problem's own run on concurrent streams in one graph
+    return fail(CEEP_EUNSUPPORTED, "mode 2 (concurrent runs) is switched off: its first run "
+                                   "ended in a host fault whose cause is not found");
   if (mode != 0 && mode != 1) return fail(CEEP_EINVAL, "mode must be 0 (persistent) or 1 (per round)");
   if (mode == 0 && m->waves != 4)
     return fail(CEEP_EUNSUPPORTED, "the persistent launch takes four-wavefront window shapes");
@@ -3020,8 +3108,10 @@ int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode) {
   CEEP_HIP(hipGraphLaunch(m->gexec[mode], m->stream));
   CEEP_HIP(hipStreamSynchronize(m->stream));
   for (size_t i = 0; i < m->hs.size(); ++i) {
-    m->hs[i]->parity = m->rounds[i] & 1;
-    m->hs[i]->has_run = true;
+    ceep_fdtd* h = m->hs[i];
+    h->parity = m->rounds[i] & 1;
+    h->has_run = true;
+    h->traces64_valid = false;
   }
   return CEEP_OK;
 }
@@ -3047,7 +3137,10 @@ int ceep_fdtd_multi_time(ceep_fdtd_multi* m, int mode, int reps, double* ms) {
 
 int ceep_fdtd_multi_info(ceep_fdtd_multi* m, long long* out) {
   if (!m || !out) return fail(CEEP_EINVAL, "null argument");
-  out[0] = (long long)m->total; out[1] = m->nseg; out[2] = m->blocks; out[3] = m->nitems;
+  unsigned long long total = 0;
+  int nseg = 0;
+  for (const MultiGroup& g : m->groups) { total += g.total; nseg += g.nseg; }
+  out[0] = (long long)total; out[1] = nseg; out[2] = m->blocks; out[3] = m->nitems;
   return CEEP_OK;
 }
 

@@ -344,7 +344,7 @@ typedef struct ceep_fdtd_multi ceep_fdtd_multi;
 
 /* waves = 1: one wavefront per window, K = steps per round (1..4), R =
  * window rows (16, 24, 32); waves = 4: four wavefronts per window (halo rows
- * through LDS), K = 4 or 6, R = 32 or 48; waves = 8: eight, K = 6, R = 64
+ * through LDS), K = 4 or 6, R = 32 or 48; waves = 8: eight, K = 6 or 8, R = 64
  * (per-round mode only).  bpw = TX planes per window.  The
  * handles must outlive the group. */
 int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int bpw, int waves,
@@ -358,14 +358,20 @@ int ceep_fdtd_multi_create(int n, ceep_fdtd* const* handles, int K, int R, int b
  * one alone (its four 4-wavefront groups split each round's windows; a
  * workgroup barrier orders rounds; no workgroup waits for another).  mode 1:
  * every round (K steps of every problem still running) is one launch over
- * all problems' windows.  Either mode gives every problem the bits of its
- * own blocked run with the same window shape. */
+ * all problems' windows, or, with tuning key 27 of the first handle = G > 1,
+ * G launches: group g takes TX chunks [c * g / G, c * (g + 1) / G) of every
+ * problem of c chunks of bpw planes, and each group's rounds run on a stream
+ * of their own (parallel branches of the graph, rejoined at its end).
+ * Either mode gives every problem the bits of its own blocked run with the
+ * same window shape.  mode 2 (every problem's own run on concurrent streams)
+ * is switched off and returns CEEP_EUNSUPPORTED. */
 int ceep_fdtd_multi_run(ceep_fdtd_multi* m, int mode);
 
 /* `reps` graph replays of `mode` bracketed by HIP events; *ms = mean ms per replay. */
 int ceep_fdtd_multi_time(ceep_fdtd_multi* m, int mode, int reps, double* ms);
 
-/* out[0] = per-round work items, out[1] = (round, problem) segments,
+/* out[0] = per-round work items, out[1] = (round, problem) segments (both
+ * summed over the TX groups of mode 1),
  * out[2] = workgroups of the persistent launch (0 for one-wavefront shapes),
  * out[3] = its (problem, TX chunk) items. */
 int ceep_fdtd_multi_info(ceep_fdtd_multi* m, long long* out);

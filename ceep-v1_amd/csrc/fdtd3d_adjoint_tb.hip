@@ -103,8 +103,8 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
   const int L = xcd_block(p.order != 0);
   int ty, tz, bt;
   if (p.order == 2) {   // TX fastest: the TX of one tile share its maps in an L2
-    bt = L % p.B;
-    const int t = L / p.B;
+    bt = L % p.nb;
+    const int t = L / p.nb;
     ty = t % p.tiles_y;
     tz = t / p.tiles_y;
   } else {              // y tiles fastest
@@ -113,6 +113,7 @@ __global__ __launch_bounds__(1024) void adj3d_tb_kernel(AdjTB3DArgs p) {
     tz = r0 % p.tiles_z;
     bt = r0 / p.tiles_z;
   }
+  bt += p.b0;
   const int y = ty * (R - 2 * K) - K + w;
   const int z = tz * (64 - 2 * K) - K + lane;
   const bool yz_in = y >= 0 && y < p.My && z >= 0 && z < p.Mz;
@@ -474,7 +475,8 @@ hipError_t launch_adj3d_tb(const AdjTB3DArgs& a, int K, int poles, hipStream_t s
     return hipErrorInvalidValue;
   if (!a.mp || (poles != 0 && !a.jp) || !a.e_lo || !a.g || a.step < K - 1)
     return hipErrorInvalidValue;
-  const long long nblk = (long long)a.tiles_y * a.tiles_z * a.B;
+  if (a.nb < 1 || a.b0 < 0 || a.b0 + a.nb > a.B) return hipErrorInvalidValue;
+  const long long nblk = (long long)a.tiles_y * a.tiles_z * a.nb;
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
   const dim3 grd((unsigned)nblk, 1, 1), blk(1024, 1, 1);
   const bool mag = a.magnetic != 0;

@@ -176,6 +176,7 @@ struct TB3DArgs {
   int order;                      // workgroup order: 0 dispatch (y tiles fastest), 1 per XCD, 2 per XCD TX
                                   // fastest, 3 per XCD in groups of tx_group TX, then y tiles
   int tx_group;
+  int b0, nb;                     // the launch steps TX [b0, b0 + nb) of the B (arrays span all B)
   // piecewise-constant media without mu_r / sigma_m maps: one byte per cell
   // (box layout, no batch axis) indexing a table of kMat3Floats floats per
   // material, in the order of a packed record (ca, cb, Debye k1, k2/dt,
@@ -231,6 +232,7 @@ struct AdjTB3DArgs {
   int tiles_y, tiles_z;           // ceil((My-1)/(16-2K)), ceil((Mz-1)/(64-2K))
   int order;                      // 0: dispatch order (y tiles fastest), 1: the same with
                                   // contiguous ranges per XCD, 2: per XCD with the TX fastest
+  int b0, nb;                     // the launch reverses TX [b0, b0 + nb) of the B (arrays span all B)
   const unsigned char* mat_id;    // material ids and table as TB3DArgs (null: packed maps)
   const float* mat_table;
   int n_mat;

@@ -97,8 +97,8 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
   const int L = xcd_block(p.order != 0);
   int ty, tz, bt;
   if (p.order == 2) {   // TX fastest: the TX of one tile share its maps in an L2
-    bt = L % p.B;
-    const int t = L / p.B;
+    bt = L % p.nb;
+    const int t = L / p.nb;
     ty = t % p.tiles_y;
     tz = t / p.tiles_y;
   } else if (p.order == 3) {   // groups of tx_group TX fastest, then y tiles: an XCD
@@ -108,13 +108,14 @@ __global__ __launch_bounds__(1024) void step3d_tb_kernel(TB3DArgs p) {
     const int r2 = r / p.tiles_y;
     tz = r2 % p.tiles_z;
     bt = (r2 / p.tiles_z) * g + L % g;
-    if (bt >= p.B) return;   // padding of the last group (uniform per workgroup)
+    if (bt >= p.nb) return;   // padding of the last group (uniform per workgroup)
   } else {              // y tiles fastest: neighbouring tiles share halo rows
     ty = L % p.tiles_y;
     const int r = L / p.tiles_y;
     tz = r % p.tiles_z;
     bt = r / p.tiles_z;
   }
+  bt += p.b0;
   const int y = ty * (R - 2 * K) - K + w;
   const int z = tz * (64 - 2 * K) - K + lane;
   const bool yz_in = y >= 0 && y < p.My && z >= 0 && z < p.Mz;
@@ -383,9 +384,10 @@ hipError_t launch_step3d_tb(const TB3DArgs& a, int K, int poles, hipStream_t s) 
   if (a.tiles_y != (a.My - 1 + (kRows - 2 * K) - 1) / (kRows - 2 * K) ||
       a.tiles_z != (a.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K))
     return hipErrorInvalidValue;
-  if (a.order == 3 && (a.tx_group < 1 || a.tx_group > a.B)) return hipErrorInvalidValue;
+  if (a.nb < 1 || a.b0 < 0 || a.b0 + a.nb > a.B) return hipErrorInvalidValue;
+  if (a.order == 3 && (a.tx_group < 1 || a.tx_group > a.nb)) return hipErrorInvalidValue;
   const long long nblk = (long long)a.tiles_y * a.tiles_z *
-                         (a.order == 3 ? (a.B + a.tx_group - 1) / a.tx_group * a.tx_group : a.B);
+                         (a.order == 3 ? (a.nb + a.tx_group - 1) / a.tx_group * a.tx_group : a.nb);
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
   const dim3 grd((unsigned)nblk, 1, 1), blk(1024, 1, 1);
   if (!a.mp || (poles != 0 && !a.jp)) return hipErrorInvalidValue;

@@ -231,7 +231,8 @@ struct ceep_fdtd {
   int* tile_lists = nullptr;
   int n_diff_tiles = 0, n_plain_tiles = 0;
   int split_diff = 1;          // tuning key 23: 0 = one difference kernel over every tile
-  int tx_streams = 2;          // tuning key 27: TX groups of the blocked kernels on 1, 2 or 4 streams
+  int tx_streams = 2;          // tuning key 27: TX groups of the 2D blocked kernels on 1, 2 or 4 streams
+  int tx_streams3d = 1;        // tuning key 28: the same for the 3D blocked forward and adjoint
   // extra streams (default priority): xs[0] also serves the difference
   // form's split modes; ev_fork / ev_join[i] fork work onto xs[i] and join it
   hipStream_t xs[3] = {nullptr, nullptr, nullptr};
@@ -965,19 +966,35 @@ int launch_range(ceep_fdtd* h, int start, int step0, int nsteps, float* e_save) 
     t.T = h->T; t.P = h->P;
     t.probe_head = h->probe_head; t.probe_next = h->probe_next; t.traces = h->traces;
     t.order = h->xcd3d;
-    t.tx_group = std::min(h->tx_group3d, h->B);
-    int cur = start;
-    for (int st = step0; st < step_end;) {
-      const int K = std::min(h->tb3d_k, step_end - st);
-      t.in_set = cur;
-      t.step = st;
-      t.tiles_y = (t.My - 1 + (16 - 2 * K) - 1) / (16 - 2 * K);
-      t.tiles_z = (t.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K);
-      t.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * 3 * h->box_floats() : nullptr;
-      CEEP_HIP(ceep::launch_step3d_tb(t, K, poles, s));
-      cur ^= 1;
-      st += K;
+    // TX groups on separate streams (key 28), as in 2D: a launch runs 2.8
+    // rounds of one-per-CU workgroups, and the other group's launch may take
+    // the CUs its last partial round leaves idle (measured slower: the
+    // groups' windows then share each XCD's L2)
+    std::vector<TxGroup> groups = tx_groups(h->B, 1, std::max(1, h->tx_streams3d));
+    const int nx = (int)groups.size() - 1;
+    if (nx > 0) {
+      int rc2 = fork_streams(h, s, nx);
+      if (rc2) return rc2;
     }
+    for (int gi = 0; gi <= nx; ++gi) {
+      t.b0 = groups[gi].b0;
+      t.nb = groups[gi].nb;
+      t.tx_group = std::min(h->tx_group3d, t.nb);
+      const hipStream_t gs = gi ? h->xs[gi - 1] : s;
+      int cur = start;
+      for (int st = step0; st < step_end;) {
+        const int K = std::min(h->tb3d_k, step_end - st);
+        t.in_set = cur;
+        t.step = st;
+        t.tiles_y = (t.My - 1 + (16 - 2 * K) - 1) / (16 - 2 * K);
+        t.tiles_z = (t.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K);
+        t.e_save = e_save ? e_save + (size_t)(st + 1 - step0) * 3 * h->box_floats() : nullptr;
+        CEEP_HIP(ceep::launch_step3d_tb(t, K, poles, gs));
+        cur ^= 1;
+        st += K;
+      }
+    }
+    if (nx > 0) return join_streams(h, s, nx);
   } else {
     ceep::Step3DArgs a{};
     a.Mx = h->M[0]; a.My = h->M[1]; a.Mz = h->M[2]; a.pitch = h->pitch; a.B = h->B;
@@ -2007,6 +2024,8 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
   t.probe_head = h->probe_head; t.probe_next = h->probe_next;
   t.g = h->g3tb; t.gstride = h->plane;
   t.order = h->adj3d_order;
+  t.b0 = 0; t.nb = B;
+  const std::vector<TxGroup> agroups = tx_groups(B, 1, std::max(1, h->tx_streams3d));
   // lam_z^T = dL/dtrace(T-1) at the probes (set 0)
   CEEP_HIP(ceep::launch_adj3d_seed(t.lam[2], h->probe_cell3, h->d_dtr, B, P, T, T - 1, h->plane, st));
   const size_t nstate = state3d(h, 0).size();
@@ -2031,17 +2050,31 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
       h->estore_off = off;
     }
     const float* ebase = h->estore + (size_t)h->estore_off * slot;
-    for (int s = s0 + n - 1; s >= s0;) {
-      const int K = std::min(h->adj3d_k, s - s0 + 1);
-      t.in_set = set;
-      t.step = s;
-      t.e_lo = ebase + (size_t)(s - K + 1 - s0) * slot;
-      t.tiles_y = (t.My - 1 + (16 - 2 * K) - 1) / (16 - 2 * K);
-      t.tiles_z = (t.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K);
-      CEEP_HIP(ceep::launch_adj3d_tb(t, K, h->poles(), st));
-      set ^= 1;
-      s -= K;
+    // the segment's reversed steps as TX groups on separate streams (key 28),
+    // forked after the recomputation and joined before the next one
+    const int nx = (int)agroups.size() - 1;
+    if (nx > 0 && (rc = fork_streams(h, st, nx))) return rc;
+    int set_end = set;
+    for (int gi = 0; gi <= nx; ++gi) {
+      t.b0 = agroups[gi].b0;
+      t.nb = agroups[gi].nb;
+      const hipStream_t gs = gi ? h->xs[gi - 1] : st;
+      int gset = set;
+      for (int s = s0 + n - 1; s >= s0;) {
+        const int K = std::min(h->adj3d_k, s - s0 + 1);
+        t.in_set = gset;
+        t.step = s;
+        t.e_lo = ebase + (size_t)(s - K + 1 - s0) * slot;
+        t.tiles_y = (t.My - 1 + (16 - 2 * K) - 1) / (16 - 2 * K);
+        t.tiles_z = (t.Mz - 1 + (64 - 2 * K) - 1) / (64 - 2 * K);
+        CEEP_HIP(ceep::launch_adj3d_tb(t, K, h->poles(), gs));
+        gset ^= 1;
+        s -= K;
+      }
+      set_end = gset;
     }
+    set = set_end;
+    if (nx > 0 && (rc = join_streams(h, st, nx))) return rc;
   }
   if (B > 1)   // fold the per-TX planes into plane 0 of each (in place: plane 0 is read first)
     for (int c = 0; c < 2; ++c)
@@ -2652,6 +2685,9 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 27) {
     if (value != 1 && value != 2 && value != 4) return fail(CEEP_EINVAL, "TX streams must be 1, 2 or 4");
     h->tx_streams = value;
+  } else if (key == 28) {
+    if (value != 1 && value != 2 && value != 4) return fail(CEEP_EINVAL, "3D TX streams must be 1, 2 or 4");
+    h->tx_streams3d = value;
   } else if (key == 25) {
     if (value < -1) return fail(CEEP_EINVAL, "3D extra E-store slots must be -1 (auto) or >= 0");
     if (value != h->estore_extra) {

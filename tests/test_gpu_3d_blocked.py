@@ -375,3 +375,32 @@ def test_unchanged_materials_are_not_rebuilt(amd):
     s._handle.reset()
     s.run()
     assert np.array_equal(s.probe_data, t0)
+
+
+@pytest.mark.parametrize("streams", [2, 4])
+def test_blocked_3d_tx_groups_on_streams(amd, streams):
+    """The TX split into groups on streams of their own (tuning key 28): the
+    forward's traces and fields and the blocked adjoint's gradients equal one
+    stream's bit for bit (5 TX: groups of unequal size)."""
+    shape, n, T, dx = (26, 40, 70), 3, 37, 1e-3
+    dt = W.dt_3d_courant(dx, 0.5)
+    w = W.modulated_gaussian(T, dt, 20e9, 30e9, delay_factor=3.0)
+    eps, sig, m = _maps(shape, 5)
+    src = [(13, 20, 35), (10, 18, 30), (15, 24, 40), (12, 14, 25), (16, 28, 45)]
+    prb = src + [(12, 30, 20)]
+    out = []
+    for ts in (1, streams):
+        s = _solver(amd, shape, n, T, dx, src, prb, w, eps, sig, m, ("debye", "lorentz"))
+        s.tuning = {"tx_streams_3d": ts}
+        print(f"3D blocked forward, {ts} stream(s)", flush=True)
+        s.run()
+        out.append((s.probe_data.copy(), s.ez, s.hy))
+    assert np.abs(out[0][0]).max() > 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    tun = [{"tx_streams_3d": 1}, {"tx_streams_3d": streams}]
+    res = _gradients(amd, (26, 30, 70), 3, 39, 20, ("debye", "lorentz"), False, tun, seed=4,
+                     src=[(13, 15, 35), (10, 12, 30), (15, 18, 40)])
+    assert res[1][0] == res[0][0]
+    assert np.array_equal(res[1][1], res[0][1]) and np.array_equal(res[1][2], res[0][2])
+    assert np.abs(res[0][1]).max() > 0

@@ -51,6 +51,9 @@ HBM_PEAK_GBS = 8000.0
 # TX) and the shared ca, cb, k1, k2 maps read once per launch (16 B per cell)
 BYTES_PER_CELL_PER_TX = 32
 BYTES_PER_CELL_SHARED = 16
+# 2D reversible adjoint, per reversed cell-step and TX: the forward's E, Hx, Hy
+# and the adjoint's lambda_e, mu_x, mu_y read + written once
+BYTES_ADJ_PER_CELL_PER_TX = 48
 BYTES3D_PER_CELL_PER_TX = 120   # 3D Debye + Lorentz: 15 state floats read + written
 BYTES3D_SHARED = 28             # ca, cb, two Debye and three Lorentz coefficient maps
 
@@ -278,6 +281,28 @@ def _adjoint_ratio(every=50):
     _device_sync()
     t_first = time.perf_counter() - t0
     t_res, _ = timed(lambda: h.run_residual(every))
+    # the reversible adjoint on its own, for its roofline: a residual run and
+    # the adjoint of a fixed dL/dtrace, minus the residual run alone
+    res, _ = h.run_residual(every)
+    r = torch.from_numpy(res).requires_grad_(True)
+    adjoint.s_norm2(r, freqs, dt).backward()
+    g = r.grad.numpy().copy()
+    t_ra, _ = timed(lambda: (h.run_residual(every), h.adjoint(g)))
+    box = h.box_info()
+    cells = int(box[0] * box[1])
+    n_steps, n_tx = s.total_steps, s.batch
+    k_adj, bpw_adj = 6, 4
+    launches = -(-n_steps // every) * -(-every // k_adj)
+    chunks = -(-n_tx // bpw_adj)
+    streams = (s.tuning or {}).get("tx_streams", 2)
+    # algorithmic bytes per launch of K reversed steps: the forward's E, Hx, Hy
+    # and the adjoint's lambda_e, mu_x, mu_y read and written once per cell,
+    # TX and step; the two gradient sums per TX chunk and the two maps once
+    bytes_adj = cells * (n_tx * BYTES_ADJ_PER_CELL_PER_TX * k_adj + chunks * 16 + 8)
+    roof_adj = _roofline(f"adj2d_rev_mw_kernel<8, {k_adj}, {bpw_adj}, false>", k_adj, bytes_adj,
+                         (t_ra - t_res) * 1e3 / launches, "gradient2d_disk_600x600x16_counters.json",
+                         min(streams, chunks), section="two_stream_gradient_run")
+    roof_adj["ms_adjoint_call"] = round((t_ra - t_res) * 1e3, 3)
     t_grad, (loss, ge, gs) = timed(lambda: adjoint.gradient(s, eps, sig, freqs, every=every,
                                                             target=(eps_t, sig_t)))
     t_grad_ag, _ = timed(lambda: adjoint.gradient(s, eps, sig, freqs, every=every,
@@ -294,6 +319,7 @@ def _adjoint_ratio(every=50):
             "ms_target_run_once": round((t_first - t_res) * 1e3, 3),
             "ratio_fp64_forward": round(t_grad64 / t_fwd, 3),
             "ratio_autograd_path": round(t_grad_ag / t_fwd, 3),
+            "roofline_adjoint": roof_adj,
             "note": ("ratio: the FP32 difference form (forward of E(eps) - E(target) with "
                      "checkpoints every 50 steps, reversible adjoint, chain rule on the device) "
                      "in one library call, S-matrix, loss and dL/dtrace on the device; "
@@ -304,7 +330,10 @@ def _adjoint_ratio(every=50):
                      "ms_target_run_once: the target's own run with its E kept at every step, "
                      "taken once per target (like S_target itself) and outside the ratio; "
                      "ratio_fp64_forward: the other accurate variant, FP64 forward fields "
-                     "against an FP64 S_target"),
+                     "against an FP64 S_target; roofline_adjoint: the reversible adjoint "
+                     "kernel, timed as one adjoint call (the residual run, then the adjoint "
+                     "of a fixed dL/dtrace, minus the residual run; the call's chain rule "
+                     "and copies included) over its launches"),
             "workload": ("2D 600x600, 16 TX/RX ring r=200, 800 steps, lossy disk phantom; "
                          "L = sum |S - S_target|^2 at 16 freqs in 1-2.5 GHz against the disk "
                          "of seed 1; dL/deps_r, dL/dsigma")}
@@ -561,7 +590,7 @@ def _bench_3d_gradient(T3=1000, every=100, single=False, tuning=None):
 
 
 def _roofline(kernel, steps, bytes_launch, launch_ms, summary, dispatches=1,
-              launch_ms_one=None):
+              launch_ms_one=None, section=None):
     """Roofline object of one kernel: `achieved` / `frac` from the HBM bytes a
     launch moves by rocprofv3 counter (`traffic`, the committed summary) over
     the live mean launch time; the algorithmic figure (every cell's state read
@@ -572,7 +601,7 @@ def _roofline(kernel, steps, bytes_launch, launch_ms, summary, dispatches=1,
     time per K steps of the whole batch.  `launch_ms_one`: the same K steps
     as one dispatch over the whole batch (`one_dispatch`, with that form's
     measured bytes)."""
-    per_dispatch = _traffic(kernel, summary)
+    per_dispatch = _traffic(kernel, summary, section)
     traffic = per_dispatch * dispatches if per_dispatch else None
     alg = bytes_launch / (launch_ms * 1e-3) / 1e9
     out = {"bound": "hbm", "kernel": kernel, "steps_per_launch": steps,
@@ -608,14 +637,18 @@ def _cpu_model():
     return platform.processor() or platform.machine()
 
 
-def _traffic(kernel, summary="step2d_debye_600x600x16_traffic.json"):
+def _traffic(kernel, summary="step2d_debye_600x600x16_traffic.json", section=None):
     """Measured HBM bytes per launch from the committed rocprofv3 counter
-    summary, if it was taken for the kernel this run uses (else None)."""
+    summary, if it was taken for the kernel this run uses (else None);
+    `section`: a named part of the summary (a later measurement) instead of
+    its top level."""
     path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if section is not None:
+        d = d.get(section, {})
     # summaries taken before the window kernels named their wavefront count
     # list the four-wavefront kernels without it (same code)
     names = [kernel]

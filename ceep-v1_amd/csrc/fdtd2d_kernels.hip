@@ -54,8 +54,8 @@ __global__ __launch_bounds__(256) void step2d_kernel(Step2DArgs p) {
     const float eR = has_r ? ez[1] : 0.f;
     const float eD = has_d ? ez[p.pitch] : 0.f;
     // H half step at this cell (run_cpu form: cells on the last row/column keep 0)
-    const float hxn = has_r ? hx[0] - ch * (eR - e0) : hx[0];
-    const float hyn = has_d ? hy[0] + ch * (eD - e0) : hy[0];
+    const float hxn = has_r ? h_minus(hx[0], ch, eR - e0) : hx[0];
+    const float hyn = has_d ? h_plus(hy[0], ch, eD - e0) : hy[0];
     p.hx_out[o] = hxn;
     p.hy_out[o] = hyn;
 
@@ -64,11 +64,11 @@ __global__ __launch_bounds__(256) void step2d_kernel(Step2DArgs p) {
       // neighbour H values at (a, c-1) and (a-1, c), recomputed bit-identically
       const float eL = ez[-1];
       const float eU = ez[-p.pitch];
-      const float hxl = hx[-1] - ch * (e0 - eL);
-      const float hyu = hy[-p.pitch] + ch * (e0 - eU);
+      const float hxl = h_minus(hx[-1], ch, e0 - eL);
+      const float hyu = h_plus(hy[-p.pitch], ch, e0 - eU);
       const float curl = (hyn - hyu) - (hxn - hxl);
       if (POLES == 0) {
-        en = ca * e0 + cb * (curl * idx);
+        en = e_step(ca, cb, e0, curl, idx);
       } else {
         float jd = 0.f, jl = 0.f, jlp = 0.f, sgj = 0.f;
         if (POLES & 1) {
@@ -80,9 +80,9 @@ __global__ __launch_bounds__(256) void step2d_kernel(Step2DArgs p) {
           jlp = p.jl_prev[o];
           sgj += jl;                                // gamma = 1
         }
-        en = ca * e0 + cb * (curl * idx) - cb * sgj;
+        en = __builtin_fmaf(-cb, sgj, e_step(ca, cb, e0, curl, idx));
         const float de = en - e0;
-        if (POLES & 1) p.jd[o] = dk1 * jd + dk2 * de;
+        if (POLES & 1) p.jd[o] = j_debye(dk1, dk2, jd, de);
         if (POLES & 2) p.jl_prev[o] = lk1 * jl + lk3 * jlp + lk2 * de;  // becomes J_cur
       }
     }

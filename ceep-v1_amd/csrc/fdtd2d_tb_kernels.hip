@@ -131,8 +131,8 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
       for (int r = s; r < R - s - 1; ++r) {
         const float eR = from_next_lane(ez[r]);
         const float eD = (r + 1 < R) ? ez[r + 1] : 0.f;
-        hx[r] = hx[r] - ch * (eR - ez[r]);
-        hy[r] = hy[r] + ch * (eD - ez[r]);
+        hx[r] = h_minus(hx[r], ch, eR - ez[r]);
+        hy[r] = h_plus(hy[r], ch, eD - ez[r]);
       }
       // E step (+ Debye ADE)
 #pragma unroll
@@ -142,11 +142,10 @@ __device__ __forceinline__ void tb_task(const TB2DArgs& p, int task, int lane) {
         const float curl = (hy[r] - hyu) - (hx[r] - hxl);
         const float e0 = ez[r];
         if (POLES == 0) {
-          ez[r] = mca[r] * e0 + mcb[r] * (curl * idx);
+          ez[r] = e_step(mca[r], mcb[r], e0, curl, idx);
         } else {
-          const float sgj = 0.5f * (1.f + mk1[r]) * jd[r];
-          const float en = mca[r] * e0 + mcb[r] * (curl * idx) - mcb[r] * sgj;
-          jd[r] = mk1[r] * jd[r] + mk2[r] * (en - e0);
+          const float en = e_step_debye(mca[r], mcb[r], e0, curl, idx, mk1[r], jd[r]);
+          jd[r] = j_debye(mk1[r], mk2[r], jd[r], en - e0);
           ez[r] = en;
         }
       }
@@ -1106,7 +1105,10 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
 // wavefronts of the calling workgroup (or of its group: the task's two
 // workgroup barriers per step then hold every group); x_e / x_h are its LDS
 // exchange rows.
-template <int RW, int K, int POLES, bool MID, int NW = 4>
+// FULL: the launch takes all K steps (p.ksteps == K).  The step loop then
+// has no early exit, so the unrolled steps need not return the fields to
+// the registers an exit would read them from (two moves per row and step).
+template <int RW, int K, int POLES, bool MID, int NW = 4, bool FULL = false>
 __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, int w,
                                         float (*x_e)[64], float (*x_h)[64]) {
   constexpr int R = NW * RW;
@@ -1239,10 +1241,10 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
 
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-      if (s >= p.ksteps) break;                        // uniform per workgroup
+      if (!FULL && s >= p.ksteps) break;               // uniform per workgroup
       const int step = p.step0 + s;
       float er2[RW];                                   // difference form: E_ref^{step+2}
-      if (dif && s + 1 < p.ksteps) load_ref(er2, step + 2);
+      if (dif && s + 1 < (FULL ? K : p.ksteps)) load_ref(er2, step + 2);
       // H half step: the E row below the last row comes from the wavefront below
       x_e[w][lane] = ez[0];
       __syncthreads();
@@ -1251,13 +1253,19 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
       for (int i = 0; i < RW; ++i) {
         const float eR = from_next_lane(ez[i]);
         const float eD = (i + 1 < RW) ? ez[i + 1] : e_dn;
-        hx[i] = hx[i] - ch * (eR - ez[i]);
-        hy[i] = hy[i] + ch * (eD - ez[i]);
+        hx[i] = h_minus(hx[i], ch, eR - ez[i]);
+        hy[i] = h_plus(hy[i], ch, eD - ez[i]);
       }
       // E step (+ Debye ADE): the new hy above the first row comes from the wavefront above
       x_h[w][lane] = hy[RW - 1];
       __syncthreads();
       const float h_up = has_up ? x_h[w - 1][lane] : 0.f;
+      // the source (not in the difference form, where it cancels): s_row is
+      // uniform, so only that row's update takes the add, on its source lane
+      // (a select over every row after the loop would rename all of them and
+      // cost two moves per row and step)
+      const bool inject = POLES != kPolesDiff && src_here;
+      const float wv = inject ? p.wave[(long long)bt * p.wave_stride_b + step] : 0.f;
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const float hxl = from_prev_lane(hx[i]);
@@ -1265,13 +1273,13 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
         const float curl = (hy[i] - hyu) - (hx[i] - hxl);
         const float e0 = ez[i];
         if (POLES == 0 || POLES == kPolesDiff) {
-          ez[i] = mca[i] * e0 + mcb[i] * (curl * idx);
+          ez[i] = e_step(mca[i], mcb[i], e0, curl, idx);
         } else {
-          const float sgj = 0.5f * (1.f + mk1[i]) * jd[i];
-          const float en = mca[i] * e0 + mcb[i] * (curl * idx) - mcb[i] * sgj;
-          jd[i] = mk1[i] * jd[i] + mk2[i] * (en - e0);
+          const float en = e_step_debye(mca[i], mcb[i], e0, curl, idx, mk1[i], jd[i]);
+          jd[i] = j_debye(mk1[i], mk2[i], jd[i], en - e0);
           ez[i] = en;
         }
+        if (inject && i == s_row && lane == s_lane) ez[i] += wv;
       }
       if (POLES == kPolesDiff) {
         // the source is common to both runs: it cancels in E - E_ref, and
@@ -1281,16 +1289,11 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
 #pragma unroll
           for (int i = 0; i < RW; ++i) {
             const float e1 = (src_here && i == s_row && lane == s_lane) ? er1[i] - wv : er1[i];
-            ez[i] += mk1[i] * er0[i] + mk2[i] * e1;
+            ez[i] += __builtin_fmaf(mk1[i], er0[i], mk2[i] * e1);
             er0[i] = er1[i];
             er1[i] = er2[i];
           }
         }
-      } else if (src_here) {
-        const float wv = p.wave[(long long)bt * p.wave_stride_b + step];
-#pragma unroll
-        for (int i = 0; i < RW; ++i)
-          if (i == s_row && lane == s_lane) ez[i] += wv;
       }
       if (p.e_save) {   // E after this step on the output window (adjoint store)
         const __amdgpu_buffer_rsrc_t r_es =
@@ -1354,14 +1357,14 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
   }
 }
 
-template <int RW, int K, int POLES, bool MID, int NW>
+template <int RW, int K, int POLES, bool MID, int NW, bool FULL>
 __global__ __launch_bounds__(64 * NW) void step2d_mw_kernel(TB2DArgs p) {
   __shared__ float x_e[NW][64], x_h[NW][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int task = xcd_block(p.xcd_order);
   if (task >= p.ntasks) return;                       // uniform per workgroup
-  mw_task<RW, K, POLES, MID, NW>(p, task, lane, w, x_e, x_h);
+  mw_task<RW, K, POLES, MID, NW, FULL>(p, task, lane, w, x_e, x_h);
 }
 
 // Uneven batch, one launch per round, NW = 4 or 8 wavefronts per window:
@@ -1554,17 +1557,27 @@ int persist_mw_blocks_per_cu(int poles, int R, int K, bool mat_ids) {
   return n;
 }
 
-template <int RW, int K, int NW>
-static hipError_t launch_mw_rk(const TB2DArgs& a, int poles, hipStream_t s) {
+template <int RW, int K, int NW, bool FULL>
+static hipError_t launch_mw_rkf(const TB2DArgs& a, int poles, hipStream_t s) {
   const dim3 blk(64 * NW, 1, 1), grd((unsigned)a.ntasks, 1, 1);
   const bool ids = a.mat_id != nullptr;
-  if (poles == 0 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, true, NW>), grd, blk, 0, s, a);
-  else if (poles == 0) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, false, NW>), grd, blk, 0, s, a);
-  else if (poles == 1 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, true, NW>), grd, blk, 0, s, a);
-  else if (poles == 1) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, false, NW>), grd, blk, 0, s, a);
-  else if (poles == kPolesDiff) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, kPolesDiff, false, NW>), grd, blk, 0, s, a);
+  if (poles == 0 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, true, NW, FULL>), grd, blk, 0, s, a);
+  else if (poles == 0) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 0, false, NW, FULL>), grd, blk, 0, s, a);
+  else if (poles == 1 && ids) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, true, NW, FULL>), grd, blk, 0, s, a);
+  else if (poles == 1) hipLaunchKernelGGL((step2d_mw_kernel<RW, K, 1, false, NW, FULL>), grd, blk, 0, s, a);
+  else if (poles == kPolesDiff)
+    hipLaunchKernelGGL((step2d_mw_kernel<RW, K, kPolesDiff, false, NW, FULL>), grd, blk, 0, s, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+// the launches of all K steps without the step loop's early exit; a run's
+// last, shorter launch with it
+template <int RW, int K, int NW>
+static hipError_t launch_mw_rk(const TB2DArgs& a, int poles, hipStream_t s) {
+  if (a.ksteps < 1 || a.ksteps > K) return hipErrorInvalidValue;
+  return a.ksteps == K ? launch_mw_rkf<RW, K, NW, true>(a, poles, s)
+                       : launch_mw_rkf<RW, K, NW, false>(a, poles, s);
 }
 
 template <int RW>

@@ -78,6 +78,32 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// The 2D TMz cell updates with their roundings fixed: every 2D forward kernel
+// (one step per launch, one- and multi-wavefront windows, uneven rounds)
+// calls these, so their results are bit-identical whatever the compiler
+// would contract on its own in each instantiation.
+//   H: h - ch d (hx), h + ch d (hy), each one fused multiply-add
+__device__ __forceinline__ float h_minus(float h, float ch, float d) {
+  return __builtin_fmaf(-ch, d, h);
+}
+__device__ __forceinline__ float h_plus(float h, float ch, float d) {
+  return __builtin_fmaf(ch, d, h);
+}
+//   E: ca e0 + cb (curl idx)
+__device__ __forceinline__ float e_step(float ca, float cb, float e0, float curl, float idx) {
+  return __builtin_fmaf(ca, e0, cb * (curl * idx));
+}
+//   E with the Debye current: ca e0 + cb (curl idx) - cb sgj, sgj = (1 + k1)/2 J
+__device__ __forceinline__ float e_step_debye(float ca, float cb, float e0, float curl, float idx,
+                                              float k1, float jd) {
+  const float sgj = 0.5f * (1.f + k1) * jd;
+  return __builtin_fmaf(-cb, sgj, __builtin_fmaf(ca, e0, cb * (curl * idx)));
+}
+//   Debye current: k1 J + k2dt (e_new - e_old)
+__device__ __forceinline__ float j_debye(float k1, float k2dt, float jd, float de) {
+  return __builtin_fmaf(k1, jd, k2dt * de);
+}
+
 template <int V>
 struct Phase {
   static constexpr int value = V;

@@ -752,7 +752,9 @@ def main():
     if K and nw in (4, 8):
         # the head phantom is piecewise constant: material ids unless switched off
         ids = "true" if (tuning or {}).get("material_ids", 1) else "false"
-        kernel = f"step2d_mw_kernel<{tuning['rows_per_wave'] // nw}, {K}, 1, {ids}, {nw}>"
+        # the full-depth form (no early exit in the step loop): T is a multiple of K
+        full = "true" if T % K == 0 else "false"
+        kernel = f"step2d_mw_kernel<{tuning['rows_per_wave'] // nw}, {K}, 1, {ids}, {nw}, {full}>"
         launches = -(-T // K)
         steps_per_launch = K
     elif K:

@@ -67,7 +67,7 @@ def candidates_2d(lorentz=False):
                         "batch_per_wave": bpw})
         # eight wavefronts of 8 (6) rows: 64- (48-) row windows at the register
         # footprint of the four-wavefront 32-row form
-        for k, r, bpw in ((6, 64, 2), (6, 64, 4), (8, 64, 2), (6, 48, 2)):
+        for k, r, bpw in ((6, 64, 2), (6, 64, 4), (8, 64, 1), (8, 64, 2), (8, 64, 4), (6, 48, 2)):
             out.append({"forward_waves": 8, "temporal_depth": k, "rows_per_wave": r,
                         "batch_per_wave": bpw})
         # the TX groups on one stream instead of two
@@ -116,7 +116,7 @@ def autotune_uneven(solvers, reps: int = 1, depths=(1, 2, 3, 4), rows=(16, 24, 3
                     batch_per_wave=(1, 2, 4), start=(4, 16, 2),
                     four_wave=((4, 32, 2), (4, 32, 4), (6, 32, 2), (6, 32, 4), (4, 48, 2),
                                (6, 48, 2), (6, 48, 4)),
-                    eight_wave=((6, 64, 2), (6, 64, 4), (8, 64, 2), (8, 64, 4)),
+                    eight_wave=((6, 64, 2), (6, 64, 4), (8, 64, 2), (8, 64, 4), (8, 64, 8)),
                     verbose: bool = False):
     """Launch shape (temporal depth K, window rows R, TX planes per window,
     wavefronts per window) for an ``UnevenBatch`` of ``solvers``: coordinate

@@ -422,7 +422,7 @@ def _bench_uneven(n_problems=64, seed=4, autotune=True, single=False, persistent
 
     solvers, grids = _uneven_solvers(n_problems, seed)
     # K, window rows, TX planes per window (per item), wavefronts: the tuners' picks
-    shape, log = (8, 64, 8, 8), []
+    shape, log = (8, 64, 4, 8), []
     shape_p, log_p = (6, 48, 2, 4), []
     if autotune:
         shape, log = autotune_uneven(solvers)
@@ -687,8 +687,8 @@ def main():
                     help="one 2D gradient call on the gradient workload (600^2 disk, 16 TX, 800 steps, "
                          "checkpoints every 50), no output (for rocprofv3 --pmc); --tuning applies")
     ap.add_argument("--single-uneven", action="store_true",
-                    help="one uneven batch of the 64 heads with the tuner's usual shape (K 8, 64 "
-                         "rows, 8 TX per window, eight wavefronts), no output (for rocprofv3 --pmc)")
+                    help="one uneven batch of the 64 heads with K 8, 64 rows, 4 TX per window, "
+                         "eight wavefronts (the tuner picks 4 or 8 TX), no output (for rocprofv3 --pmc)")
     ap.add_argument("--uneven-persistent", action="store_true",
                     help="with --single-uneven: the persistent launch instead of per-round launches")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -155,6 +155,7 @@ TUNE_ADJ_PREFETCH = 24
 TUNE_ESTORE_EXTRA_3D = 25
 TUNE_TX_STREAMS = 27
 TUNE_TX_STREAMS_3D = 28
+TUNE_TX_STREAMS_3D_ADJ = 29
 TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "temporal_depth": TUNE_TEMPORAL_DEPTH, "rows_per_wave": TUNE_ROWS_PER_WAVE,
                "batch_per_wave": TUNE_BATCH_PER_WAVE, "fp64_forward": TUNE_FP64_FORWARD,
@@ -170,7 +171,8 @@ TUNING_KEYS = {"batch_per_thread": TUNE_BATCH_PER_THREAD, "graph": TUNE_GRAPH,
                "split_difference": TUNE_SPLIT_DIFF,
                "adjoint_prefetch": TUNE_ADJ_PREFETCH,
                "estore_extra_3d": TUNE_ESTORE_EXTRA_3D,
-               "tx_streams": TUNE_TX_STREAMS, "tx_streams_3d": TUNE_TX_STREAMS_3D}
+               "tx_streams": TUNE_TX_STREAMS, "tx_streams_3d": TUNE_TX_STREAMS_3D,
+               "tx_streams_3d_adjoint": TUNE_TX_STREAMS_3D_ADJ}
 
 
 class Handle:

@@ -232,7 +232,8 @@ struct ceep_fdtd {
   int n_diff_tiles = 0, n_plain_tiles = 0;
   int split_diff = 1;          // tuning key 23: 0 = one difference kernel over every tile
   int tx_streams = 2;          // tuning key 27: TX groups of the 2D blocked kernels on 1, 2 or 4 streams
-  int tx_streams3d = 1;        // tuning key 28: the same for the 3D blocked forward and adjoint
+  int tx_streams3d = 1;        // tuning key 28: the same for the 3D blocked forward
+  int tx_streams3d_adj = 1;    // tuning key 29: and for the 3D blocked adjoint
   // extra streams (default priority): xs[0] also serves the difference
   // form's split modes; ev_fork / ev_join[i] fork work onto xs[i] and join it
   hipStream_t xs[3] = {nullptr, nullptr, nullptr};
@@ -2025,7 +2026,7 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
   t.g = h->g3tb; t.gstride = h->plane;
   t.order = h->adj3d_order;
   t.b0 = 0; t.nb = B;
-  const std::vector<TxGroup> agroups = tx_groups(B, 1, std::max(1, h->tx_streams3d));
+  const std::vector<TxGroup> agroups = tx_groups(B, 1, std::max(1, h->tx_streams3d_adj));
   // lam_z^T = dL/dtrace(T-1) at the probes (set 0)
   CEEP_HIP(ceep::launch_adj3d_seed(t.lam[2], h->probe_cell3, h->d_dtr, B, P, T, T - 1, h->plane, st));
   const size_t nstate = state3d(h, 0).size();
@@ -2050,7 +2051,7 @@ int adjoint3d_blocked(ceep_fdtd* h, const std::vector<float>& dtr, std::vector<f
       h->estore_off = off;
     }
     const float* ebase = h->estore + (size_t)h->estore_off * slot;
-    // the segment's reversed steps as TX groups on separate streams (key 28),
+    // the segment's reversed steps as TX groups on separate streams (key 29),
     // forked after the recomputation and joined before the next one
     const int nx = (int)agroups.size() - 1;
     if (nx > 0 && (rc = fork_streams(h, st, nx))) return rc;
@@ -2688,6 +2689,10 @@ int ceep_fdtd_set_tuning(ceep_fdtd* h, int key, int value) {
   } else if (key == 28) {
     if (value != 1 && value != 2 && value != 4) return fail(CEEP_EINVAL, "3D TX streams must be 1, 2 or 4");
     h->tx_streams3d = value;
+  } else if (key == 29) {
+    if (value != 1 && value != 2 && value != 4)
+      return fail(CEEP_EINVAL, "3D adjoint TX streams must be 1, 2 or 4");
+    h->tx_streams3d_adj = value;
   } else if (key == 25) {
     if (value < -1) return fail(CEEP_EINVAL, "3D extra E-store slots must be -1 (auto) or >= 0");
     if (value != h->estore_extra) {

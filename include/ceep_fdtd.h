@@ -296,10 +296,12 @@ int ceep_fdtd_store_info(ceep_fdtd* h, int64_t* out);
  *          once before a run (or a checkpoint segment) and join after it, so
  *          one group's launches fill the partly idle last rounds of the
  *          others'.  Results are bit-identical.
- *   key 28: the same for the 3D blocked forward and adjoint, groups of
- *          whole TX: 1 (default), 2 or 4.  Bit-identical; measured slower
- *          for the forward (200^3 x 16 TX: 2130 against 2055 ms per run) and
- *          no faster for the gradient.
+ *   key 28: the same for the 3D blocked forward, groups of whole TX: 1
+ *          (default), 2 or 4.  Bit-identical; measured slower (200^3 x 16
+ *          TX: 2130 against 2055 ms per run).
+ *   key 29: the same for the 3D blocked adjoint: 1 (default), 2 or 4.
+ *          Bit-identical; measured slower (the 200^3 x 16 TX gradient:
+ *          6.61 and 6.49 s on 2 and 4 streams against 6.24 s on one).
  *   key 25: 3D gradient (blocked adjoint): extra E-store slots beyond one
  *          segment.  The checkpointed forward also keeps E of the last
  *          `extra` steps of the segment before the last, so that segment's

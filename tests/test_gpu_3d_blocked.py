@@ -379,7 +379,7 @@ def test_unchanged_materials_are_not_rebuilt(amd):
 
 @pytest.mark.parametrize("streams", [2, 4])
 def test_blocked_3d_tx_groups_on_streams(amd, streams):
-    """The TX split into groups on streams of their own (tuning key 28): the
+    """The TX split into groups on streams of their own (tuning keys 28, 29): the
     forward's traces and fields and the blocked adjoint's gradients equal one
     stream's bit for bit (5 TX: groups of unequal size)."""
     shape, n, T, dx = (26, 40, 70), 3, 37, 1e-3
@@ -398,7 +398,7 @@ def test_blocked_3d_tx_groups_on_streams(amd, streams):
     assert np.abs(out[0][0]).max() > 0
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
-    tun = [{"tx_streams_3d": 1}, {"tx_streams_3d": streams}]
+    tun = [{"tx_streams_3d": 1}, {"tx_streams_3d": streams, "tx_streams_3d_adjoint": streams}]
     res = _gradients(amd, (26, 30, 70), 3, 39, 20, ("debye", "lorentz"), False, tun, seed=4,
                      src=[(13, 15, 35), (10, 12, 30), (15, 18, 40)])
     assert res[1][0] == res[0][0]

@@ -1160,6 +1160,10 @@ int run_device(ceep_fdtd* h) {
     rc = launch_steps(h, start);
   } else {
     if (!h->gexec[start]) {
+      // the TX groups' extra streams and events exist before the capture
+      // starts: no HIP object is created while the stream captures
+      const int nx = std::max(h->tx_streams, h->tx_streams3d) - 1;
+      if (nx > 0 && (rc = ensure_streams(h, nx))) return rc;
       hipGraph_t g = nullptr;
       CEEP_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       rc = launch_steps(h, start);

@@ -9,12 +9,16 @@ the running DFT of all 256 traces at 301 frequencies over 0.5-3 GHz into the
 (301, 16, 16) S-matrix.  Inputs are resident on the device before timing.
 
 Prints one JSON line.  Its `roofline` is for the dominant kernel: `traffic`
-is the HBM bytes one launch moves by rocprofv3 counter (the committed summary
-under profiles/), `achieved` and `frac` that over the launch's mean HIP-event
-time; `achieved_algorithmic` / `frac_algorithmic` count every stepped cell's
-fields and ADE current read and written once per TX plus its maps once, times
-the steps per launch -- the temporal blocking moves less than that, so that
-figure exceeds the peak.
+is the HBM bytes one launch (K steps of the whole batch) moves by rocprofv3
+counter (the committed summary under profiles/; with the TX in groups on
+separate streams a launch is `dispatches_per_launch` dispatches, each counted),
+`achieved` and `frac` that over the launch's mean HIP-event time;
+`one_dispatch` the same K steps as a single dispatch over the whole batch;
+`achieved_algorithmic` / `frac_algorithmic` count every stepped cell's fields
+and ADE current read and written once per TX plus its maps once, times the
+steps per launch -- the temporal blocking moves less than that, so that figure
+exceeds the peak.  The `adjoint` leg carries the reversible adjoint kernel's
+roofline the same way (`roofline_adjoint`).
 """
 from __future__ import annotations
 

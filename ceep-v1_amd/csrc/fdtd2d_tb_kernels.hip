@@ -1104,7 +1104,12 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
 // One task (output tile, TX chunk) advanced p.ksteps steps by the NW
 // wavefronts of the calling workgroup (or of its group: the task's two
 // workgroup barriers per step then hold every group); x_e / x_h are its LDS
-// exchange rows.
+// exchange rows, NW + 1 of each: wavefront w writes x_e[w] and x_h[w + 1]
+// and reads x_e[w + 1] and x_h[w], so row NW of x_e and row 0 of x_h stand
+// for the zero rows beyond the window's edges.  The caller zeroes those two
+// rows once, before the task's first barrier; no task writes them.  The
+// reads then need no branch, and each lane shift stays in the block of the
+// subtraction that uses it, which folds the two into one instruction.
 // FULL: the launch takes all K steps (p.ksteps == K).  The step loop then
 // has no early exit, so the unrolled steps need not return the fields to
 // the registers an exit would read them from (two moves per row and step).
@@ -1130,7 +1135,6 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
     return (a >= 0 && a < p.Mx) ? lane_off + (unsigned)i * pitch4 : kOffNone;
   };
   auto is_out = [&](int i) -> bool { return r0 + i >= K && r0 + i < R - K; };
-  const bool has_up = w > 0, has_down = w < NW - 1;
 
   float mca[RW], mcb[RW], mk1[RW], mk2[RW];
   if (MID) {
@@ -1248,7 +1252,7 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
       // H half step: the E row below the last row comes from the wavefront below
       x_e[w][lane] = ez[0];
       __syncthreads();
-      const float e_dn = has_down ? x_e[w + 1][lane] : 0.f;
+      const float e_dn = x_e[w + 1][lane];             // row NW: zeros
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const float eR = from_next_lane(ez[i]);
@@ -1257,9 +1261,9 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
         hy[i] = h_plus(hy[i], ch, eD - ez[i]);
       }
       // E step (+ Debye ADE): the new hy above the first row comes from the wavefront above
-      x_h[w][lane] = hy[RW - 1];
+      x_h[w + 1][lane] = hy[RW - 1];
       __syncthreads();
-      const float h_up = has_up ? x_h[w - 1][lane] : 0.f;
+      const float h_up = x_h[w][lane];                 // row 0: zeros
       // the source (not in the difference form, where it cancels): s_row is
       // uniform, so only that row's update takes the add, on its source lane
       // (a select over every row after the loop would rename all of them and
@@ -1359,11 +1363,15 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
 
 template <int RW, int K, int POLES, bool MID, int NW, bool FULL>
 __global__ __launch_bounds__(64 * NW) void step2d_mw_kernel(TB2DArgs p) {
-  __shared__ float x_e[NW][64], x_h[NW][64];
+  __shared__ float x_e[NW + 1][64], x_h[NW + 1][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int task = xcd_block(p.xcd_order);
   if (task >= p.ntasks) return;                       // uniform per workgroup
+  if (w == 0) {                                       // the edge rows (see mw_task)
+    x_e[NW][lane] = 0.f;
+    x_h[0][lane] = 0.f;
+  }
   mw_task<RW, K, POLES, MID, NW, FULL>(p, task, lane, w, x_e, x_h);
 }
 
@@ -1383,11 +1391,15 @@ constexpr int round_mw_min_waves() {
 template <int RW, int K, int POLES, bool MID, int NW = 4>
 __global__ __launch_bounds__(64 * NW, (round_mw_min_waves<NW>())) void round_mw_kernel(
     MultiArgs m, int seg0, int seg1, long long first, long long count) {
-  __shared__ float x_e[NW][64], x_h[NW][64];
+  __shared__ float x_e[NW + 1][64], x_h[NW + 1][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long wi = xcd_block(m.xcd_order);
   if (wi >= count) return;                            // uniform per workgroup
+  if (w == 0) {                                       // the edge rows (see mw_task)
+    x_e[NW][lane] = 0.f;
+    x_h[0][lane] = 0.f;
+  }
   const long long i = first + wi;
   int lo = seg0, hi = seg1 - 1;
   while (lo < hi) {
@@ -1465,11 +1477,15 @@ hipError_t launch_round_mw(const MultiArgs& m, int poles, int R, int K, int NW, 
 // every wavefront meets the same barriers.
 template <int RW, int K, int POLES, bool MID, int G>
 __global__ __launch_bounds__(256 * G) void persist_mw_kernel(PersistArgs m) {
-  __shared__ float x_e[G][4][64], x_h[G][4][64];   // per group: the task's exchange rows
+  __shared__ float x_e[G][5][64], x_h[G][5][64];   // per group: the task's exchange rows
   __shared__ int item_s;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = wave >> 2, w = wave & 3;
+  if (w == 0) {                                       // each group's edge rows (see mw_task)
+    x_e[g][4][lane] = 0.f;
+    x_h[g][0][lane] = 0.f;
+  }
   for (;;) {
     if (threadIdx.x == 0) item_s = (int)atomicAdd(m.queue, 1u);
     __syncthreads();

@@ -777,11 +777,21 @@ __global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_r
     AdjTB2DArgs p) {
   constexpr int R = NW * RW;
   constexpr int NO = R - 2 * K;
-  __shared__ float x_fy[NW][64], x_es[NW][64], x_q[NW][64], x_my[NW][64];
+  // exchange rows with one zero row beyond the window's edge, as in mw_task:
+  // rows read from the wavefront above (x_fy, x_my) are written at w + 1 and
+  // read at w (row 0 zero), rows from the one below (x_es, x_q) written at w
+  // and read at w + 1 (row NW zero), so the edge wavefronts read no branch
+  __shared__ float x_fy[NW + 1][64], x_es[NW + 1][64], x_q[NW + 1][64], x_my[NW + 1][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int task = xcd_block(p.xcd_order);
   if (task >= p.ntasks) return;                       // uniform per workgroup
+  if (w == 0) {                                       // before the first barrier
+    x_fy[0][lane] = 0.f;
+    x_my[0][lane] = 0.f;
+    x_es[NW][lane] = 0.f;
+    x_q[NW][lane] = 0.f;
+  }
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = task % tiles;
   const int chunk = task / tiles;
@@ -803,7 +813,6 @@ __global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_r
   auto is_out = [&](int i) -> bool {                    // window row inside [K, R - K)
     return r0 + i >= K && r0 + i < R - K;
   };
-  const bool has_up = w > 0, has_down = w < NW - 1;
 
   float mca[RW], mica[RW], mcbi[RW];
   {
@@ -892,9 +901,9 @@ __global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_r
     // forward E one step back needs the forward hy of the row above this
     // wavefront's first row, from the wavefront above: for the first reversed
     // step here, for later ones with mu_y at the step before's second barrier
-    x_fy[w][lane] = fy[RW - 1];
+    x_fy[w + 1][lane] = fy[RW - 1];
     lds_barrier();
-    float fy_up = has_up ? x_fy[w - 1][lane] : 0.f;
+    float fy_up = x_fy[w][lane];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (k >= p.ksteps) break;                        // uniform per workgroup
@@ -921,8 +930,8 @@ __global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_r
       x_es[w][lane] = fe[0];
       x_q[w][lane] = mcbi[0] * le[0];
       lds_barrier();
-      const float es_dn = has_down ? x_es[w + 1][lane] : 0.f;
-      const float q_dn = has_down ? x_q[w + 1][lane] : 0.f;
+      const float es_dn = x_es[w + 1][lane];
+      const float q_dn = x_q[w + 1][lane];
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const float eR = from_next_lane(fe[i]);
@@ -942,11 +951,11 @@ __global__ __launch_bounds__(64 * NW, (adj_mw_min_waves<RW, NW>())) void adj2d_r
       }
       // adjoint E step: the new mu_y above comes from the wavefront above,
       // and with it the forward hy the next reversed step starts from
-      x_my[w][lane] = my[RW - 1];
-      if (more) x_fy[w][lane] = fy[RW - 1];
+      x_my[w + 1][lane] = my[RW - 1];
+      if (more) x_fy[w + 1][lane] = fy[RW - 1];
       lds_barrier();
-      const float my_up = has_up ? x_my[w - 1][lane] : 0.f;
-      if (more) fy_up = has_up ? x_fy[w - 1][lane] : 0.f;
+      const float my_up = x_my[w][lane];
+      if (more) fy_up = x_fy[w][lane];
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const float mxL = from_prev_lane(mx[i]);

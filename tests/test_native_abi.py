@@ -22,6 +22,44 @@ def test_header_and_binding_agree():
     assert _header_functions() == sorted(_native.SIGNATURES)
 
 
+def _header_prototypes():
+    with open(os.path.join(ROOT, "include", "ceep_fdtd.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"(int|const char\s*\*)\s+(ceep_[a-z0-9_]+)\(([^;]*?)\);", text, re.S):
+        args = " ".join(m.group(3).split())
+        out[m.group(2)] = (m.group(1), [] if args in ("", "void") else [a.strip() for a in args.split(",")])
+    return out
+
+
+def test_binding_types_match_the_prototypes():
+    """Every ctypes signature has the C prototype's argument count and, per
+    argument, its width: int -> c_int, double -> c_double, double* and
+    int64_t* -> typed pointers, other pointers -> pointers; the return type
+    int -> c_int."""
+    protos = _header_prototypes()
+    assert sorted(protos) == _header_functions()
+    for name, (ret, args) in protos.items():
+        restype, argtypes = _native.SIGNATURES[name]
+        assert (restype is ctypes.c_int) == (ret == "int"), name
+        assert len(argtypes) == len(args), name
+        for ct, a in zip(argtypes, args):
+            a = a.replace("const ", "")
+            if "*" in a:
+                base = a.split("*")[0].strip()
+                assert ct not in (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_float), (name, a)
+                if base == "double":
+                    assert ct is _native._PD, (name, a)
+                elif base == "int64_t":
+                    assert ct is _native._PI64, (name, a)
+            else:
+                t = a.rsplit(" ", 1)[0].strip()
+                want = {"int": ctypes.c_int, "double": ctypes.c_double,
+                        "long long": ctypes.c_longlong}[t]
+                assert ct is want, (name, a)
+
+
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(_native.lib_path())
     for name in _header_functions():

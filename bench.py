@@ -23,11 +23,16 @@ roofline the same way (`roofline_adjoint`).
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import platform
 import sys
 import time
+
+# a fatal signal (an abort from a heap check, a segmentation fault) prints the
+# Python stack of every thread to stderr before the process ends
+faulthandler.enable()
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "ceep-v1_amd")):

@@ -746,8 +746,10 @@ def main():
     s, ph, pos, w, dt = _workload()
     tuning = None
     if not args.no_autotune:
-        from ceep_amd.autotune import autotune_2d
-        tuning = autotune_2d(s, reps=2)
+        from ceep_amd.autotune import autotune_2d, candidates_2d
+        # the one-step kernel and the multi-wavefront windows (the one-wavefront
+        # windows, never the fastest here, are left to autotune_2d's full list)
+        tuning = autotune_2d(s, reps=2, candidates=candidates_2d(one_wavefront=False))
         _progress(f"autotuned 2D launch: {tuning}")
     else:
         from ceep_amd.autotune import tuning_for

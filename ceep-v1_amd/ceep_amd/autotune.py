@@ -53,13 +53,17 @@ def tuning_for(ndim, shape, n, batch, debye=False, lorentz=False) -> dict:
     return dict(DEFAULT_2D) if ndim == 2 else {"batch_per_thread": 4, "graph": 1}
 
 
-def candidates_2d(lorentz=False):
+def candidates_2d(lorentz=False, one_wavefront=True):
+    """Launch shapes autotune_2d tries: the one-step kernel, then (without
+    Lorentz poles) the blocked windows of one wavefront (`one_wavefront`;
+    none has been the fastest on MI355X), four and eight wavefronts."""
     out = [{"temporal_depth": 0, "batch_per_thread": bb} for bb in (2, 4, 8)]
-    if not lorentz:
+    if not lorentz and one_wavefront:
         for k, r, bpw in itertools.product((1, 2, 3, 4), (16, 24, 32), (1, 2, 4)):
             if r - 2 * k >= 4:
                 out.append({"forward_waves": 1, "temporal_depth": k, "rows_per_wave": r,
                             "batch_per_wave": bpw})
+    if not lorentz:
         # four wavefronts per window (taller windows, deeper blocking)
         for k, r, bpw in ((4, 32, 1), (4, 32, 2), (4, 32, 4), (6, 32, 2), (4, 48, 2), (6, 48, 2),
                           (8, 64, 2)):

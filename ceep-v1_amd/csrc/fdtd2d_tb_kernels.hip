@@ -1122,7 +1122,7 @@ hipError_t launch_adj2d_tb(const AdjTB2DArgs& a, int R, int K, hipStream_t s) {
 // FULL: the launch takes all K steps (p.ksteps == K).  The step loop then
 // has no early exit, so the unrolled steps need not return the fields to
 // the registers an exit would read them from (two moves per row and step).
-template <int RW, int K, int POLES, bool MID, int NW = 4, bool FULL = false>
+template <int RW, int K, int POLES, bool MID, int NW = 4, bool FULL = false, bool WPRE = false>
 __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, int w,
                                         float (*x_e)[64], float (*x_h)[64]) {
   constexpr int R = NW * RW;
@@ -1251,6 +1251,23 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
     const int s_row = sc / p.pitch - (a0 + r0);        // local row
     const int s_lane = sc % p.pitch - cb0;
     const bool src_here = !p.no_inject && s_row >= 0 && s_row < RW && s_lane >= 0 && s_lane < 64;
+    // WPRE: the launch's K waveform samples are read up front and
+    // unconditionally, as uniform scalar loads (every run holds T samples per
+    // TX; the index is clamped for the steps past T of a short last launch,
+    // never used).  The sample load under `inject` inside the step splits the
+    // block, and the H update's lane shifts, left in the block before the
+    // split, then stay separate DPP moves (one per row and step) instead of
+    // folding into their subtractions.  Measured on MI355X: the uneven
+    // batch's round kernel gains 8 % with WPRE (678 -> 736 GCell-steps/s),
+    // while the uniform-batch kernel loses 4 % (842 -> 808), so only the
+    // round kernel takes it.
+    typedef __attribute__((address_space(4))) const float cfloat;
+    cfloat* wvp = (cfloat*)(p.wave + (long long)bt * p.wave_stride_b);
+    float wsrc[K];
+    if (WPRE) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) wsrc[k] = wvp[FULL ? p.step0 + k : min(p.step0 + k, p.T - 1)];
+    }
 
 #pragma unroll
     for (int s = 0; s < K; ++s) {
@@ -1278,7 +1295,8 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
       // (a select over every row after the loop would rename all of them and
       // cost two moves per row and step)
       const bool inject = POLES != kPolesDiff && src_here;
-      const float wv = inject ? p.wave[(long long)bt * p.wave_stride_b + step] : 0.f;
+      const float wv = !inject ? 0.f
+                       : WPRE ? wsrc[s] : p.wave[(long long)bt * p.wave_stride_b + step];
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const float hxl = from_prev_lane(hx[i]);
@@ -1298,7 +1316,8 @@ __device__ __forceinline__ void mw_task(const TB2DArgs& p, int task, int lane, i
         // the source is common to both runs: it cancels in E - E_ref, and
         // E_ref^{s+1} - source^s is the reference's ca_r E_ref^s + cb_r curl
         if (dif) {
-          const float wv = src_here ? p.wave[(long long)bt * p.wave_stride_b + step] : 0.f;
+          const float wv = !src_here ? 0.f
+                           : WPRE ? wsrc[s] : p.wave[(long long)bt * p.wave_stride_b + step];
 #pragma unroll
           for (int i = 0; i < RW; ++i) {
             const float e1 = (src_here && i == s_row && lane == s_lane) ? er1[i] - wv : er1[i];
@@ -1429,7 +1448,7 @@ __global__ __launch_bounds__(64 * NW, (round_mw_min_waves<NW>())) void round_mw_
   a.e_save = nullptr;
   // every problem of the batch is or is not given material ids (the host
   // sets them for all or none)
-  mw_task<RW, K, POLES, MID, NW>(a, task, lane, w, x_e, x_h);
+  mw_task<RW, K, POLES, MID, NW, false, true>(a, task, lane, w, x_e, x_h);
 }
 
 template <int RW, int K, bool MID, int NW = 4>
